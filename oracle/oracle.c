@@ -1,0 +1,1322 @@
+/*
+ * oracle.c — FP64 CPU restatement of rev-sim's reversible-simulation gradient loop for flexible
+ * 3-site water.
+ *
+ * TEST INFRASTRUCTURE ONLY. This file is the checker for librevsim.so: only the test suite, the
+ * smoke check and the CPU-baseline leg of the benchmark load it (through oracle/oracle.py). The
+ * product library never links, loads or calls it.
+ *
+ * What each function restates (greener-group/rev-sim):
+ *   or_forces           calc_F!                      water/water.jl:877-975 (LJ force :160-170)
+ *   or_adjoint_terms    calc_F_and_grads!            water/water.jl:1212-1433 (LJ grads :172-193),
+ *                       weighted dF/dθ sum           water/water.jl:1517-1519
+ *   or_noise            generate_noise! semantics    water/water.jl:867-873 (Philox4x64-10 stream)
+ *   or_forward          forward_sim loop             water/water.jl:1029-1045 (e, s at :1019-1020)
+ *   or_reverse          reverse_sim (accumulators)   water/water.jl:1435-1558; additive injection of
+ *                       intermediate-frame gradients (diffusion/diffusion.jl:1534-1539 reset form)
+ *   or_rdf              rdf_differentiable + rrule   water/water.jl:631-736 (binning per config)
+ *   or_msd              calculate_D + loss_fn_final  diffusion/diffusion.jl:861-911
+ *   or_value_and_grad_stored
+ *                       independent check: textbook (λx, λv) adjoint over the stored forward
+ *                       trajectory — plays the role of the DMS gradient (water/water.jl:1052-1099)
+ *
+ * Parity pins (tests/): energy finite differences for forces, force finite differences for the
+ * Hessian-vector product and the θ-VJP, numpy Philox fixtures for the noise bits, brute-force and
+ * closed-form RDF/MSD cases, reversibility, and central finite differences of the whole
+ * trajectory loss for dL/dθ.
+ *
+ * Pair membership (which pairs interact) is defined exactly so that the GPU library and this
+ * oracle agree bit for bit:
+ *   FP64 rule  dx = x_b - x_a; dx -= L*rint(dx*(1/L)); r2 = fma(dz,dz,fma(dy,dy,dx*dx)); r2 <= rc*rc
+ *   FP32 rule  q = (uint32)llrint(x * 2^32/L) per dimension; dxi = (int32)(q_b - q_a);
+ *              dxf = (float)dxi * (float)(L/2^32); r2f = fmaf(dzf,dzf,fmaf(dyf,dyf,dxf*dxf));
+ *              r2f <= (float)(rc*rc)
+ * Compile with -ffp-contract=off (oracle/Makefile) so these expressions are evaluated as written.
+ */
+#include "../include/revsim.h"
+
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define NTH 6 /* theta length */
+
+typedef struct {
+  int32_t step;
+  double* x;
+  double* v;
+} snap_t;
+
+typedef struct or_ctx {
+  rs_config cfg;
+  int M, N, N3;
+  double L[3], invL[3], qscale[3];
+  float unitf[3];
+  double theta[NTH];
+  double krf, rc2;
+  float rc2f;
+  double* mass3; /* per degree of freedom */
+  /* atom-pair half list (a<b, different molecules), FP64 mic distance <= rlist */
+  double rlist;
+  int32_t* pa;
+  int32_t* pb;
+  int64_t npairs, cap;
+  int64_t* cnt;
+  /* loss */
+  double* target;
+  /* external noise */
+  double* ext;
+  int ext_steps;
+  /* forward records for the reverse pass */
+  snap_t* snaps;
+  int nsnaps, capsnaps;
+  double* frames; /* msd frames: nf * 3M */
+  int nframes;
+  int32_t rec_steps;
+  uint64_t rec_seed;
+  char err[256];
+} or_ctx;
+
+static void set_err(or_ctx* o, const char* m) {
+  if (o) snprintf(o->err, sizeof(o->err), "%s", m);
+}
+
+const char* or_last_error(const or_ctx* o) { return o ? o->err : "null context"; }
+
+/* ---------------------------------------------------------------- geometry helpers */
+
+static inline double mic1(double d, double L, double invL) { return d - L * rint(d * invL); }
+
+static inline void mic3(const or_ctx* o, const double* xa, const double* xb, double dr[3]) {
+  for (int k = 0; k < 3; ++k) dr[k] = mic1(xb[k] - xa[k], o->L[k], o->invL[k]);
+}
+
+static inline double r2_of(const double dr[3]) {
+  return fma(dr[2], dr[2], fma(dr[1], dr[1], dr[0] * dr[0]));
+}
+
+static inline uint32_t quant1(double x, double s) { return (uint32_t)(int64_t)llrint(x * s); }
+
+/* FP32 membership rule on quantized positions; also returns the FP32 r2. */
+static inline float r2f_quant(const or_ctx* o, const double* xa, const double* xb) {
+  float d[3];
+  for (int k = 0; k < 3; ++k) {
+    uint32_t qa = quant1(xa[k], o->qscale[k]);
+    uint32_t qb = quant1(xb[k], o->qscale[k]);
+    int32_t di = (int32_t)(qb - qa);
+    d[k] = (float)di * o->unitf[k];
+  }
+  return fmaf(d[2], d[2], fmaf(d[1], d[1], d[0] * d[0]));
+}
+
+static inline int pair_in(const or_ctx* o, const double* xa, const double* xb, double r2) {
+  if (o->cfg.precision == RS_PRECISION_FP32) return r2f_quant(o, xa, xb) <= o->rc2f;
+  return r2 <= o->rc2;
+}
+
+static inline double wrap1(double x, double L) { return x - floor(x / L) * L; }
+
+/* ---------------------------------------------------------------- context */
+
+void or_default_config(rs_config* c) {
+  memset(c, 0, sizeof(*c));
+  c->abi_version = RS_ABI_VERSION;
+  c->n_molecules = 216;
+  c->box[0] = c->box[1] = c->box[2] = 1.862;
+  c->cutoff = 0.8;
+  c->skin = 0.1;
+  c->nl_every = 10;
+  c->precision = RS_PRECISION_FP64;
+  c->dt = 0.001;
+  c->gamma = 1.0;
+  c->temperature = 300.0;
+  c->eps_rf = 78.3;
+  c->coulomb_k = 138.935457644;
+  c->kB = 0.00831446262;
+  c->mass_O = 15.99943;
+  c->mass_H = 1.007947;
+  c->theta0 = 1.82421813418;
+  c->rdf_bins = 200;
+  c->rdf_rmax = 0.8;
+  c->rdf_width = 0.0005;
+  c->rdf_window = 0.04;
+  c->w_rdf = 1.0;
+  c->clip_norm = INFINITY;
+}
+
+static void free_records(or_ctx* o) {
+  for (int i = 0; i < o->nsnaps; ++i) {
+    free(o->snaps[i].x);
+    free(o->snaps[i].v);
+  }
+  o->nsnaps = 0;
+  free(o->frames);
+  o->frames = NULL;
+  o->nframes = 0;
+}
+
+int or_destroy(or_ctx* o) {
+  if (!o) return 0;
+  free_records(o);
+  free(o->snaps);
+  free(o->mass3);
+  free(o->pa);
+  free(o->pb);
+  free(o->cnt);
+  free(o->target);
+  free(o->ext);
+  free(o);
+  return 0;
+}
+
+int or_create(const rs_config* cfg, or_ctx** out) {
+  *out = NULL;
+  if (!cfg || cfg->abi_version != RS_ABI_VERSION || cfg->n_molecules < 2) return RS_E_CONFIG;
+  or_ctx* o = (or_ctx*)calloc(1, sizeof(or_ctx));
+  o->cfg = *cfg;
+  o->M = cfg->n_molecules;
+  o->N = 3 * o->M;
+  o->N3 = 3 * o->N;
+  for (int k = 0; k < 3; ++k) {
+    o->L[k] = cfg->box[k];
+    o->invL[k] = 1.0 / cfg->box[k];
+    o->qscale[k] = 4294967296.0 / cfg->box[k];
+    o->unitf[k] = (float)(cfg->box[k] / 4294967296.0);
+    if (cfg->cutoff > 0.5 * cfg->box[k]) {
+      free(o);
+      return RS_E_CONFIG;
+    }
+  }
+  double rc = cfg->cutoff;
+  o->krf = (1.0 / (rc * rc * rc)) * ((cfg->eps_rf - 1.0) / (2.0 * cfg->eps_rf + 1.0));
+  o->rc2 = rc * rc;
+  o->rc2f = (float)(rc * rc);
+  o->mass3 = (double*)malloc(sizeof(double) * o->N3);
+  for (int a = 0; a < o->N; ++a)
+    for (int k = 0; k < 3; ++k) o->mass3[3 * a + k] = (a % 3 == 0) ? cfg->mass_O : cfg->mass_H;
+  /* atom list radius as in the reference (dist_neighbors = cutoff + 0.2, water/water.jl:38-39) */
+  o->rlist = rc + 0.2;
+  o->theta[0] = 0.315;
+  o->theta[1] = 0.636;
+  o->theta[2] = 0.417;
+  o->theta[3] = 462750.4;
+  o->theta[4] = 0.09572;
+  o->theta[5] = 836.8;
+  o->target = (double*)calloc(cfg->rdf_bins > 0 ? cfg->rdf_bins : 1, sizeof(double));
+  *out = o;
+  return 0;
+}
+
+int or_set_theta(or_ctx* o, const double* th) {
+  memcpy(o->theta, th, sizeof(double) * NTH);
+  return 0;
+}
+
+int or_set_target_rdf(or_ctx* o, const double* g, int32_t n) {
+  if (n != o->cfg.rdf_bins) return RS_E_CONFIG;
+  memcpy(o->target, g, sizeof(double) * n);
+  return 0;
+}
+
+int or_set_external_noise(or_ctx* o, const double* xi, int32_t n_steps) {
+  free(o->ext);
+  o->ext = NULL;
+  o->ext_steps = 0;
+  if (!xi) return 0;
+  o->ext = (double*)malloc(sizeof(double) * (size_t)o->N3 * n_steps);
+  memcpy(o->ext, xi, sizeof(double) * (size_t)o->N3 * n_steps);
+  o->ext_steps = n_steps;
+  return 0;
+}
+
+/* ---------------------------------------------------------------- neighbour list (atoms) */
+
+/* Half list of atom pairs in different molecules with FP64 minimum-image distance <= rlist.
+   Plays the role of Molly's CellListMap finder (water/water.jl:1025,1038). Cells of edge >= rlist;
+   fewer than 3 cells along any axis falls back to all pairs. Order: by a, then cell order, then
+   b ascending — deterministic. */
+static void build_list(or_ctx* o, const double* x) {
+  const int N = o->N;
+  int nc[3], all = 0;
+  for (int k = 0; k < 3; ++k) {
+    nc[k] = (int)floor(o->L[k] / o->rlist);
+    if (nc[k] < 3) all = 1;
+  }
+  double rl2 = o->rlist * o->rlist;
+  if (!o->cnt) o->cnt = (int64_t*)malloc(sizeof(int64_t) * (N + 1));
+  int* cell_of = NULL;
+  int *cstart = NULL, *cidx = NULL;
+  int ncell = 1;
+  if (!all) {
+    ncell = nc[0] * nc[1] * nc[2];
+    cell_of = (int*)malloc(sizeof(int) * N);
+    cstart = (int*)calloc(ncell + 1, sizeof(int));
+    cidx = (int*)malloc(sizeof(int) * N);
+    for (int a = 0; a < N; ++a) {
+      int c[3];
+      for (int k = 0; k < 3; ++k) {
+        double w = wrap1(x[3 * a + k], o->L[k]);
+        c[k] = (int)(w / o->L[k] * nc[k]);
+        if (c[k] >= nc[k]) c[k] = nc[k] - 1;
+        if (c[k] < 0) c[k] = 0;
+      }
+      cell_of[a] = (c[2] * nc[1] + c[1]) * nc[0] + c[0];
+      cstart[cell_of[a] + 1]++;
+    }
+    for (int c = 0; c < ncell; ++c) cstart[c + 1] += cstart[c];
+    int* fill = (int*)malloc(sizeof(int) * ncell);
+    memcpy(fill, cstart, sizeof(int) * ncell);
+    for (int a = 0; a < N; ++a) cidx[fill[cell_of[a]]++] = a; /* ascending a within a cell */
+    free(fill);
+  }
+  for (int pass = 0; pass < 2; ++pass) {
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int a = 0; a < N; ++a) {
+      int64_t n = 0, base = pass ? o->cnt[a] : 0;
+      int ma = a / 3;
+      if (all) {
+        for (int b = a + 1; b < N; ++b) {
+          if (b / 3 == ma) continue;
+          double dr[3];
+          mic3(o, x + 3 * a, x + 3 * b, dr);
+          if (r2_of(dr) <= rl2) {
+            if (pass) {
+              o->pa[base + n] = a;
+              o->pb[base + n] = b;
+            }
+            ++n;
+          }
+        }
+      } else {
+        int c = cell_of[a];
+        int cx = c % nc[0], cy = (c / nc[0]) % nc[1], cz = c / (nc[0] * nc[1]);
+        for (int dz = -1; dz <= 1; ++dz)
+          for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+              int ex = (cx + dx + nc[0]) % nc[0], ey = (cy + dy + nc[1]) % nc[1],
+                  ez = (cz + dz + nc[2]) % nc[2];
+              int e = (ez * nc[1] + ey) * nc[0] + ex;
+              for (int t = cstart[e]; t < cstart[e + 1]; ++t) {
+                int b = cidx[t];
+                if (b <= a || b / 3 == ma) continue;
+                double dr[3];
+                mic3(o, x + 3 * a, x + 3 * b, dr);
+                if (r2_of(dr) <= rl2) {
+                  if (pass) {
+                    o->pa[base + n] = a;
+                    o->pb[base + n] = b;
+                  }
+                  ++n;
+                }
+              }
+            }
+      }
+      if (!pass) o->cnt[a] = n;
+    }
+    if (!pass) {
+      int64_t tot = 0;
+      for (int a = 0; a < N; ++a) {
+        int64_t c = o->cnt[a];
+        o->cnt[a] = tot;
+        tot += c;
+      }
+      o->cnt[N] = tot;
+      if (tot > o->cap) {
+        free(o->pa);
+        free(o->pb);
+        o->cap = tot + tot / 4 + 16;
+        o->pa = (int32_t*)malloc(sizeof(int32_t) * o->cap);
+        o->pb = (int32_t*)malloc(sizeof(int32_t) * o->cap);
+      }
+      o->npairs = tot;
+    }
+  }
+  free(cell_of);
+  free(cstart);
+  free(cidx);
+}
+
+/* Molecule pairs i<j with FP64 O-O minimum-image distance <= r (brute force, for list parity). */
+int or_molecule_pairs(or_ctx* o, const double* x, double r, int32_t* pairs, int64_t capacity,
+                      int64_t* n_pairs) {
+  int64_t n = 0;
+  double r2 = r * r;
+  for (int i = 0; i < o->M; ++i)
+    for (int j = i + 1; j < o->M; ++j) {
+      double dr[3];
+      mic3(o, x + 9 * i, x + 9 * j, dr);
+      if (r2_of(dr) <= r2) {
+        if (n < capacity) {
+          pairs[2 * n] = i;
+          pairs[2 * n + 1] = j;
+        }
+        ++n;
+      }
+    }
+  *n_pairs = n;
+  return n > capacity ? RS_E_OVERFLOW : 0;
+}
+
+/* ---------------------------------------------------------------- dual numbers (angle HVP) */
+
+typedef struct {
+  double v, d;
+} dual;
+
+static inline dual D(double v, double d) {
+  dual r = {v, d};
+  return r;
+}
+static inline dual dadd(dual a, dual b) { return D(a.v + b.v, a.d + b.d); }
+static inline dual dsub(dual a, dual b) { return D(a.v - b.v, a.d - b.d); }
+static inline dual dmul(dual a, dual b) { return D(a.v * b.v, a.d * b.v + a.v * b.d); }
+static inline dual ddiv(dual a, dual b) {
+  return D(a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v));
+}
+static inline dual dsqrt(dual a) {
+  double s = sqrt(a.v);
+  return D(s, a.d / (2.0 * s));
+}
+static inline dual dacos(dual a) { return D(acos(a.v), -a.d / sqrt(1.0 - a.v * a.v)); }
+static inline dual dneg(dual a) { return D(-a.v, -a.d); }
+
+static inline void dcross(const dual a[3], const dual b[3], dual c[3]) {
+  c[0] = dsub(dmul(a[1], b[2]), dmul(a[2], b[1]));
+  c[1] = dsub(dmul(a[2], b[0]), dmul(a[0], b[2]));
+  c[2] = dsub(dmul(a[0], b[1]), dmul(a[1], b[0]));
+}
+static inline dual ddot(const dual a[3], const dual b[3]) {
+  return dadd(dadd(dmul(a[0], b[0]), dmul(a[1], b[1])), dmul(a[2], b[2]));
+}
+
+/* Harmonic angle force at (i, j=centre, k), restating water/water.jl:936-968 with dual numbers:
+   the value parts are the forces, the tangent parts are the directional derivative of the forces
+   along (ui, uj, uk), i.e. (dF/dx) u = (dF/dx)^T u since dF/dx is the negative Hessian.
+   Also returns the θ-derivative weights (disp) and the unit directions. Returns 0 if skipped. */
+static int angle_dual(const or_ctx* o, const double* xi, const double* xj, const double* xk,
+                      const double* ui, const double* uj, const double* uk, double ka,
+                      double theta0, double fi[3], double fj[3], double fk[3], double hi[3],
+                      double hj[3], double hk[3], double* disp_out, double p1d_out[3],
+                      double p3d_out[3], double* energy) {
+  double dji[3], djk[3];
+  mic3(o, xj, xi, dji);
+  mic3(o, xj, xk, djk);
+  dual a[3], b[3];
+  for (int k = 0; k < 3; ++k) {
+    a[k] = D(dji[k], ui ? ui[k] - uj[k] : 0.0);
+    b[k] = D(djk[k], uk ? uk[k] - uj[k] : 0.0);
+  }
+  dual cr[3];
+  dcross(a, b, cr);
+  if (cr[0].v == 0.0 && cr[1].v == 0.0 && cr[2].v == 0.0) return 0;
+  dual r2ji = ddot(a, a), r2jk = ddot(b, b);
+  dual rji = dsqrt(r2ji), rjk = dsqrt(r2jk);
+  dual p1u[3], p3u[3], mb[3];
+  dcross(a, cr, p1u);
+  for (int k = 0; k < 3; ++k) mb[k] = dneg(b[k]);
+  dcross(mb, cr, p3u);
+  dual p1n = dsqrt(ddot(p1u, p1u)), p3n = dsqrt(ddot(p3u, p3u));
+  dual p1d[3], p3d[3];
+  for (int k = 0; k < 3; ++k) {
+    p1d[k] = ddiv(ddiv(p1u[k], p1n), rji);
+    p3d[k] = ddiv(ddiv(p3u[k], p3n), rjk);
+  }
+  dual dotp = ddot(a, b);
+  dual cs = ddiv(dotp, dmul(rji, rjk));
+  dual th = dacos(cs);
+  dual disp = dsub(D(theta0, 0.0), th);
+  dual F = dmul(D(ka, 0.0), disp);
+  for (int k = 0; k < 3; ++k) {
+    dual gi = dmul(F, p1d[k]);
+    dual gk = dmul(F, p3d[k]);
+    dual gj = dneg(dadd(gi, gk));
+    fi[k] = gi.v;
+    fk[k] = gk.v;
+    fj[k] = gj.v;
+    hi[k] = gi.d;
+    hk[k] = gk.d;
+    hj[k] = gj.d;
+    p1d_out[k] = p1d[k].v;
+    p3d_out[k] = p3d[k].v;
+  }
+  *disp_out = disp.v;
+  if (energy) *energy = 0.5 * ka * disp.v * disp.v;
+  return 1;
+}
+
+/* ---------------------------------------------------------------- forces and adjoint terms */
+
+/* One pass over all interactions. With c == NULL only forces (and energy) are produced
+   (calc_F!, water/water.jl:877-975); otherwise the adjoint quantities of calc_F_and_grads!
+   (water/water.jl:1212-1433): h = (dF/dx)^T (c/m) and vjp[k] = sum_dof w_dof dF_dof/dθ_k with
+   θ = (σ_O, ε_O, q_H, k_b, r0, k_a). Per-thread buffers followed by a reduction, as in the
+   reference (water/water.jl:879-885,970-972,1424-1430). */
+static void eval_terms(or_ctx* o, const double* x, const double* c, const double* w, double* F,
+                       double* h, double* vjp, double* energy) {
+  const int N3 = o->N3;
+  const double sig = o->theta[0], eps = o->theta[1], qH = o->theta[2];
+  const double kb = o->theta[3], r0 = o->theta[4], ka = o->theta[5];
+  const double qO = -2.0 * qH;
+  const double kc = o->cfg.coulomb_k, krf = o->krf;
+  const int adj = (c != NULL);
+  int nth = 1;
+#ifdef _OPENMP
+  nth = omp_get_max_threads();
+#endif
+  double* Fb = (double*)calloc((size_t)N3 * nth, sizeof(double));
+  double* hb = adj ? (double*)calloc((size_t)N3 * nth, sizeof(double)) : NULL;
+  double* vb = (double*)calloc((size_t)NTH * nth, sizeof(double));
+  double* eb = (double*)calloc((size_t)nth, sizeof(double));
+  double* u = NULL;
+  if (adj) {
+    u = (double*)malloc(sizeof(double) * N3);
+    for (int i = 0; i < N3; ++i) u[i] = c[i] / o->mass3[i];
+  }
+  const int64_t np = o->npairs;
+#pragma omp parallel
+  {
+    int t = 0;
+#ifdef _OPENMP
+    t = omp_get_thread_num();
+#endif
+    double* Ft = Fb + (size_t)N3 * t;
+    double* ht = adj ? hb + (size_t)N3 * t : NULL;
+    double* vt = vb + NTH * t;
+    double et = 0.0;
+#pragma omp for schedule(static)
+    for (int64_t p = 0; p < np; ++p) {
+      int a = o->pa[p], b = o->pb[p];
+      double dr[3];
+      mic3(o, x + 3 * a, x + 3 * b, dr);
+      double r2 = r2_of(dr);
+      if (!pair_in(o, x + 3 * a, x + 3 * b, r2)) continue;
+      int aO = (a % 3 == 0), bO = (b % 3 == 0);
+      double qa = aO ? qO : qH, qb = bO ? qO : qH;
+      double r = sqrt(r2);
+      double invr2 = 1.0 / r2;
+      double invr3 = invr2 / r;
+      double ndr[3] = {dr[0] / r, dr[1] / r, dr[2] / r};
+      double cp = kc * qa * qb;
+      double Fp = -2.0 * cp * (invr3 + krf);
+      double Frf = cp * (invr2 - 2.0 * krf * r);
+      double Fv = Frf;
+      double dFds = 0.0, dFde = 0.0;
+      if (energy) et += cp * (1.0 / r + krf * r2);
+      if (aO && bO && eps != 0.0) {
+        double six = pow(sig * sig * invr2, 3);
+        double twelve = six * six;
+        double Flj = (24.0 * eps / r) * (2.0 * twelve - six);
+        Fp += -24.0 * eps * invr2 * (26.0 * twelve - 7.0 * six);
+        dFds = (144.0 * eps * invr2 / sig) * (4.0 * twelve - six); /* times dr */
+        dFde = Flj / eps;                                            /* times ndr */
+        Fv += Flj;
+        if (energy) et += 4.0 * eps * (twelve - six);
+      }
+      for (int k = 0; k < 3; ++k) {
+        Ft[3 * a + k] -= Fv * ndr[k];
+        Ft[3 * b + k] += Fv * ndr[k];
+      }
+      if (adj) {
+        /* K = F' r̂r̂ᵀ + (F/r)(I - r̂r̂ᵀ); h_a += K (u_a - u_b); h_b -= K (u_a - u_b) */
+        double du[3], dot = 0.0;
+        for (int k = 0; k < 3; ++k) {
+          du[k] = u[3 * a + k] - u[3 * b + k];
+          dot += ndr[k] * du[k];
+        }
+        double Fr = Fv / r;
+        for (int k = 0; k < 3; ++k) {
+          double hv = Fr * du[k] + (Fp - Fr) * dot * ndr[k];
+          ht[3 * a + k] += hv;
+          ht[3 * b + k] -= hv;
+        }
+        double wd_n = 0.0, wd_r = 0.0; /* ndr.(w_b - w_a), dr.(w_b - w_a) */
+        for (int k = 0; k < 3; ++k) {
+          double dw = w[3 * b + k] - w[3 * a + k];
+          wd_n += ndr[k] * dw;
+          wd_r += dr[k] * dw;
+        }
+        vt[0] += dFds * wd_r;
+        vt[1] += dFde * wd_n;
+        vt[2] += (2.0 * Frf / qH) * wd_n;
+      }
+    }
+    /* bonded terms, per molecule: O-H1, O-H2 bonds (water/water.jl:915-934,1279-1305) and the
+       H1-O-H2 angle (water/water.jl:936-968,1307-1422). */
+#pragma omp for schedule(static)
+    for (int m = 0; m < o->M; ++m) {
+      int O = 3 * m;
+      for (int hh = 1; hh <= 2; ++hh) {
+        int i = O, j = O + hh;
+        double dr[3];
+        mic3(o, x + 3 * i, x + 3 * j, dr);
+        double r2 = r2_of(dr);
+        double r = sqrt(r2);
+        double ndr[3] = {dr[0] / r, dr[1] / r, dr[2] / r};
+        double disp = r - r0;
+        double Fv = -kb * disp;
+        double Fp = -kb;
+        if (energy) et += 0.5 * kb * disp * disp;
+        for (int k = 0; k < 3; ++k) {
+          Ft[3 * i + k] -= Fv * ndr[k];
+          Ft[3 * j + k] += Fv * ndr[k];
+        }
+        if (adj) {
+          double du[3], dot = 0.0;
+          for (int k = 0; k < 3; ++k) {
+            du[k] = u[3 * i + k] - u[3 * j + k];
+            dot += ndr[k] * du[k];
+          }
+          double Fr = Fv / r;
+          for (int k = 0; k < 3; ++k) {
+            double hv = Fr * du[k] + (Fp - Fr) * dot * ndr[k];
+            ht[3 * i + k] += hv;
+            ht[3 * j + k] -= hv;
+          }
+          /* dF_i/dkb = disp*ndr, dF_j/dkb = -disp*ndr; dF_i/dr0 = -kb*ndr, dF_j/dr0 = kb*ndr */
+          for (int k = 0; k < 3; ++k) {
+            double dw = w[3 * i + k] - w[3 * j + k];
+            vt[3] += disp * ndr[k] * dw;
+            vt[4] += -kb * ndr[k] * dw;
+          }
+        }
+      }
+      double fi[3], fj[3], fk[3], hi[3], hj[3], hk[3], p1d[3], p3d[3], disp, ea = 0.0;
+      int ok = angle_dual(o, x + 3 * (O + 1), x + 3 * O, x + 3 * (O + 2), adj ? u + 3 * (O + 1) : NULL,
+                          adj ? u + 3 * O : NULL, adj ? u + 3 * (O + 2) : NULL, ka, o->cfg.theta0,
+                          fi, fj, fk, hi, hj, hk, &disp, p1d, p3d, energy ? &ea : NULL);
+      if (ok) {
+        et += ea;
+        for (int k = 0; k < 3; ++k) {
+          Ft[3 * (O + 1) + k] += fi[k];
+          Ft[3 * O + k] += fj[k];
+          Ft[3 * (O + 2) + k] += fk[k];
+          if (adj) {
+            ht[3 * (O + 1) + k] += hi[k];
+            ht[3 * O + k] += hj[k];
+            ht[3 * (O + 2) + k] += hk[k];
+            vt[5] += disp * (p1d[k] * w[3 * (O + 1) + k] + p3d[k] * w[3 * (O + 2) + k] -
+                             (p1d[k] + p3d[k]) * w[3 * O + k]);
+          }
+        }
+      }
+    }
+    eb[t] = et;
+  }
+  for (int i = 0; i < N3; ++i) {
+    double s = 0.0;
+    for (int t = 0; t < nth; ++t) s += Fb[(size_t)N3 * t + i];
+    F[i] = s;
+    if (adj) {
+      double sh = 0.0;
+      for (int t = 0; t < nth; ++t) sh += hb[(size_t)N3 * t + i];
+      h[i] = sh;
+    }
+  }
+  if (vjp) {
+    for (int k = 0; k < NTH; ++k) {
+      double s = 0.0;
+      for (int t = 0; t < nth; ++t) s += vb[NTH * t + k];
+      vjp[k] = s;
+    }
+  }
+  if (energy) {
+    double s = 0.0;
+    for (int t = 0; t < nth; ++t) s += eb[t];
+    *energy = s;
+  }
+  free(Fb);
+  free(hb);
+  free(vb);
+  free(eb);
+  free(u);
+}
+
+int or_forces(or_ctx* o, const double* x, double* F, double* energy) {
+  build_list(o, x);
+  eval_terms(o, x, NULL, NULL, F, NULL, NULL, energy);
+  return 0;
+}
+
+int or_adjoint_terms(or_ctx* o, const double* x, const double* c, const double* w, double* F,
+                     double* h, double* vjp6) {
+  build_list(o, x);
+  eval_terms(o, x, c, w, F, h, vjp6, NULL);
+  return 0;
+}
+
+/* ---------------------------------------------------------------- Philox noise */
+
+#define PHILOX_M0 0xD2E7470EE14C6C93ULL
+#define PHILOX_M1 0xCA5A826395121157ULL
+#define PHILOX_W0 0x9E3779B97F4A7C15ULL
+#define PHILOX_W1 0xBB67AE8584CAA73BULL
+
+/* Philox4x64-10 (Salmon et al., SC'11), the generator numpy exposes as np.random.Philox. */
+void or_philox4x64(const uint64_t ctr_in[4], const uint64_t key_in[2], uint64_t out[4]) {
+  uint64_t c0 = ctr_in[0], c1 = ctr_in[1], c2 = ctr_in[2], c3 = ctr_in[3];
+  uint64_t k0 = key_in[0], k1 = key_in[1];
+  for (int r = 0; r < 10; ++r) {
+    __uint128_t p0 = (__uint128_t)PHILOX_M0 * c0;
+    __uint128_t p1 = (__uint128_t)PHILOX_M1 * c2;
+    uint64_t hi0 = (uint64_t)(p0 >> 64), lo0 = (uint64_t)p0;
+    uint64_t hi1 = (uint64_t)(p1 >> 64), lo1 = (uint64_t)p1;
+    uint64_t n0 = hi1 ^ c1 ^ k0, n1 = lo1, n2 = hi0 ^ c3 ^ k1, n3 = lo0;
+    c0 = n0;
+    c1 = n1;
+    c2 = n2;
+    c3 = n3;
+    k0 += PHILOX_W0;
+    k1 += PHILOX_W1;
+  }
+  out[0] = c0;
+  out[1] = c1;
+  out[2] = c2;
+  out[3] = c3;
+}
+
+/* Standard normals of atom a at step: counter (step, a, 0, 0), key (seed, 0); Box-Muller on
+   53-bit uniforms u1 in (0,1], u2 in [0,1). */
+static void normals3(uint64_t seed, int32_t step, int a, double n[3]) {
+  uint64_t ctr[4] = {(uint64_t)(uint32_t)step, (uint64_t)a, 0, 0}, key[2] = {seed, 0}, r[4];
+  or_philox4x64(ctr, key, r);
+  const double two53 = 1.0 / 9007199254740992.0;
+  double u1 = (double)((r[0] >> 11) + 1) * two53, u2 = (double)(r[1] >> 11) * two53;
+  double u3 = (double)((r[2] >> 11) + 1) * two53, u4 = (double)(r[3] >> 11) * two53;
+  double rad1 = sqrt(-2.0 * log(u1)), rad2 = sqrt(-2.0 * log(u3));
+  const double twopi = 6.283185307179586;
+  n[0] = rad1 * cos(twopi * u2);
+  n[1] = rad1 * sin(twopi * u2);
+  n[2] = rad2 * cos(twopi * u4);
+}
+
+/* generate_noise! (water/water.jl:867-873): ξ_a = sqrt(kB T / m_a) · N(0, I3). */
+int or_noise(or_ctx* o, uint64_t seed, int32_t step, double* xi) {
+  if (o->ext) {
+    if (step < 1 || step > o->ext_steps) return RS_E_CONFIG;
+    memcpy(xi, o->ext + (size_t)(step - 1) * o->N3, sizeof(double) * o->N3);
+    return 0;
+  }
+  double kT = o->cfg.kB * o->cfg.temperature;
+#pragma omp parallel for schedule(static)
+  for (int a = 0; a < o->N; ++a) {
+    double n[3];
+    normals3(seed, step, a, n);
+    double s = sqrt(kT / o->mass3[3 * a]);
+    for (int k = 0; k < 3; ++k) xi[3 * a + k] = s * n[k];
+  }
+  return 0;
+}
+
+/* ---------------------------------------------------------------- exp used by the RDF */
+
+/* exp(t) for t <= 0 by Cody-Waite reduction t = k ln2 + f (fdlibm's split of ln2) and a Taylor
+   polynomial in f evaluated with fused multiply-adds; exact scaling by 2^k. The GPU library
+   evaluates the same sequence so that smoothed-histogram contributions agree bit for bit. */
+static inline float rs_expf(float t) {
+  float k = rintf(t * 1.44269504088896341f);
+  float f = fmaf(-k, 0.693145751953125f, t);
+  f = fmaf(-k, 1.428606765330187045e-06f, f);
+  float p = 1.0f / 5040.0f;
+  p = fmaf(p, f, 1.0f / 720.0f);
+  p = fmaf(p, f, 1.0f / 120.0f);
+  p = fmaf(p, f, 1.0f / 24.0f);
+  p = fmaf(p, f, 1.0f / 6.0f);
+  p = fmaf(p, f, 0.5f);
+  p = fmaf(p, f, 1.0f);
+  p = fmaf(p, f, 1.0f);
+  return ldexpf(p, (int)k);
+}
+
+static inline double rs_exp(double t) {
+  double k = rint(t * 1.4426950408889634);
+  double f = fma(-k, 6.93147180369123816490e-01, t);
+  f = fma(-k, 1.90821492927058770002e-10, f);
+  static const double inv_fact[14] = {1.0,
+                                      1.0,
+                                      1.0 / 2.0,
+                                      1.0 / 6.0,
+                                      1.0 / 24.0,
+                                      1.0 / 120.0,
+                                      1.0 / 720.0,
+                                      1.0 / 5040.0,
+                                      1.0 / 40320.0,
+                                      1.0 / 362880.0,
+                                      1.0 / 3628800.0,
+                                      1.0 / 39916800.0,
+                                      1.0 / 479001600.0,
+                                      1.0 / 6227020800.0};
+  double p = inv_fact[13];
+  for (int i = 12; i >= 0; --i) p = fma(p, f, inv_fact[i]);
+  return ldexp(p, (int)k);
+}
+
+float or_expf(float t) { return rs_expf(t); }
+double or_exp(double t) { return rs_exp(t); }
+
+/* ---------------------------------------------------------------- RDF loss */
+
+typedef struct {
+  int lo, hi;
+} win_t;
+
+/* Smoothed-histogram contributions of one O-O distance (rdf_differentiable,
+   water/water.jl:631-691, re-binned per the config: bin centres x_b = (b+1/2)Δ, Δ = rmax/bins,
+   window |x_b - r| <= window, weights exp(-(x_b-r)^2/width) normalised over the window).
+   Adds llrint(φ_b · 2^32) into hist. Returns the window. */
+static win_t rdf_contrib(const or_ctx* o, double r64, float r32, int64_t* hist) {
+  const int nb = o->cfg.rdf_bins;
+  win_t w = {0, -1};
+  if (o->cfg.precision == RS_PRECISION_FP32) {
+    float invd = (float)(nb / o->cfg.rdf_rmax), c = (float)o->cfg.rdf_window;
+    float dx = (float)(o->cfg.rdf_rmax / nb), iw = (float)(1.0 / o->cfg.rdf_width);
+    int lo = (int)ceilf((r32 - c) * invd - 0.5f), hi = (int)floorf((r32 + c) * invd - 0.5f);
+    if (lo < 0) lo = 0;
+    if (hi > nb - 1) hi = nb - 1;
+    w.lo = lo;
+    w.hi = hi;
+    if (lo > hi) return w;
+    float e[512];
+    float den = 0.0f;
+    for (int b = lo; b <= hi; ++b) {
+      float t = ((float)b + 0.5f) * dx - r32;
+      e[b - lo] = rs_expf(-(t * t) * iw);
+      den += e[b - lo];
+    }
+    for (int b = lo; b <= hi; ++b) hist[b] += (int64_t)llrintf((e[b - lo] / den) * 4294967296.0f);
+  } else {
+    double invd = nb / o->cfg.rdf_rmax, c = o->cfg.rdf_window;
+    double dx = o->cfg.rdf_rmax / nb, iw = 1.0 / o->cfg.rdf_width;
+    int lo = (int)ceil((r64 - c) * invd - 0.5), hi = (int)floor((r64 + c) * invd - 0.5);
+    if (lo < 0) lo = 0;
+    if (hi > nb - 1) hi = nb - 1;
+    w.lo = lo;
+    w.hi = hi;
+    if (lo > hi) return w;
+    double e[512];
+    double den = 0.0;
+    for (int b = lo; b <= hi; ++b) {
+      double t = ((double)b + 0.5) * dx - r64;
+      e[b - lo] = rs_exp(-(t * t) * iw);
+      den += e[b - lo];
+    }
+    for (int b = lo; b <= hi; ++b) hist[b] += (int64_t)llrint((e[b - lo] / den) * 4294967296.0);
+  }
+  return w;
+}
+
+/* g_b, L = sum_b (g_b - target_b)^2 and dL/dx (water/water.jl:693-736 with a squared loss). The
+   gradient differentiates φ_b(r) = e_b(r)/den(r) inside the fixed window, as the reference's
+   Enzyme rrule does. */
+int or_rdf(or_ctx* o, const double* x, double* g, int64_t* hist_out, double* loss, double* dLdx) {
+  const int nb = o->cfg.rdf_bins, M = o->M;
+  if (nb <= 0 || 2.0 * o->cfg.rdf_window * nb / o->cfg.rdf_rmax + 3.0 > 512.0) return RS_E_CONFIG;
+  build_list(o, x);
+  int nth = 1;
+#ifdef _OPENMP
+  nth = omp_get_max_threads();
+#endif
+  int64_t* hb = (int64_t*)calloc((size_t)nb * nth, sizeof(int64_t));
+  const int64_t np = o->npairs;
+  const double rmaxw = o->cfg.rdf_rmax + o->cfg.rdf_window;
+#pragma omp parallel
+  {
+    int t = 0;
+#ifdef _OPENMP
+    t = omp_get_thread_num();
+#endif
+    int64_t* ht = hb + (size_t)nb * t;
+#pragma omp for schedule(static)
+    for (int64_t p = 0; p < np; ++p) {
+      int a = o->pa[p], b = o->pb[p];
+      if (a % 3 || b % 3) continue;
+      double dr[3];
+      mic3(o, x + 3 * a, x + 3 * b, dr);
+      double r64 = sqrt(r2_of(dr));
+      float r32 = sqrtf(r2f_quant(o, x + 3 * a, x + 3 * b));
+      if (r64 > rmaxw + 0.01) continue;
+      rdf_contrib(o, r64, r32, ht);
+    }
+  }
+  int64_t* H = (int64_t*)calloc(nb, sizeof(int64_t));
+  for (int b = 0; b < nb; ++b)
+    for (int t = 0; t < nth; ++t) H[b] += hb[(size_t)nb * t + b];
+  free(hb);
+  double V = o->L[0] * o->L[1] * o->L[2];
+  double dx = o->cfg.rdf_rmax / nb;
+  double pre = V / (4.0 * M_PI * (double)M * (double)M * dx);
+  double* G = (double*)malloc(sizeof(double) * nb);
+  double L = 0.0;
+  for (int b = 0; b < nb; ++b) {
+    double xb = ((double)b + 0.5) * dx;
+    double gb = 2.0 * pre / (xb * xb) * ((double)H[b] / 4294967296.0);
+    if (g) g[b] = gb;
+    double d = gb - o->target[b];
+    L += d * d;
+    G[b] = 2.0 * d * 2.0 * pre / (xb * xb);
+  }
+  if (hist_out) memcpy(hist_out, H, sizeof(int64_t) * nb);
+  if (loss) *loss = L;
+  if (dLdx) {
+    memset(dLdx, 0, sizeof(double) * o->N3);
+    double iw = 1.0 / o->cfg.rdf_width;
+    for (int64_t p = 0; p < np; ++p) {
+      int a = o->pa[p], b = o->pb[p];
+      if (a % 3 || b % 3) continue;
+      double dr[3];
+      mic3(o, x + 3 * a, x + 3 * b, dr);
+      double r = sqrt(r2_of(dr));
+      if (r > rmaxw + 0.01) continue;
+      float r32 = sqrtf(r2f_quant(o, x + 3 * a, x + 3 * b));
+      win_t w;
+      /* window exactly as used for the histogram */
+      if (o->cfg.precision == RS_PRECISION_FP32) {
+        float invd = (float)(nb / o->cfg.rdf_rmax), c = (float)o->cfg.rdf_window;
+        w.lo = (int)ceilf((r32 - c) * invd - 0.5f);
+        w.hi = (int)floorf((r32 + c) * invd - 0.5f);
+      } else {
+        double invd = nb / o->cfg.rdf_rmax, c = o->cfg.rdf_window;
+        w.lo = (int)ceil((r - c) * invd - 0.5);
+        w.hi = (int)floor((r + c) * invd - 0.5);
+      }
+      if (w.lo < 0) w.lo = 0;
+      if (w.hi > nb - 1) w.hi = nb - 1;
+      if (w.lo > w.hi) continue;
+      double den = 0.0, dden = 0.0, s1 = 0.0, s2 = 0.0;
+      for (int bb = w.lo; bb <= w.hi; ++bb) {
+        double t = ((double)bb + 0.5) * dx - r;
+        double e = rs_exp(-(t * t) * iw);
+        double de = e * 2.0 * t * iw; /* d e / d r */
+        den += e;
+        dden += de;
+        s1 += G[bb] * de;
+        s2 += G[bb] * e;
+      }
+      double dLdr = s1 / den - dden * s2 / (den * den);
+      for (int k = 0; k < 3; ++k) {
+        double gk = dLdr * dr[k] / r;
+        dLdx[3 * b + k] += gk;
+        dLdx[3 * a + k] -= gk;
+      }
+    }
+  }
+  free(H);
+  free(G);
+  return 0;
+}
+
+/* ---------------------------------------------------------------- MSD loss */
+
+/* calculate_D (diffusion/diffusion.jl:861-877) on the O atoms of frames f = 0..2K-1 spaced
+   msd_every*dt apart; unwrapped displacement by summed minimum-image increments; D in nm^2/ps;
+   L = w_msd 1e18 (1e-6 D - target)^2 (diffusion/diffusion.jl:897-911). dL/dframes analytic:
+   the increments telescope, so dΔ_s/dx_{s+K} = I and dΔ_s/dx_s = -I. */
+int or_msd(or_ctx* o, const double* frames, int32_t nf, double* Dout, double* loss, double* dLdf) {
+  const int M = o->M;
+  if (nf < 2 || nf % 2) return RS_E_CONFIG;
+  int K = nf / 2;
+  double lag = K * o->cfg.msd_every * o->cfg.dt;
+  double* del = (double*)malloc(sizeof(double) * (size_t)K * M * 3);
+  double Dsum = 0.0;
+  for (int s = 0; s < K; ++s) {
+    double msd = 0.0;
+    for (int i = 0; i < M; ++i) {
+      double acc[3] = {0, 0, 0};
+      for (int f = s; f < s + K; ++f) {
+        double d[3];
+        mic3(o, frames + (size_t)f * 3 * M + 3 * i, frames + (size_t)(f + 1) * 3 * M + 3 * i, d);
+        for (int k = 0; k < 3; ++k) acc[k] += d[k];
+      }
+      for (int k = 0; k < 3; ++k) del[((size_t)s * M + i) * 3 + k] = acc[k];
+      msd += acc[0] * acc[0] + acc[1] * acc[1] + acc[2] * acc[2];
+    }
+    msd /= M;
+    Dsum += msd / (K * lag);
+  }
+  double Dv = Dsum / (6.0 * K);
+  double diff = 1e-6 * Dv - o->cfg.msd_target;
+  double L = o->cfg.w_msd * 1e18 * diff * diff;
+  if (Dout) *Dout = Dv;
+  if (loss) *loss = L;
+  if (dLdf) {
+    double dLdD = o->cfg.w_msd * 1e18 * 2.0 * diff * 1e-6;
+    double c = dLdD * 2.0 / (6.0 * K * K * lag * M);
+    memset(dLdf, 0, sizeof(double) * (size_t)nf * 3 * M);
+    for (int s = 0; s < K; ++s)
+      for (int i = 0; i < M; ++i)
+        for (int k = 0; k < 3; ++k) {
+          double d = del[((size_t)s * M + i) * 3 + k];
+          dLdf[(size_t)(s + K) * 3 * M + 3 * i + k] += c * d;
+          dLdf[(size_t)s * 3 * M + 3 * i + k] -= c * d;
+        }
+  }
+  free(del);
+  return 0;
+}
+
+/* ---------------------------------------------------------------- trajectory loop */
+
+typedef struct {
+  int n_loss, n_blocks, n_trunc, n_log;
+  int msd_on, nf, first_frame;
+} plan_t;
+
+static int make_plan(const or_ctx* o, int n_steps, plan_t* p) {
+  const rs_config* c = &o->cfg;
+  p->n_loss = c->n_steps_loss > 0 ? c->n_steps_loss : n_steps;
+  if (n_steps % p->n_loss) return RS_E_CONFIG;
+  p->n_blocks = n_steps / p->n_loss;
+  p->n_trunc = c->n_steps_trunc > 0 ? c->n_steps_trunc : p->n_loss;
+  if (p->n_trunc > p->n_loss) return RS_E_CONFIG;
+  p->n_log = c->n_steps_log;
+  p->msd_on = (c->msd_window_steps > 0 && c->msd_every > 0 && c->w_msd != 0.0);
+  p->nf = 0;
+  p->first_frame = 0;
+  if (p->msd_on) {
+    if (c->msd_window_steps % c->msd_every || c->msd_window_steps > n_steps) return RS_E_CONFIG;
+    p->nf = c->msd_window_steps / c->msd_every;
+    if (p->nf < 2 || p->nf % 2) return RS_E_CONFIG;
+    p->first_frame = n_steps - c->msd_window_steps + c->msd_every;
+  }
+  return 0;
+}
+
+static int frame_index(const plan_t* p, int t, int every) {
+  if (!p->msd_on || t < p->first_frame) return -1;
+  if ((t - p->first_frame) % every) return -1;
+  return (t - p->first_frame) / every;
+}
+
+static void wrap_all(const or_ctx* o, double* x) {
+  for (int a = 0; a < o->N; ++a)
+    for (int k = 0; k < 3; ++k) x[3 * a + k] = wrap1(x[3 * a + k], o->L[k]);
+}
+
+static void add_snap(or_ctx* o, int step, const double* x, const double* v) {
+  if (o->nsnaps == o->capsnaps) {
+    o->capsnaps = o->capsnaps ? 2 * o->capsnaps : 16;
+    o->snaps = (snap_t*)realloc(o->snaps, sizeof(snap_t) * o->capsnaps);
+  }
+  snap_t* s = &o->snaps[o->nsnaps++];
+  s->step = step;
+  s->x = (double*)malloc(sizeof(double) * o->N3);
+  s->v = (double*)malloc(sizeof(double) * o->N3);
+  memcpy(s->x, x, sizeof(double) * o->N3);
+  memcpy(s->v, v, sizeof(double) * o->N3);
+}
+
+static snap_t* find_snap(or_ctx* o, int step) {
+  for (int i = 0; i < o->nsnaps; ++i)
+    if (o->snaps[i].step == step) return &o->snaps[i];
+  return NULL;
+}
+
+/* forward_sim loop (water/water.jl:1029-1045), run_grads=false, with the loss of each block and
+   the MSD frames. Optionally stores every state (for the stored-trajectory adjoint). */
+static int forward_impl(or_ctx* o, const double* x0, const double* v0, uint64_t seed, int n_steps,
+                        double* x_out, double* v_out, double* loss_out, double* traj_x) {
+  plan_t p;
+  int rc = make_plan(o, n_steps, &p);
+  if (rc) return rc;
+  if (o->ext && o->ext_steps < n_steps) return RS_E_CONFIG;
+  const int N3 = o->N3, M = o->M;
+  const double dt = o->cfg.dt, e = exp(-o->cfg.gamma * dt), s = sqrt(1.0 - e * e);
+  double* x = (double*)malloc(sizeof(double) * N3);
+  double* v = (double*)malloc(sizeof(double) * N3);
+  double* F = (double*)malloc(sizeof(double) * N3);
+  double* xi = (double*)malloc(sizeof(double) * N3);
+  memcpy(x, x0, sizeof(double) * N3);
+  memcpy(v, v0, sizeof(double) * N3);
+  wrap_all(o, x);
+  free_records(o);
+  if (p.msd_on) {
+    o->nframes = p.nf;
+    o->frames = (double*)malloc(sizeof(double) * (size_t)p.nf * 3 * M);
+  }
+  add_snap(o, 0, x, v);
+  if (traj_x) memcpy(traj_x, x, sizeof(double) * N3);
+  build_list(o, x);
+  double lrdf = 0.0;
+  for (int n = 1; n <= n_steps; ++n) {
+    eval_terms(o, x, NULL, NULL, F, NULL, NULL, NULL);
+    or_noise(o, seed, n, xi);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < N3; ++i) {
+      double m = o->mass3[i];
+      double vi = v[i] + (F[i] * dt) / m;
+      double xx = x[i] + (vi * dt) / 2.0;
+      vi = vi * e + xi[i] * s;
+      xx = xx + (vi * dt) / 2.0;
+      x[i] = wrap1(xx, o->L[i % 3]);
+      v[i] = vi;
+    }
+    if (n % o->cfg.nl_every == 0) build_list(o, x);
+    if (traj_x) memcpy(traj_x + (size_t)n * N3, x, sizeof(double) * N3);
+    if ((p.n_log > 0 && n % p.n_log == 0) || n % p.n_loss == 0) add_snap(o, n, x, v);
+    if (n % p.n_loss == 0 && o->cfg.w_rdf != 0.0) {
+      double l;
+      or_rdf(o, x, NULL, NULL, &l, NULL);
+      lrdf += l;
+    }
+    int f = frame_index(&p, n, o->cfg.msd_every);
+    if (f >= 0)
+      for (int i = 0; i < M; ++i)
+        for (int k = 0; k < 3; ++k) o->frames[(size_t)f * 3 * M + 3 * i + k] = x[9 * i + k];
+  }
+  double loss = o->cfg.w_rdf * lrdf / p.n_blocks;
+  if (p.msd_on) {
+    double lm;
+    or_msd(o, o->frames, p.nf, NULL, &lm, NULL);
+    loss += lm;
+  }
+  o->rec_steps = n_steps;
+  o->rec_seed = seed;
+  if (x_out) memcpy(x_out, x, sizeof(double) * N3);
+  if (v_out) memcpy(v_out, v, sizeof(double) * N3);
+  if (loss_out) *loss_out = loss;
+  free(x);
+  free(v);
+  free(F);
+  free(xi);
+  return 0;
+}
+
+int or_forward(or_ctx* o, const double* x0, const double* v0, uint64_t seed, int32_t n_steps,
+               double* x_out, double* v_out, double* loss_out) {
+  return forward_impl(o, x0, v0, seed, n_steps, x_out, v_out, loss_out, NULL);
+}
+
+/* Gradient seed at state t: RDF block loss (weight w_rdf/n_blocks) and the MSD frame gradient. */
+static double seed_grad(or_ctx* o, const plan_t* p, int t, const double* x, const double* dLdf,
+                        double* g, int with_rdf) {
+  const int N3 = o->N3, M = o->M;
+  memset(g, 0, sizeof(double) * N3);
+  double l = 0.0;
+  if (with_rdf && o->cfg.w_rdf != 0.0) {
+    double* gr = (double*)malloc(sizeof(double) * N3);
+    or_rdf(o, x, NULL, NULL, &l, gr);
+    double wgt = o->cfg.w_rdf / p->n_blocks;
+    for (int i = 0; i < N3; ++i) g[i] += wgt * gr[i];
+    l *= wgt;
+    free(gr);
+  }
+  int f = frame_index(p, t, o->cfg.msd_every);
+  if (f >= 0 && dLdf)
+    for (int i = 0; i < M; ++i)
+      for (int k = 0; k < 3; ++k) g[9 * i + k] += dLdf[(size_t)f * 3 * M + 3 * i + k];
+  return l;
+}
+
+/* reverse_sim (water/water.jl:1435-1558): reverse integration with regenerated noise and the
+   A, B, C, D accumulator recurrence; block losses reset the accumulators, MSD frame gradients
+   inside a block are added to them (same initial terms, by linearity of the recurrence). */
+int or_reverse(or_ctx* o, const double* xN, const double* vN, uint64_t seed, int32_t n_steps,
+               double* loss_out, double* dLdth, double* x0_out, double* v0_out) {
+  plan_t p;
+  int rc = make_plan(o, n_steps, &p);
+  if (rc) return rc;
+  if (o->rec_steps != n_steps || o->rec_seed != seed) return RS_E_STATE;
+  const int N3 = o->N3, M = o->M;
+  const double dt = o->cfg.dt, dt2 = dt * dt, e = exp(-o->cfg.gamma * dt), s = sqrt(1.0 - e * e);
+  double *x = malloc(sizeof(double) * N3), *v = malloc(sizeof(double) * N3);
+  double *A = malloc(sizeof(double) * N3), *B = malloc(sizeof(double) * N3);
+  double *C = calloc(N3, sizeof(double)), *Dd = malloc(sizeof(double) * N3);
+  double *h = calloc(N3, sizeof(double)), *F = malloc(sizeof(double) * N3);
+  double *g = malloc(sizeof(double) * N3), *xi = malloc(sizeof(double) * N3);
+  double* w = malloc(sizeof(double) * N3);
+  double* dLdf = NULL;
+  double loss = 0.0;
+  memcpy(x, xN, sizeof(double) * N3);
+  memcpy(v, vN, sizeof(double) * N3);
+  if (p.msd_on) {
+    double lm;
+    dLdf = (double*)malloc(sizeof(double) * (size_t)p.nf * 3 * M);
+    or_msd(o, o->frames, p.nf, NULL, &lm, dLdf);
+    loss += lm;
+  }
+  loss += seed_grad(o, &p, n_steps, x, dLdf, g, 1);
+  for (int i = 0; i < N3; ++i) {
+    A[i] = g[i] * (1.0 + e);
+    B[i] = g[i] * (1.0 + e);
+    Dd[i] = g[i] * (1.0 + 2.0 * e + e * e) / (2.0 * e);
+  }
+  double th[NTH] = {0};
+  build_list(o, x);
+  int since_build = 0, trunc_count = 0;
+  for (int n = n_steps; n >= 1; --n) {
+    trunc_count++;
+    if (trunc_count <= p.n_trunc) {
+      or_noise(o, seed, n, xi);
+#pragma omp parallel for schedule(static)
+      for (int i = 0; i < N3; ++i) {
+        double xx = x[i] - (v[i] * dt) / 2.0;
+        double vi = (v[i] - xi[i] * s) / e;
+        xx = xx - (vi * dt) / 2.0;
+        x[i] = wrap1(xx, o->L[i % 3]);
+        v[i] = vi;
+        Dd[i] = e * Dd[i] + ((dt2 * h[i]) * (1.0 + e)) / 2.0;
+        C[i] = C[i] + Dd[i];
+        w[i] = A[i] * (dt2 / (2.0 * o->mass3[i]));
+      }
+      if (++since_build >= o->cfg.nl_every) {
+        build_list(o, x);
+        since_build = 0;
+      }
+      double vj[NTH];
+      eval_terms(o, x, C, w, F, h, vj, NULL);
+      if (o->cfg.clip_norm > 0.0 && isfinite(o->cfg.clip_norm)) {
+        double nn = 0.0;
+        for (int i = 0; i < N3; ++i) nn += h[i] * h[i];
+        nn = sqrt(nn);
+        if (nn > o->cfg.clip_norm)
+          for (int i = 0; i < N3; ++i) h[i] = h[i] * o->cfg.clip_norm / nn;
+      }
+#pragma omp parallel for schedule(static)
+      for (int i = 0; i < N3; ++i) {
+        v[i] = v[i] - (F[i] * dt) / o->mass3[i];
+        B[i] = e * B[i] + dt2 * h[i];
+        A[i] = A[i] + B[i];
+      }
+      for (int k = 0; k < NTH; ++k) th[k] += vj[k];
+    }
+    int t = n - 1;
+    if (t == 0) break;
+    int restored = 0;
+    if (p.n_log > 0 && t % p.n_log == 0 && (trunc_count <= p.n_trunc || t % p.n_loss == 0)) {
+      snap_t* sn = find_snap(o, t);
+      if (sn) {
+        memcpy(x, sn->x, sizeof(double) * N3);
+        memcpy(v, sn->v, sizeof(double) * N3);
+        restored = 1;
+      }
+    }
+    if (t % p.n_loss == 0) {
+      if (!restored && trunc_count > p.n_trunc) {
+        snap_t* sn = find_snap(o, t);
+        memcpy(x, sn->x, sizeof(double) * N3);
+        memcpy(v, sn->v, sizeof(double) * N3);
+      }
+      trunc_count = 0;
+      loss += seed_grad(o, &p, t, x, dLdf, g, 1);
+      build_list(o, x);
+      since_build = 0;
+      for (int i = 0; i < N3; ++i) {
+        A[i] = g[i] * (1.0 + e);
+        B[i] = g[i] * (1.0 + e);
+        C[i] = 0.0;
+        Dd[i] = g[i] * (1.0 + 2.0 * e + e * e) / (2.0 * e);
+        h[i] = 0.0;
+      }
+    } else if (frame_index(&p, t, o->cfg.msd_every) >= 0) {
+      seed_grad(o, &p, t, x, dLdf, g, 0);
+      for (int i = 0; i < N3; ++i) {
+        A[i] += g[i] * (1.0 + e);
+        B[i] += g[i] * (1.0 + e);
+        Dd[i] += g[i] * (1.0 + 2.0 * e + e * e) / (2.0 * e);
+      }
+    }
+  }
+  if (loss_out) *loss_out = loss;
+  if (dLdth) memcpy(dLdth, th, sizeof(th));
+  if (x0_out) memcpy(x0_out, x, sizeof(double) * N3);
+  if (v0_out) memcpy(v0_out, v, sizeof(double) * N3);
+  free(x);
+  free(v);
+  free(A);
+  free(B);
+  free(C);
+  free(Dd);
+  free(h);
+  free(F);
+  free(g);
+  free(xi);
+  free(w);
+  free(dLdf);
+  return 0;
+}
+
+int or_value_and_grad(or_ctx* o, const double* x0, const double* v0, uint64_t seed,
+                      int32_t n_steps, double* loss_out, double* dLdth) {
+  const int N3 = o->N3;
+  double *x = malloc(sizeof(double) * N3), *v = malloc(sizeof(double) * N3);
+  int rc = or_forward(o, x0, v0, seed, n_steps, x, v, NULL);
+  if (!rc) rc = or_reverse(o, x, v, seed, n_steps, loss_out, dLdth, NULL, NULL);
+  free(x);
+  free(v);
+  return rc;
+}
+
+/* Independent check of the reverse pass: the textbook adjoint of the forward map over the STORED
+   forward trajectory (no time reversal). For step n: a = (dt/2)(1+e) λx_n + e λv_n,
+   λx_{n-1} = λx_n + (dF/dx)(x_{n-1})^T (dt a/m) + g_{n-1}, λv_{n-1} = a,
+   dL/dθ += (dt a/m) · dF/dθ(x_{n-1}). Full gradient: requires a single loss block (no reset). */
+int or_value_and_grad_stored(or_ctx* o, const double* x0, const double* v0, uint64_t seed,
+                             int32_t n_steps, double* loss_out, double* dLdth) {
+  plan_t p;
+  int rc = make_plan(o, n_steps, &p);
+  if (rc) return rc;
+  if (p.n_blocks != 1 || p.n_trunc != p.n_loss) return RS_E_CONFIG;
+  const int N3 = o->N3, M = o->M;
+  const double dt = o->cfg.dt, e = exp(-o->cfg.gamma * dt);
+  double* traj = (double*)malloc(sizeof(double) * (size_t)(n_steps + 1) * N3);
+  double loss;
+  rc = forward_impl(o, x0, v0, seed, n_steps, NULL, NULL, &loss, traj);
+  if (rc) {
+    free(traj);
+    return rc;
+  }
+  double *lx = malloc(sizeof(double) * N3), *lv = calloc(N3, sizeof(double));
+  double *g = malloc(sizeof(double) * N3), *F = malloc(sizeof(double) * N3);
+  double *hh = malloc(sizeof(double) * N3), *cc = malloc(sizeof(double) * N3);
+  double* dLdf = NULL;
+  if (p.msd_on) {
+    dLdf = (double*)malloc(sizeof(double) * (size_t)p.nf * 3 * M);
+    or_msd(o, o->frames, p.nf, NULL, NULL, dLdf);
+  }
+  seed_grad(o, &p, n_steps, traj + (size_t)n_steps * N3, dLdf, lx, 1);
+  double th[NTH] = {0};
+  for (int n = n_steps; n >= 1; --n) {
+    const double* xp = traj + (size_t)(n - 1) * N3;
+    for (int i = 0; i < N3; ++i) {
+      double a = (dt / 2.0) * (1.0 + e) * lx[i] + e * lv[i];
+      lv[i] = a;
+      /* pass c with c/m = dt a/m and w = dt a/m: c = dt*a, w = dt*a/m */
+      cc[i] = dt * a;
+      hh[i] = dt * a / o->mass3[i];
+    }
+    double vj[NTH];
+    build_list(o, xp);
+    eval_terms(o, xp, cc, hh, F, g, vj, NULL);
+    for (int i = 0; i < N3; ++i) lx[i] += g[i];
+    for (int k = 0; k < NTH; ++k) th[k] += vj[k];
+    int t = n - 1;
+    if (t > 0 && frame_index(&p, t, o->cfg.msd_every) >= 0) {
+      seed_grad(o, &p, t, traj + (size_t)t * N3, dLdf, g, 0);
+      for (int i = 0; i < N3; ++i) lx[i] += g[i];
+    }
+  }
+  if (loss_out) *loss_out = loss;
+  if (dLdth) memcpy(dLdth, th, sizeof(th));
+  free(traj);
+  free(lx);
+  free(lv);
+  free(g);
+  free(F);
+  free(hh);
+  free(cc);
+  free(dLdf);
+  return 0;
+}
+
+double or_list_radius(const or_ctx* o) { return o->rlist; }
+int or_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}

@@ -1,0 +1,202 @@
+"""ctypes wrapper of the FP64 CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: used by tests/, the smoke check and the CPU-baseline leg of bench.py as
+the checker of librevsim.so. Nothing in the product package imports this module.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "..", "rev-sim_amd"))
+from revsim._abi import RsConfig  # noqa: E402  (config struct layout only)
+
+_P = ctypes.POINTER
+_dp = _P(ctypes.c_double)
+_ip = _P(ctypes.c_int32)
+_lp = _P(ctypes.c_int64)
+_vp = ctypes.c_void_p
+_u64 = ctypes.c_uint64
+
+_SIGS = [
+    ("or_create", ctypes.c_int, [_P(RsConfig), _P(_vp)]),
+    ("or_destroy", ctypes.c_int, [_vp]),
+    ("or_last_error", ctypes.c_char_p, [_vp]),
+    ("or_set_theta", ctypes.c_int, [_vp, _dp]),
+    ("or_set_target_rdf", ctypes.c_int, [_vp, _dp, ctypes.c_int32]),
+    ("or_set_external_noise", ctypes.c_int, [_vp, _dp, ctypes.c_int32]),
+    ("or_forces", ctypes.c_int, [_vp, _dp, _dp, _dp]),
+    ("or_adjoint_terms", ctypes.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp]),
+    ("or_noise", ctypes.c_int, [_vp, _u64, ctypes.c_int32, _dp]),
+    ("or_rdf", ctypes.c_int, [_vp, _dp, _dp, _lp, _dp, _dp]),
+    ("or_msd", ctypes.c_int, [_vp, _dp, ctypes.c_int32, _dp, _dp, _dp]),
+    ("or_molecule_pairs", ctypes.c_int, [_vp, _dp, ctypes.c_double, _ip, ctypes.c_int64, _lp]),
+    ("or_forward", ctypes.c_int, [_vp, _dp, _dp, _u64, ctypes.c_int32, _dp, _dp, _dp]),
+    ("or_reverse", ctypes.c_int, [_vp, _dp, _dp, _u64, ctypes.c_int32, _dp, _dp, _dp, _dp]),
+    ("or_value_and_grad", ctypes.c_int, [_vp, _dp, _dp, _u64, ctypes.c_int32, _dp, _dp]),
+    ("or_value_and_grad_stored", ctypes.c_int, [_vp, _dp, _dp, _u64, ctypes.c_int32, _dp, _dp]),
+    ("or_list_radius", ctypes.c_double, [_vp]),
+    ("or_num_threads", ctypes.c_int, []),
+    ("or_expf", ctypes.c_float, [ctypes.c_float]),
+    ("or_exp", ctypes.c_double, [ctypes.c_double]),
+    ("or_philox4x64", None, [_P(ctypes.c_uint64), _P(ctypes.c_uint64), _P(ctypes.c_uint64)]),
+]
+
+_LIB = None
+
+
+def build() -> str:
+    """Compile oracle/liboracle.so with its Makefile (gcc, OpenMP)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return os.path.join(HERE, "liboracle.so")
+
+
+def lib() -> ctypes.CDLL:
+    global _LIB
+    if _LIB is None:
+        p = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(p):
+            build()
+        L = ctypes.CDLL(p)
+        for n, r, a in _SIGS:
+            f = getattr(L, n)
+            f.restype = r
+            f.argtypes = a
+        _LIB = L
+    return _LIB
+
+
+def _d(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous, (a.dtype, a.flags)
+    return a.ctypes.data_as(_dp)
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+class Oracle:
+    """FP64 restatement of the reference loop with the same call surface as revsim.Simulator."""
+
+    def __init__(self, cfg: RsConfig):
+        self.cfg = cfg
+        self.L = lib()
+        h = _vp()
+        rc = self.L.or_create(ctypes.byref(cfg), ctypes.byref(h))
+        if rc:
+            raise OracleError(f"or_create failed ({rc})")
+        self.h = h
+        self.N3 = 9 * cfg.n_molecules
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.or_destroy(self.h)
+            self.h = None
+
+    def _chk(self, rc):
+        if rc:
+            raise OracleError(f"oracle error {rc}: {self.L.or_last_error(self.h).decode()}")
+
+    def set_theta(self, th):
+        th = np.ascontiguousarray(th, dtype=np.float64)
+        self._chk(self.L.or_set_theta(self.h, _d(th)))
+
+    def set_target_rdf(self, g):
+        g = np.ascontiguousarray(g, dtype=np.float64)
+        self._chk(self.L.or_set_target_rdf(self.h, _d(g), g.size))
+
+    def set_external_noise(self, xi, n_steps):
+        xi = None if xi is None else np.ascontiguousarray(xi, dtype=np.float64)
+        self._chk(self.L.or_set_external_noise(self.h, _d(xi), int(n_steps)))
+
+    def forces(self, x, with_energy=False):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        F = np.zeros(self.N3)
+        e = np.zeros(1)
+        self._chk(self.L.or_forces(self.h, _d(x), _d(F), _d(e)))
+        return (F, float(e[0])) if with_energy else F
+
+    def adjoint_terms(self, x, c, w):
+        x, c, w = (np.ascontiguousarray(a, dtype=np.float64) for a in (x, c, w))
+        F, h, v = np.zeros(self.N3), np.zeros(self.N3), np.zeros(6)
+        self._chk(self.L.or_adjoint_terms(self.h, _d(x), _d(c), _d(w), _d(F), _d(h), _d(v)))
+        return F, h, v
+
+    def noise(self, seed, step):
+        xi = np.zeros(self.N3)
+        self._chk(self.L.or_noise(self.h, seed, step, _d(xi)))
+        return xi
+
+    def rdf(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        nb = self.cfg.rdf_bins
+        g, H = np.zeros(nb), np.zeros(nb, dtype=np.int64)
+        loss, d = np.zeros(1), np.zeros(self.N3)
+        self._chk(self.L.or_rdf(self.h, _d(x), _d(g), H.ctypes.data_as(_lp), _d(loss), _d(d)))
+        return g, H, float(loss[0]), d
+
+    def msd(self, frames):
+        frames = np.ascontiguousarray(frames, dtype=np.float64)
+        nf = frames.size // (3 * self.cfg.n_molecules)
+        D, loss, d = np.zeros(1), np.zeros(1), np.zeros(frames.size)
+        self._chk(self.L.or_msd(self.h, _d(frames), nf, _d(D), _d(loss), _d(d)))
+        return float(D[0]), float(loss[0]), d
+
+    def molecule_pairs(self, x, r):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        cap = self.cfg.n_molecules * (self.cfg.n_molecules - 1) // 2
+        cap = min(cap, 4_000_000)
+        pairs = np.zeros(2 * cap, dtype=np.int32)
+        n = ctypes.c_int64()
+        self._chk(self.L.or_molecule_pairs(self.h, _d(x), float(r), pairs.ctypes.data_as(_ip), cap,
+                                           ctypes.byref(n)))
+        return pairs[: 2 * n.value].reshape(-1, 2)
+
+    def forward(self, x0, v0, seed, n_steps):
+        x0, v0 = (np.ascontiguousarray(a, dtype=np.float64) for a in (x0, v0))
+        x, v, loss = np.zeros(self.N3), np.zeros(self.N3), np.zeros(1)
+        self._chk(self.L.or_forward(self.h, _d(x0), _d(v0), seed, n_steps, _d(x), _d(v), _d(loss)))
+        return x, v, float(loss[0])
+
+    def reverse(self, xN, vN, seed, n_steps):
+        xN, vN = (np.ascontiguousarray(a, dtype=np.float64) for a in (xN, vN))
+        loss, g = np.zeros(1), np.zeros(6)
+        x0, v0 = np.zeros(self.N3), np.zeros(self.N3)
+        self._chk(self.L.or_reverse(self.h, _d(xN), _d(vN), seed, n_steps, _d(loss), _d(g), _d(x0),
+                                    _d(v0)))
+        return float(loss[0]), g, x0, v0
+
+    def value_and_grad(self, x0, v0, seed, n_steps):
+        x0, v0 = (np.ascontiguousarray(a, dtype=np.float64) for a in (x0, v0))
+        loss, g = np.zeros(1), np.zeros(6)
+        self._chk(self.L.or_value_and_grad(self.h, _d(x0), _d(v0), seed, n_steps, _d(loss), _d(g)))
+        return float(loss[0]), g
+
+    def value_and_grad_stored(self, x0, v0, seed, n_steps):
+        x0, v0 = (np.ascontiguousarray(a, dtype=np.float64) for a in (x0, v0))
+        loss, g = np.zeros(1), np.zeros(6)
+        self._chk(self.L.or_value_and_grad_stored(self.h, _d(x0), _d(v0), seed, n_steps, _d(loss),
+                                                  _d(g)))
+        return float(loss[0]), g
+
+    def list_radius(self):
+        return self.L.or_list_radius(self.h)
+
+
+def philox4x64(ctr, key):
+    c = (ctypes.c_uint64 * 4)(*ctr)
+    k = (ctypes.c_uint64 * 2)(*key)
+    o = (ctypes.c_uint64 * 4)()
+    lib().or_philox4x64(c, k, o)
+    return [int(v) for v in o]
+
+
+def num_threads() -> int:
+    return lib().or_num_threads()
