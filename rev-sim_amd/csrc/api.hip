@@ -1,0 +1,783 @@
+// api.hip — the C ABI of librevsim.so (include/revsim.h): context, buffers and the host-side
+// orchestration of the forward trajectory and of the reverse-time adjoint.
+//
+// forward_sim  (water/water.jl:977-1105, run_grads=false) -> rs_forward
+// reverse_sim  (water/water.jl:1435-1558)                 -> rs_reverse
+// All device work of a call is queued on the context's stream; the host synchronises once at the
+// end of the call to copy results back and check the device flags.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "internal.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct Plan {
+  int n_loss, n_blocks, n_trunc, n_log;
+  bool msd_on;
+  int nf, first_frame, every;
+  int frame_of(int t) const {
+    if (!msd_on || t < first_frame || (t - first_frame) % every) return -1;
+    return (t - first_frame) / every;
+  }
+  bool snap_at(int t) const { return t == 0 || (n_log > 0 && t % n_log == 0) || t % n_loss == 0; }
+};
+
+}  // namespace
+
+struct rs_ctx {
+  rs_config cfg;
+  int device = 0;
+  hipStream_t s = nullptr;
+  Launch L;
+  double theta[6];
+  int M = 0, N = 0, N3 = 0;
+  double rlist = 0.0;
+  int nslots = 0;
+  // records of the last forward trajectory
+  bool rec_valid = false;
+  int rec_steps = 0;
+  uint64_t rec_seed = 0;
+  std::vector<int> snap_steps;
+  std::vector<double*> snap_x, snap_v;
+  int frames_cap = 0, msd_del_cap = 0;
+  double msd_loss = 0.0;
+  int ext_steps = 0;
+  int since_build = 0;
+  std::string err;
+  // timing
+  bool timing = false;
+  double ms[RS_PHASE_COUNT] = {0};
+  std::vector<double> slot_host;
+};
+
+namespace {
+
+int fail(rs_ctx* c, int code, const std::string& msg) {
+  g_last_error = msg;
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPCHK(c, expr)                                                                    \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      return fail((c), RS_E_HIP, std::string(#expr ": ") + hipGetErrorString(e_));          \
+  } while (0)
+
+const double H_EXTENT_FACTOR = 1.25;  // h_ext = 1.25 r0 + 0.005 nm (checked every step)
+
+void set_params(rs_ctx* c) {
+  SysParams& p = c->L.p;
+  const rs_config& g = c->cfg;
+  p.M = c->M;
+  p.N = c->N;
+  for (int k = 0; k < 3; ++k) {
+    p.L[k] = g.box[k];
+    p.invL[k] = 1.0 / g.box[k];
+    p.qscale[k] = 4294967296.0 / g.box[k];
+    p.unitf[k] = (float)(g.box[k] / 4294967296.0);
+  }
+  const double rc = g.cutoff;
+  p.rc2 = rc * rc;
+  p.rc2f = (float)(rc * rc);
+  p.krf = (1.0 / (rc * rc * rc)) * ((g.eps_rf - 1.0) / (2.0 * g.eps_rf + 1.0));
+  p.krff = (float)p.krf;
+  p.kc = g.coulomb_k;
+  p.sig = c->theta[0];
+  p.eps = c->theta[1];
+  p.qH = c->theta[2];
+  p.qO = -2.0 * c->theta[2];
+  p.kb = c->theta[3];
+  p.r0 = c->theta[4];
+  p.ka = c->theta[5];
+  p.theta0 = g.theta0;
+  p.kqq[0] = g.coulomb_k * p.qO * p.qO;
+  p.kqq[1] = g.coulomb_k * p.qO * p.qH;
+  p.kqq[2] = g.coulomb_k * p.qH * p.qH;
+  for (int k = 0; k < 3; ++k) p.kqqf[k] = (float)p.kqq[k];
+  p.sig2f = (float)(p.sig * p.sig);
+  p.eps24f = (float)(24.0 * p.eps);
+  p.mO = g.mass_O;
+  p.mH = g.mass_H;
+  p.hext = H_EXTENT_FACTOR * std::fabs(p.r0) + 0.005;
+  c->rlist = rc + 2.0 * p.hext + g.skin;
+  p.rlist2 = c->rlist * c->rlist;
+  p.skin_half2 = 0.25 * g.skin * g.skin;
+  p.brute = 0;
+  p.ncell = 1;
+  for (int k = 0; k < 3; ++k) {
+    p.nc[k] = (int)std::floor(g.box[k] / c->rlist);
+    if (p.nc[k] < 3) p.brute = 1;
+  }
+  if (p.brute) {
+    p.nc[0] = p.nc[1] = p.nc[2] = 1;
+  } else {
+    p.ncell = p.nc[0] * p.nc[1] * p.nc[2];
+  }
+}
+
+int default_capc(const rs_ctx* c) {
+  const SysParams& p = c->L.p;
+  if (p.brute) return (p.M + p.CH - 1) / p.CH + 1;
+  const double V = c->cfg.box[0] * c->cfg.box[1] * c->cfg.box[2];
+  const double nexp = (p.M / V) * (4.0 / 3.0) * M_PI * c->rlist * c->rlist * c->rlist;
+  return (int)std::ceil(1.3 * nexp / p.CH + 24.0);
+}
+
+template <class T>
+int dalloc(rs_ctx* c, T** p, size_t n) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  HIPCHK(c, hipMalloc((void**)p, n * sizeof(T)));
+  HIPCHK(c, hipMemsetAsync(*p, 0, n * sizeof(T), c->s));
+  return 0;
+}
+
+int alloc_list(rs_ctx* c) {
+  DevBufs& b = c->L.b;
+  if (b.nbr) hipFree(b.nbr);
+  b.nbr = nullptr;
+  const size_t lanes = (size_t)pair_blocks(c->L.p) * RS_BLOCK;
+  return dalloc(c, &b.nbr, lanes * c->L.p.capc);
+}
+
+int alloc_all(rs_ctx* c) {
+  DevBufs& b = c->L.b;
+  std::memset(&b, 0, sizeof(b));
+  const size_t N3 = c->N3, M = c->M, N = c->N;
+  const SysParams& p = c->L.p;
+  const size_t lanes = (size_t)pair_blocks(p) * RS_BLOCK;
+  int r = 0;
+  double** st[] = {&b.x, &b.v, &b.F, &b.h, &b.A, &b.B, &b.C, &b.D, &b.grad};
+  for (auto q : st)
+    if ((r = dalloc(c, q, N3))) return r;
+  if ((r = dalloc(c, &b.cell, M)) || (r = dalloc(c, &b.cell_count, p.ncell + 1)) ||
+      (r = dalloc(c, &b.cell_start, p.ncell + 1)) || (r = dalloc(c, &b.cell_fill, p.ncell + 1)) ||
+      (r = dalloc(c, &b.tmp, M)) || (r = dalloc(c, &b.order, M)) || (r = dalloc(c, &b.rank, M)))
+    return r;
+  if ((r = dalloc(c, &b.posq, N)) || (r = dalloc(c, &b.pos64, N)) || (r = dalloc(c, &b.uf, N)) ||
+      (r = dalloc(c, &b.ud, N)) || (r = dalloc(c, &b.O64, M)))
+    return r;
+  if ((r = dalloc(c, &b.ncnt, lanes)) || (r = dalloc(c, &b.flags, FLAG_COUNT)) ||
+      (r = dalloc(c, &b.slots, (size_t)pair_blocks(p) * 8)) || (r = dalloc(c, &b.dth, 8)) ||
+      (r = dalloc(c, &b.scal, 16)) || (r = dalloc(c, &b.hpart, 256)))
+    return r;
+  const int nb = std::max(1, c->cfg.rdf_bins);
+  if ((r = dalloc(c, &b.hist, nb)) || (r = dalloc(c, &b.rdfG, nb)) || (r = dalloc(c, &b.target, nb)) ||
+      (r = dalloc(c, &b.rdf_g, nb)))
+    return r;
+  if ((r = dalloc(c, &b.msd_acc, 64))) return r;
+  return alloc_list(c);
+}
+
+void free_records(rs_ctx* c) {
+  for (auto q : c->snap_x) hipFree(q);
+  for (auto q : c->snap_v) hipFree(q);
+  c->snap_x.clear();
+  c->snap_v.clear();
+  c->snap_steps.clear();
+  c->rec_valid = false;
+}
+
+int make_plan(rs_ctx* c, int n_steps, Plan* p) {
+  const rs_config& g = c->cfg;
+  if (n_steps < 1) return fail(c, RS_E_CONFIG, "n_steps must be >= 1");
+  p->n_loss = g.n_steps_loss > 0 ? g.n_steps_loss : n_steps;
+  if (n_steps % p->n_loss) return fail(c, RS_E_CONFIG, "n_steps_loss must divide n_steps");
+  p->n_blocks = n_steps / p->n_loss;
+  p->n_trunc = g.n_steps_trunc > 0 ? g.n_steps_trunc : p->n_loss;
+  if (p->n_trunc > p->n_loss) return fail(c, RS_E_CONFIG, "n_steps_trunc must not exceed n_steps_loss");
+  p->n_log = g.n_steps_log;
+  p->msd_on = g.msd_window_steps > 0 && g.msd_every > 0 && g.w_msd != 0.0;
+  p->nf = 0;
+  p->first_frame = 0;
+  p->every = std::max(1, g.msd_every);
+  if (p->msd_on) {
+    if (g.msd_window_steps % g.msd_every || g.msd_window_steps > n_steps)
+      return fail(c, RS_E_CONFIG, "msd window must be a multiple of msd_every and <= n_steps");
+    p->nf = g.msd_window_steps / g.msd_every;
+    if (p->nf < 2 || p->nf % 2 || p->nf > 128)
+      return fail(c, RS_E_CONFIG, "msd window must hold an even number of frames in [2, 128]");
+    p->first_frame = n_steps - g.msd_window_steps + g.msd_every;
+  }
+  return 0;
+}
+
+int ensure_frames(rs_ctx* c, int nf) {
+  DevBufs& b = c->L.b;
+  if (nf <= c->frames_cap) return 0;
+  if (b.frames) hipFree(b.frames);
+  if (b.msd_grad) hipFree(b.msd_grad);
+  if (b.msd_del) hipFree(b.msd_del);
+  int r;
+  if ((r = dalloc(c, &b.frames, (size_t)nf * 3 * c->M)) ||
+      (r = dalloc(c, &b.msd_grad, (size_t)nf * 3 * c->M)) ||
+      (r = dalloc(c, &b.msd_del, (size_t)(nf / 2 + 1) * 3 * c->M)))
+    return r;
+  c->frames_cap = nf;
+  return 0;
+}
+
+void rebuild(rs_ctx* c, bool with_u) {
+  launch_list_build(c->L, with_u);
+  c->since_build = 0;
+}
+
+int reset_flags(rs_ctx* c) {
+  HIPCHK(c, hipMemsetAsync(c->L.b.flags, 0, sizeof(int) * FLAG_COUNT, c->s));
+  return 0;
+}
+
+// Synchronise and translate the device flags into a status. Sets *overflow for a retry.
+int finish(rs_ctx* c, bool* overflow) {
+  HIPCHK(c, hipStreamSynchronize(c->s));
+  int f[FLAG_COUNT];
+  HIPCHK(c, hipMemcpy(f, c->L.b.flags, sizeof(f), hipMemcpyDeviceToHost));
+  if (overflow) *overflow = f[FLAG_LIST_OVERFLOW] != 0;
+  if (f[FLAG_LIST_OVERFLOW] && !overflow)
+    return fail(c, RS_E_OVERFLOW, "molecule neighbour list capacity exceeded");
+  if (f[FLAG_NONFINITE]) return fail(c, RS_E_NONFINITE, "non-finite positions, velocities or forces");
+  if (f[FLAG_STALE])
+    return fail(c, RS_E_STALE, "a molecule moved more than skin/2 between neighbour-list builds");
+  if (f[FLAG_HEXTENT])
+    return fail(c, RS_E_STALE, "an O-H distance exceeded the extent assumed by the list radius");
+  return 0;
+}
+
+int grow_list(rs_ctx* c) {
+  c->L.p.capc *= 2;
+  return alloc_list(c);
+}
+
+double kT_of(const rs_ctx* c) { return c->cfg.kB * c->cfg.temperature; }
+
+int forward_impl(rs_ctx* c, const double* x0, const double* v0, uint64_t seed, int n_steps,
+                 double* x_out, double* v_out, double* loss_out) {
+  Plan P;
+  int r = make_plan(c, n_steps, &P);
+  if (r) return r;
+  if (c->ext_steps && c->ext_steps < n_steps)
+    return fail(c, RS_E_CONFIG, "external noise holds fewer steps than requested");
+  DevBufs& b = c->L.b;
+  const rs_config& g = c->cfg;
+  const double dt = g.dt, e = std::exp(-g.gamma * dt), s = std::sqrt(1.0 - e * e), kT = kT_of(c);
+  free_records(c);
+  if (P.msd_on && (r = ensure_frames(c, P.nf))) return r;
+  for (int t = 0; t <= n_steps; ++t)
+    if (P.snap_at(t)) {
+      double *sx, *sv;
+      if ((r = dalloc(c, &sx, c->N3)) || (r = dalloc(c, &sv, c->N3))) return r;
+      c->snap_steps.push_back(t);
+      c->snap_x.push_back(sx);
+      c->snap_v.push_back(sv);
+    }
+  for (int attempt = 0;; ++attempt) {
+    HIPCHK(c, hipMemcpyAsync(b.x, x0, sizeof(double) * c->N3, hipMemcpyHostToDevice, c->s));
+    HIPCHK(c, hipMemcpyAsync(b.v, v0, sizeof(double) * c->N3, hipMemcpyHostToDevice, c->s));
+    if ((r = reset_flags(c))) return r;
+    HIPCHK(c, hipMemsetAsync(b.scal, 0, sizeof(double) * 16, c->s));
+    launch_wrap_state(c->L);
+    rebuild(c, false);
+    size_t snap = 0;
+    auto take_snap = [&](int t) {
+      if (snap < c->snap_steps.size() && c->snap_steps[snap] == t) {
+        hipMemcpyAsync(c->snap_x[snap], b.x, sizeof(double) * c->N3, hipMemcpyDeviceToDevice, c->s);
+        hipMemcpyAsync(c->snap_v[snap], b.v, sizeof(double) * c->N3, hipMemcpyDeviceToDevice, c->s);
+        ++snap;
+      }
+    };
+    take_snap(0);
+    for (int n = 1; n <= n_steps; ++n) {
+      launch_pair(c->L, false, nullptr, 0, 0, nullptr, 0, 0);
+      launch_step_fwd(c->L, n, seed, dt, e, s, kT);
+      if (n % g.nl_every == 0) rebuild(c, false);
+      take_snap(n);
+      if (n % P.n_loss == 0 && g.w_rdf != 0.0)
+        launch_rdf(c->L, g.rdf_bins, g.rdf_rmax, g.rdf_width, g.rdf_window, g.w_rdf / P.n_blocks,
+                   false, b.scal + 0);
+      int f = P.frame_of(n);
+      if (f >= 0) launch_frame_copy(c->L, f);
+    }
+    if (P.msd_on) launch_msd(c->L, P.nf, (P.nf / 2) * g.msd_every * dt, g.w_msd, g.msd_target, b.scal + 0);
+    bool ovf = false;
+    if ((r = finish(c, &ovf))) return r;
+    if (ovf) {
+      if (attempt >= 4) return fail(c, RS_E_OVERFLOW, "neighbour list capacity exceeded after retries");
+      if ((r = grow_list(c))) return r;
+      continue;
+    }
+    break;
+  }
+  double sc[4];
+  HIPCHK(c, hipMemcpy(sc, b.scal, sizeof(sc), hipMemcpyDeviceToHost));
+  c->msd_loss = P.msd_on ? sc[2] : 0.0;
+  if (x_out) HIPCHK(c, hipMemcpy(x_out, b.x, sizeof(double) * c->N3, hipMemcpyDeviceToHost));
+  if (v_out) HIPCHK(c, hipMemcpy(v_out, b.v, sizeof(double) * c->N3, hipMemcpyDeviceToHost));
+  if (loss_out) *loss_out = sc[0];
+  c->rec_valid = true;
+  c->rec_steps = n_steps;
+  c->rec_seed = seed;
+  return 0;
+}
+
+// Gradient seed at state t into b.grad: RDF block loss (weight w_rdf / n_blocks) plus the MSD frame
+// gradient; then reset or add into the accumulators.
+void seed_at(rs_ctx* c, const Plan& P, int t, bool with_rdf, bool reset, double e) {
+  const rs_config& g = c->cfg;
+  DevBufs& b = c->L.b;
+  launch_zero(c->L, b.grad, c->N3);
+  if (with_rdf && g.w_rdf != 0.0)
+    launch_rdf(c->L, g.rdf_bins, g.rdf_rmax, g.rdf_width, g.rdf_window, g.w_rdf / P.n_blocks, true,
+               b.scal + 0);
+  int f = P.frame_of(t);
+  if (f >= 0) launch_msd_inject(c->L, f);
+  launch_seed(c->L, reset, e);
+}
+
+int restore(rs_ctx* c, int t) {
+  for (size_t k = 0; k < c->snap_steps.size(); ++k)
+    if (c->snap_steps[k] == t) {
+      hipMemcpyAsync(c->L.b.x, c->snap_x[k], sizeof(double) * c->N3, hipMemcpyDeviceToDevice, c->s);
+      hipMemcpyAsync(c->L.b.v, c->snap_v[k], sizeof(double) * c->N3, hipMemcpyDeviceToDevice, c->s);
+      return 1;
+    }
+  return 0;
+}
+
+int reverse_impl(rs_ctx* c, const double* xN, const double* vN, uint64_t seed, int n_steps,
+                 double* loss_out, double* dth_out, double* x0_out, double* v0_out) {
+  if (!c->rec_valid || c->rec_steps != n_steps || c->rec_seed != seed)
+    return fail(c, RS_E_STATE, "rs_reverse needs the rs_forward of the same seed and n_steps first");
+  Plan P;
+  int r = make_plan(c, n_steps, &P);
+  if (r) return r;
+  DevBufs& b = c->L.b;
+  const rs_config& g = c->cfg;
+  const double dt = g.dt, e = std::exp(-g.gamma * dt), s = std::sqrt(1.0 - e * e), kT = kT_of(c);
+  const double cs[2] = {1.0 / g.mass_O, 1.0 / g.mass_H};
+  const double ws[2] = {dt * dt / (2.0 * g.mass_O), dt * dt / (2.0 * g.mass_H)};
+  const bool clip = g.clip_norm > 0.0 && std::isfinite(g.clip_norm);
+  const int nslots = pair_blocks(c->L.p);
+  for (int attempt = 0;; ++attempt) {
+    if (xN) HIPCHK(c, hipMemcpyAsync(b.x, xN, sizeof(double) * c->N3, hipMemcpyHostToDevice, c->s));
+    if (vN) HIPCHK(c, hipMemcpyAsync(b.v, vN, sizeof(double) * c->N3, hipMemcpyHostToDevice, c->s));
+    if ((r = reset_flags(c))) return r;
+    HIPCHK(c, hipMemsetAsync(b.dth, 0, sizeof(double) * 8, c->s));
+    HIPCHK(c, hipMemsetAsync(b.scal, 0, sizeof(double) * 16, c->s));
+    rebuild(c, false);
+    seed_at(c, P, n_steps, true, true, e);
+    int trunc_count = 0;
+    bool pending = false;
+    for (int n = n_steps; n >= 1; --n) {
+      ++trunc_count;
+      if (trunc_count <= P.n_trunc) {
+        launch_step_rev(c->L, pending, true, n, seed, dt, e, s, kT, nslots, pending && clip);
+        pending = false;
+        if (++c->since_build >= g.nl_every) rebuild(c, true);
+        launch_pair(c->L, true, b.C, cs[0], cs[1], b.A, ws[0], ws[1]);
+        if (clip) launch_hnorm(c->L, g.clip_norm);
+        pending = true;
+      }
+      const int t = n - 1;
+      if (t == 0) break;
+      const bool blk = t % P.n_loss == 0;
+      const bool logr = P.n_log > 0 && t % P.n_log == 0 && (trunc_count <= P.n_trunc || blk);
+      const bool frame = P.frame_of(t) >= 0;
+      if (!(blk || logr || frame)) continue;
+      if (pending) {
+        launch_step_rev(c->L, true, false, n, seed, dt, e, s, kT, nslots, clip);
+        pending = false;
+      }
+      bool restored = false;
+      if (logr) restored = restore(c, t);
+      if (blk) {
+        if (!restored && trunc_count > P.n_trunc) restored = restore(c, t);
+        trunc_count = 0;
+        rebuild(c, false);
+        seed_at(c, P, t, true, true, e);
+      } else if (frame) {
+        seed_at(c, P, t, false, false, e);
+      }
+    }
+    if (pending) launch_step_rev(c->L, true, false, 1, seed, dt, e, s, kT, nslots, clip);
+    bool ovf = false;
+    if ((r = finish(c, &ovf))) return r;
+    if (ovf) {
+      if (attempt >= 4) return fail(c, RS_E_OVERFLOW, "neighbour list capacity exceeded after retries");
+      if ((r = grow_list(c))) return r;
+      if (!xN || !vN) return fail(c, RS_E_OVERFLOW, "neighbour list overflow in a device-state reverse");
+      continue;
+    }
+    break;
+  }
+  double sc[4], th[8];
+  HIPCHK(c, hipMemcpy(sc, b.scal, sizeof(sc), hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(th, b.dth, sizeof(th), hipMemcpyDeviceToHost));
+  if (loss_out) *loss_out = sc[0] + c->msd_loss;
+  if (dth_out) std::memcpy(dth_out, th, sizeof(double) * 6);
+  if (x0_out) HIPCHK(c, hipMemcpy(x0_out, b.x, sizeof(double) * c->N3, hipMemcpyDeviceToHost));
+  if (v0_out) HIPCHK(c, hipMemcpy(v0_out, b.v, sizeof(double) * c->N3, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+// Single evaluation helpers: upload x, build the list (retrying on overflow), run `body`.
+template <class Body>
+int single_eval(rs_ctx* c, const double* x, bool with_u, Body body) {
+  DevBufs& b = c->L.b;
+  int r;
+  for (int attempt = 0;; ++attempt) {
+    HIPCHK(c, hipMemcpyAsync(b.x, x, sizeof(double) * c->N3, hipMemcpyHostToDevice, c->s));
+    if ((r = reset_flags(c))) return r;
+    rebuild(c, with_u);
+    body();
+    bool ovf = false;
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    int f[FLAG_COUNT];
+    HIPCHK(c, hipMemcpy(f, b.flags, sizeof(f), hipMemcpyDeviceToHost));
+    ovf = f[FLAG_LIST_OVERFLOW] != 0;
+    if (f[FLAG_NONFINITE]) return fail(c, RS_E_NONFINITE, "non-finite forces");
+    if (ovf) {
+      if (attempt >= 4) return fail(c, RS_E_OVERFLOW, "neighbour list capacity exceeded");
+      if ((r = grow_list(c))) return r;
+      continue;
+    }
+    return 0;
+  }
+}
+
+int check_ctx(rs_ctx* c) {
+  if (!c) return fail(nullptr, RS_E_CONFIG, "null context");
+  HIPCHK(c, hipSetDevice(c->device));
+  return 0;
+}
+
+}  // namespace
+
+// ============================================================================ C ABI
+
+extern "C" {
+
+int rs_version(void) { return RS_ABI_VERSION; }
+int32_t rs_config_size(void) { return (int32_t)sizeof(rs_config); }
+
+void rs_default_config(rs_config* g) {
+  std::memset(g, 0, sizeof(*g));
+  g->abi_version = RS_ABI_VERSION;
+  g->n_molecules = 216;
+  g->box[0] = g->box[1] = g->box[2] = 1.862;
+  g->cutoff = 0.8;
+  g->skin = 0.1;
+  g->nl_every = 10;
+  g->precision = RS_PRECISION_FP32;
+  g->dt = 0.001;
+  g->gamma = 1.0;
+  g->temperature = 300.0;
+  g->eps_rf = 78.3;
+  g->coulomb_k = 138.935457644;
+  g->kB = 0.00831446262;
+  g->mass_O = 15.99943;
+  g->mass_H = 1.007947;
+  g->theta0 = 1.82421813418;
+  g->rdf_bins = 200;
+  g->rdf_rmax = 0.8;
+  g->rdf_width = 0.0005;
+  g->rdf_window = 0.04;
+  g->w_rdf = 1.0;
+  g->msd_target = 2.3e-9;
+  g->clip_norm = INFINITY;
+}
+
+const char* rs_last_error(const rs_ctx* c) { return c ? c->err.c_str() : g_last_error.c_str(); }
+
+int rs_create(const rs_config* cfg, int32_t device, rs_ctx** out) {
+  if (!out) return fail(nullptr, RS_E_CONFIG, "null output pointer");
+  *out = nullptr;
+  if (!cfg || cfg->abi_version != RS_ABI_VERSION)
+    return fail(nullptr, RS_E_CONFIG, "abi_version mismatch");
+  const rs_config& g = *cfg;
+  if (g.n_molecules < 2) return fail(nullptr, RS_E_CONFIG, "n_molecules must be >= 2");
+  for (int k = 0; k < 3; ++k)
+    if (!(g.box[k] > 0.0) || g.cutoff > 0.5 * g.box[k])
+      return fail(nullptr, RS_E_CONFIG, "cutoff must be positive and <= half of every box edge");
+  if (!(g.cutoff > 0.0) || !(g.dt > 0.0) || !(g.gamma > 0.0) || g.nl_every < 1 || !(g.skin >= 0.0))
+    return fail(nullptr, RS_E_CONFIG, "cutoff, dt, gamma must be > 0, nl_every >= 1, skin >= 0");
+  if (g.precision != RS_PRECISION_FP32 && g.precision != RS_PRECISION_FP64)
+    return fail(nullptr, RS_E_CONFIG, "precision must be RS_PRECISION_FP64 or RS_PRECISION_FP32");
+  if (g.rdf_bins < 1 || g.rdf_bins > 1024 || !(g.rdf_rmax > 0.0) || !(g.rdf_width > 0.0) ||
+      !(g.rdf_window > 0.0))
+    return fail(nullptr, RS_E_CONFIG, "rdf_bins in [1,1024] and positive rdf_rmax/width/window required");
+  if (g.use_mlp) return fail(nullptr, RS_E_CONFIG, "use_mlp is not available in this build");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0)
+    return fail(nullptr, RS_E_NODEVICE, "no HIP device with that index");
+  rs_ctx* c = new rs_ctx();
+  c->cfg = g;
+  c->device = device;
+  c->M = g.n_molecules;
+  c->N = 3 * c->M;
+  c->N3 = 9 * c->M;
+  const double th0[6] = {0.315, 0.636, 0.417, 462750.4, 0.09572, 836.8};
+  std::memcpy(c->theta, th0, sizeof(th0));
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->s) != hipSuccess) {
+    delete c;
+    return fail(nullptr, RS_E_HIP, "hipSetDevice/hipStreamCreate failed");
+  }
+  c->L.s = c->s;
+  c->L.precision = g.precision;
+  c->L.p.CH = g.pair_chunks > 0 ? g.pair_chunks : 16;
+  if (c->L.p.CH != 8 && c->L.p.CH != 16) {
+    rs_destroy(c);
+    return fail(nullptr, RS_E_CONFIG, "pair_chunks must be 8 or 16");
+  }
+  set_params(c);
+  if (g.rdf_rmax + g.rdf_window > c->rlist) {
+    rs_destroy(c);
+    return fail(nullptr, RS_E_CONFIG, "rdf_rmax + rdf_window must not exceed the list radius");
+  }
+  c->L.p.capc = default_capc(c);
+  if (alloc_all(c)) {
+    std::string m = c->err;
+    rs_destroy(c);
+    return fail(nullptr, RS_E_HIP, m);
+  }
+  hipStreamSynchronize(c->s);
+  *out = c;
+  return 0;
+}
+
+int rs_destroy(rs_ctx* c) {
+  if (!c) return 0;
+  hipSetDevice(c->device);
+  if (c->s) hipStreamSynchronize(c->s);
+  free_records(c);
+  DevBufs& b = c->L.b;
+  void* ptrs[] = {b.x, b.v, b.F, b.h, b.A, b.B, b.C, b.D, b.grad, b.cell, b.cell_count,
+                  b.cell_start, b.cell_fill, b.tmp, b.order, b.rank, b.posq, b.pos64, b.uf,
+                  b.ud, b.O64, b.nbr, b.ncnt, b.flags, b.slots, b.dth, b.scal, b.hist, b.rdfG,
+                  b.target, b.rdf_g, b.frames, b.msd_del, b.msd_acc, b.msd_grad, b.ext_noise,
+                  b.hpart};
+  for (void* q : ptrs)
+    if (q) hipFree(q);
+  if (c->s) hipStreamDestroy(c->s);
+  delete c;
+  return 0;
+}
+
+int rs_set_theta(rs_ctx* c, const double* th) {
+  int r = check_ctx(c);
+  if (r) return r;
+  if (!th) return fail(c, RS_E_CONFIG, "null theta");
+  for (int k = 0; k < 6; ++k)
+    if (!std::isfinite(th[k])) return fail(c, RS_E_CONFIG, "non-finite theta");
+  if (!(th[0] > 0.0) || !(th[2] != 0.0) || !(th[4] > 0.0))
+    return fail(c, RS_E_CONFIG, "sigma_O and r0 must be > 0 and q_H != 0");
+  std::memcpy(c->theta, th, sizeof(double) * 6);
+  const int capc = c->L.p.capc;
+  set_params(c);
+  c->L.p.capc = capc;
+  if (c->cfg.rdf_rmax + c->cfg.rdf_window > c->rlist)
+    return fail(c, RS_E_CONFIG, "rdf_rmax + rdf_window must not exceed the list radius");
+  c->rec_valid = false;
+  return 0;
+}
+
+int rs_set_target_rdf(rs_ctx* c, const double* t, int32_t n) {
+  int r = check_ctx(c);
+  if (r) return r;
+  if (!t || n != c->cfg.rdf_bins) return fail(c, RS_E_CONFIG, "target must hold rdf_bins values");
+  HIPCHK(c, hipMemcpy(c->L.b.target, t, sizeof(double) * n, hipMemcpyHostToDevice));
+  return 0;
+}
+
+int rs_set_external_noise(rs_ctx* c, const double* xi, int32_t n_steps) {
+  int r = check_ctx(c);
+  if (r) return r;
+  if (c->L.b.ext_noise) hipFree(c->L.b.ext_noise);
+  c->L.b.ext_noise = nullptr;
+  c->ext_steps = 0;
+  c->rec_valid = false;
+  if (!xi) return 0;
+  if (n_steps < 1) return fail(c, RS_E_CONFIG, "n_steps must be >= 1");
+  size_t n = (size_t)n_steps * c->N3;
+  HIPCHK(c, hipMalloc((void**)&c->L.b.ext_noise, n * sizeof(double)));
+  HIPCHK(c, hipMemcpy(c->L.b.ext_noise, xi, n * sizeof(double), hipMemcpyHostToDevice));
+  c->ext_steps = n_steps;
+  return 0;
+}
+
+int rs_forward(rs_ctx* c, const double* x0, const double* v0, uint64_t seed, int32_t n_steps,
+               double* x_out, double* v_out, double* loss_out) {
+  int r = check_ctx(c);
+  if (r) return r;
+  if (!x0 || !v0) return fail(c, RS_E_CONFIG, "null x0/v0");
+  return forward_impl(c, x0, v0, seed, n_steps, x_out, v_out, loss_out);
+}
+
+int rs_reverse(rs_ctx* c, const double* xN, const double* vN, uint64_t seed, int32_t n_steps,
+               double* loss_out, double* dth, double* x0_out, double* v0_out) {
+  int r = check_ctx(c);
+  if (r) return r;
+  if (!xN || !vN) return fail(c, RS_E_CONFIG, "null xN/vN");
+  return reverse_impl(c, xN, vN, seed, n_steps, loss_out, dth, x0_out, v0_out);
+}
+
+int rs_value_and_grad(rs_ctx* c, const double* x0, const double* v0, uint64_t seed,
+                      int32_t n_steps, double* loss_out, double* dth, float* dL_dmlp) {
+  int r = check_ctx(c);
+  if (r) return r;
+  if (!x0 || !v0) return fail(c, RS_E_CONFIG, "null x0/v0");
+  std::vector<double> xN(c->N3), vN(c->N3);
+  if ((r = forward_impl(c, x0, v0, seed, n_steps, xN.data(), vN.data(), nullptr))) return r;
+  (void)dL_dmlp;
+  return reverse_impl(c, xN.data(), vN.data(), seed, n_steps, loss_out, dth, nullptr, nullptr);
+}
+
+int rs_forces(rs_ctx* c, const double* x, double* F) {
+  int r = check_ctx(c);
+  if (r) return r;
+  if (!x || !F) return fail(c, RS_E_CONFIG, "null argument");
+  if ((r = single_eval(c, x, false, [&] { launch_pair(c->L, false, nullptr, 0, 0, nullptr, 0, 0); })))
+    return r;
+  HIPCHK(c, hipMemcpy(F, c->L.b.F, sizeof(double) * c->N3, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int rs_adjoint_terms(rs_ctx* c, const double* x, const double* cc, const double* w, double* F,
+                     double* h, double* vjp6) {
+  int r = check_ctx(c);
+  if (r) return r;
+  if (!x || !cc || !w) return fail(c, RS_E_CONFIG, "null argument");
+  DevBufs& b = c->L.b;
+  HIPCHK(c, hipMemcpy(b.C, cc, sizeof(double) * c->N3, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(b.A, w, sizeof(double) * c->N3, hipMemcpyHostToDevice));
+  const double cs[2] = {1.0 / c->cfg.mass_O, 1.0 / c->cfg.mass_H};
+  if ((r = single_eval(c, x, true, [&] { launch_pair(c->L, true, b.C, cs[0], cs[1], b.A, 1.0, 1.0); })))
+    return r;
+  if (F) HIPCHK(c, hipMemcpy(F, b.F, sizeof(double) * c->N3, hipMemcpyDeviceToHost));
+  if (h) HIPCHK(c, hipMemcpy(h, b.h, sizeof(double) * c->N3, hipMemcpyDeviceToHost));
+  if (vjp6) {
+    const int ns = pair_blocks(c->L.p);
+    c->slot_host.resize((size_t)ns * 8);
+    HIPCHK(c, hipMemcpy(c->slot_host.data(), b.slots, sizeof(double) * ns * 8, hipMemcpyDeviceToHost));
+    for (int k = 0; k < 6; ++k) {
+      double s = 0.0;
+      for (int q = 0; q < ns; ++q) s += c->slot_host[(size_t)q * 8 + k];
+      vjp6[k] = s;
+    }
+  }
+  return 0;
+}
+
+int rs_noise(rs_ctx* c, uint64_t seed, int32_t step, double* xi) {
+  int r = check_ctx(c);
+  if (r) return r;
+  if (!xi || step < 1) return fail(c, RS_E_CONFIG, "step must be >= 1 and xi non-null");
+  if (c->ext_steps && step > c->ext_steps) return fail(c, RS_E_CONFIG, "step beyond external noise");
+  launch_noise(c->L, step, seed, kT_of(c), c->L.b.grad);
+  HIPCHK(c, hipStreamSynchronize(c->s));
+  HIPCHK(c, hipMemcpy(xi, c->L.b.grad, sizeof(double) * c->N3, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int rs_rdf(rs_ctx* c, const double* x, double* g, int64_t* hist, double* loss, double* dLdx) {
+  int r = check_ctx(c);
+  if (r) return r;
+  if (!x) return fail(c, RS_E_CONFIG, "null x");
+  const rs_config& cf = c->cfg;
+  DevBufs& b = c->L.b;
+  if ((r = single_eval(c, x, false, [&] {
+         launch_zero(c->L, b.grad, c->N3);
+         launch_rdf(c->L, cf.rdf_bins, cf.rdf_rmax, cf.rdf_width, cf.rdf_window, 1.0, dLdx != nullptr,
+                    nullptr);
+       })))
+    return r;
+  const int nb = cf.rdf_bins;
+  if (g) HIPCHK(c, hipMemcpy(g, b.rdf_g, sizeof(double) * nb, hipMemcpyDeviceToHost));
+  if (hist) HIPCHK(c, hipMemcpy(hist, b.hist, sizeof(int64_t) * nb, hipMemcpyDeviceToHost));
+  if (loss) HIPCHK(c, hipMemcpy(loss, b.scal + 1, sizeof(double), hipMemcpyDeviceToHost));
+  if (dLdx) HIPCHK(c, hipMemcpy(dLdx, b.grad, sizeof(double) * c->N3, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int rs_msd(rs_ctx* c, const double* frames, int32_t nf, double* D, double* loss, double* dLdf) {
+  int r = check_ctx(c);
+  if (r) return r;
+  if (!frames || nf < 2 || nf % 2 || nf / 2 > 64)
+    return fail(c, RS_E_CONFIG, "frames must hold an even number of frames in [2, 128]");
+  if (c->cfg.msd_every < 1) return fail(c, RS_E_CONFIG, "msd_every must be >= 1");
+  if ((r = ensure_frames(c, nf))) return r;
+  DevBufs& b = c->L.b;
+  const size_t n = (size_t)nf * 3 * c->M;
+  HIPCHK(c, hipMemcpy(b.frames, frames, sizeof(double) * n, hipMemcpyHostToDevice));
+  launch_msd(c->L, nf, (nf / 2) * c->cfg.msd_every * c->cfg.dt, c->cfg.w_msd, c->cfg.msd_target, nullptr);
+  HIPCHK(c, hipStreamSynchronize(c->s));
+  double sc[4];
+  HIPCHK(c, hipMemcpy(sc, b.scal, sizeof(sc), hipMemcpyDeviceToHost));
+  if (D) *D = sc[3];
+  if (loss) *loss = sc[2];
+  if (dLdf) HIPCHK(c, hipMemcpy(dLdf, b.msd_grad, sizeof(double) * n, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int rs_molecule_pairs(rs_ctx* c, const double* x, int32_t* pairs, int64_t capacity, int64_t* n_pairs) {
+  int r = check_ctx(c);
+  if (r) return r;
+  if (!x || !n_pairs || (capacity > 0 && !pairs)) return fail(c, RS_E_CONFIG, "null argument");
+  if ((r = single_eval(c, x, false, [] {}))) return r;
+  const SysParams& p = c->L.p;
+  const size_t lanes = (size_t)pair_blocks(p) * RS_BLOCK;
+  std::vector<int> nbr(lanes * p.capc), cnt(lanes), order(c->M);
+  HIPCHK(c, hipMemcpy(nbr.data(), c->L.b.nbr, sizeof(int) * nbr.size(), hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(cnt.data(), c->L.b.ncnt, sizeof(int) * cnt.size(), hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(order.data(), c->L.b.order, sizeof(int) * c->M, hipMemcpyDeviceToHost));
+  std::vector<std::pair<int, int>> out;
+  for (int i = 0; i < c->M; ++i)
+    for (int ch = 0; ch < p.CH; ++ch) {
+      const size_t gt = (size_t)i * p.CH + ch;
+      for (int k = 0; k < cnt[gt]; ++k) {
+        int j = nbr[((gt / RS_WAVE) * p.capc + k) * RS_WAVE + (gt % RS_WAVE)];
+        int a = order[i], bb = order[j];
+        if (a < bb) out.emplace_back(a, bb);
+      }
+    }
+  std::sort(out.begin(), out.end());
+  *n_pairs = (int64_t)out.size();
+  const int64_t nw = std::min<int64_t>(capacity, (int64_t)out.size());
+  for (int64_t k = 0; k < nw; ++k) {
+    pairs[2 * k] = out[k].first;
+    pairs[2 * k + 1] = out[k].second;
+  }
+  if ((int64_t)out.size() > capacity) return fail(c, RS_E_OVERFLOW, "pairs capacity too small");
+  return 0;
+}
+
+double rs_list_radius(const rs_ctx* c) { return c ? c->rlist : 0.0; }
+
+int rs_timings(rs_ctx* c, double* ms, int32_t n) {
+  int r = check_ctx(c);
+  if (r) return r;
+  for (int k = 0; k < n && k < RS_PHASE_COUNT; ++k) {
+    ms[k] = c->ms[k];
+    c->ms[k] = 0.0;
+  }
+  return 0;
+}
+
+int rs_enable_timing(rs_ctx* c, int32_t on) {
+  int r = check_ctx(c);
+  if (r) return r;
+  c->timing = on != 0;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,168 @@
+// common.h — device-side definitions shared by the librevsim kernels (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/revsim.h"
+
+#define RS_WAVE 64
+#define RS_BLOCK 256
+
+// Error/diagnostic flag slots (device int32 array, checked by the host at the end of a call).
+enum {
+  FLAG_LIST_OVERFLOW = 0,  // a molecule's sub-list exceeded its capacity
+  FLAG_STALE = 1,          // an O moved more than skin/2 since the last list build
+  FLAG_NONFINITE = 2,      // NaN/Inf in positions, velocities or forces
+  FLAG_HEXTENT = 3,        // an O-H distance exceeded the extent assumed by the list radius
+  FLAG_COUNT = 8
+};
+
+// Everything a kernel needs about the system, passed by value.
+struct SysParams {
+  int M, N;
+  double L[3], invL[3], qscale[3];
+  float unitf[3];
+  double rc2, krf, kc;
+  float rc2f, krff;
+  // θ-derived
+  double sig, eps, qH, qO, kb, r0, ka, theta0;
+  double kqq[3];   // k*qO*qO, k*qO*qH, k*qH*qH
+  float kqqf[3];
+  float sig2f, eps24f;
+  double mO, mH;
+  // list
+  double rlist2, skin_half2, hext;
+  int nc[3], ncell, brute;
+  int CH, capc;
+};
+
+// ----------------------------------------------------------------------------------------------
+// Pair membership. The two rules below are part of the library's definition of "which pairs
+// interact" and are evaluated exactly as written (no contraction) so that the CPU oracle,
+// compiled with -ffp-contract=off, makes identical decisions.
+//   FP64: dx = x_b - x_a; dx -= L*rint(dx*(1/L)); r2 = fma(dz,dz,fma(dy,dy,dx*dx))
+//   FP32: dxi = (int32)(q_b - q_a) on uint32 positions q = llrint(x*2^32/L);
+//         dxf = (float)dxi * (float)(L/2^32); r2f = fmaf(dzf,dzf,fmaf(dyf,dyf,dxf*dxf))
+// ----------------------------------------------------------------------------------------------
+__device__ __forceinline__ double mic1(double d, double L, double invL) {
+#pragma clang fp contract(off)
+  return d - L * rint(d * invL);
+}
+
+__device__ __forceinline__ double r2_of(double dx, double dy, double dz) {
+#pragma clang fp contract(off)
+  return fma(dz, dz, fma(dy, dy, dx * dx));
+}
+
+__device__ __forceinline__ float r2f_of(float dx, float dy, float dz) {
+#pragma clang fp contract(off)
+  return fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+}
+
+__device__ __forceinline__ float qdiff(uint32_t qa, uint32_t qb, float unit) {
+#pragma clang fp contract(off)
+  return (float)(int32_t)(qb - qa) * unit;
+}
+
+__device__ __forceinline__ uint32_t quant1(double x, double s) {
+#pragma clang fp contract(off)
+  return (uint32_t)(int64_t)rint(x * s);
+}
+
+__device__ __forceinline__ double wrap1(double x, double L) {
+#pragma clang fp contract(off)
+  return x - floor(x / L) * L;
+}
+
+// ----------------------------------------------------------------------------------------------
+// exp(t) for t <= 0: Cody-Waite reduction t = k ln2 + f with fdlibm's two-part ln2, Taylor
+// polynomial in f by fused multiply-adds, exact scaling by 2^k. Used for the RDF smoothing
+// weights so that histogram contributions are reproducible bit for bit on the host.
+// ----------------------------------------------------------------------------------------------
+__device__ __forceinline__ float rs_expf(float t) {
+#pragma clang fp contract(off)
+  float k = rintf(t * 1.44269504088896341f);
+  float f = fmaf(-k, 0.693145751953125f, t);
+  f = fmaf(-k, 1.428606765330187045e-06f, f);
+  float p = 1.0f / 5040.0f;
+  p = fmaf(p, f, 1.0f / 720.0f);
+  p = fmaf(p, f, 1.0f / 120.0f);
+  p = fmaf(p, f, 1.0f / 24.0f);
+  p = fmaf(p, f, 1.0f / 6.0f);
+  p = fmaf(p, f, 0.5f);
+  p = fmaf(p, f, 1.0f);
+  p = fmaf(p, f, 1.0f);
+  return ldexpf(p, (int)k);
+}
+
+__device__ __forceinline__ double rs_exp(double t) {
+#pragma clang fp contract(off)
+  double k = rint(t * 1.4426950408889634);
+  double f = fma(-k, 6.93147180369123816490e-01, t);
+  f = fma(-k, 1.90821492927058770002e-10, f);
+  double p = 1.0 / 6227020800.0;
+  p = fma(p, f, 1.0 / 479001600.0);
+  p = fma(p, f, 1.0 / 39916800.0);
+  p = fma(p, f, 1.0 / 3628800.0);
+  p = fma(p, f, 1.0 / 362880.0);
+  p = fma(p, f, 1.0 / 40320.0);
+  p = fma(p, f, 1.0 / 5040.0);
+  p = fma(p, f, 1.0 / 720.0);
+  p = fma(p, f, 1.0 / 120.0);
+  p = fma(p, f, 1.0 / 24.0);
+  p = fma(p, f, 1.0 / 6.0);
+  p = fma(p, f, 1.0 / 2.0);
+  p = fma(p, f, 1.0);
+  p = fma(p, f, 1.0);
+  return ldexp(p, (int)k);
+}
+
+// ----------------------------------------------------------------------------------------------
+// Philox4x64-10 (Salmon, Moraes, Dror, Shaw, SC'11), the generator numpy exposes as
+// np.random.Philox. Counter (step, atom, 0, 0), key (seed, 0); four 64-bit outputs give the three
+// normals of one atom by Box-Muller on 53-bit uniforms.
+// ----------------------------------------------------------------------------------------------
+__device__ __forceinline__ void philox4x64(uint64_t c[4], uint64_t k0, uint64_t k1) {
+  const uint64_t M0 = 0xD2E7470EE14C6C93ULL, M1 = 0xCA5A826395121157ULL;
+  const uint64_t W0 = 0x9E3779B97F4A7C15ULL, W1 = 0xBB67AE8584CAA73BULL;
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    uint64_t hi0 = __umul64hi(M0, c[0]), lo0 = M0 * c[0];
+    uint64_t hi1 = __umul64hi(M1, c[2]), lo1 = M1 * c[2];
+    uint64_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+    c[0] = n0;
+    c[1] = lo1;
+    c[2] = n2;
+    c[3] = lo0;
+    k0 += W0;
+    k1 += W1;
+  }
+}
+
+__device__ __forceinline__ void normals3(uint64_t seed, int step, int atom, double n[3]) {
+  uint64_t c[4] = {(uint64_t)(uint32_t)step, (uint64_t)atom, 0, 0};
+  philox4x64(c, seed, 0);
+  const double two53 = 1.0 / 9007199254740992.0;
+  double u1 = (double)((c[0] >> 11) + 1) * two53, u2 = (double)(c[1] >> 11) * two53;
+  double u3 = (double)((c[2] >> 11) + 1) * two53, u4 = (double)(c[3] >> 11) * two53;
+  double rad1 = sqrt(-2.0 * log(u1)), rad2 = sqrt(-2.0 * log(u3));
+  const double twopi = 6.283185307179586;
+  n[0] = rad1 * cos(twopi * u2);
+  n[1] = rad1 * sin(twopi * u2);
+  n[2] = rad2 * cos(twopi * u4);
+}
+
+// Wave-level reductions over lanes that share a molecule (CH consecutive lanes).
+template <int CH>
+__device__ __forceinline__ float chunk_sum(float v) {
+#pragma unroll
+  for (int o = CH / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, RS_WAVE);
+  return v;
+}
+template <int CH>
+__device__ __forceinline__ double chunk_sum(double v) {
+#pragma unroll
+  for (int o = CH / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, RS_WAVE);
+  return v;
+}

@@ -1,0 +1,63 @@
+// internal.h — host-side launch interface between the librevsim translation units.
+#pragma once
+
+#include "common.h"
+
+// Device buffers of one context. Arrays named "orig" are in the caller's atom order; arrays named
+// "sorted" are indexed by the cell-sorted molecule index of the last list build.
+struct DevBufs {
+  // state, original order, 3N doubles each
+  double *x, *v, *F, *h, *A, *B, *C, *D, *grad;
+  // cell sort
+  int *cell, *cell_count, *cell_start, *cell_fill, *tmp, *order, *rank;
+  // sorted pair inputs
+  int4* posq;      // FP32 mode: quantized positions, 3 per molecule
+  double4* pos64;  // FP64 mode: positions, 3 per molecule
+  float4* uf;      // FP32 mode: accum_C / m, 3 per molecule
+  double4* ud;     // FP64 mode: accum_C / m
+  double4* O64;    // O position at the last list build, sorted
+  // molecule list, ELL by wave: nbr[((gtid/64)*capc + k)*64 + gtid%64], gtid = i*CH + c
+  int *nbr, *ncnt;
+  int* flags;
+  double* slots;   // per-block θ-VJP partials [nblocks * 8]
+  double* dth;     // dL/dθ accumulator [8]
+  double* scal;    // device scalars: [0] loss, [1] rdf loss (last), [2] msd loss, [3] D, [4] clip scale
+  long long* hist; // RDF bins (fixed point 2^32)
+  double *rdfG, *target, *rdf_g;
+  double* frames;  // MSD frames [nf * 3M]
+  double* msd_del; // [K * M * 3]
+  long long* msd_acc; // [K] fixed point 2^40
+  double* msd_grad;   // dL/dframes [nf * 3M]
+  double* ext_noise;  // [steps * 3N] or null
+  double* hpart;      // h-norm partials
+};
+
+struct Launch {
+  SysParams p;
+  DevBufs b;
+  int precision;
+  hipStream_t s;
+};
+
+// list
+void launch_list_build(const Launch& L, bool with_u);
+// pair
+void launch_pair(const Launch& L, bool adjoint, const double* csrc, double cscale_O, double cscale_H,
+                 const double* wsrc, double wscale_O, double wscale_H);
+// integrator
+void launch_wrap_state(const Launch& L);
+void launch_step_fwd(const Launch& L, int step, uint64_t seed, double dt, double e, double s,
+                     double kT);
+void launch_step_rev(const Launch& L, bool do_r2, bool do_r1, int step, uint64_t seed, double dt,
+                     double e, double s, double kT, int nslots, bool clip);
+void launch_seed(const Launch& L, bool reset, double e);
+void launch_noise(const Launch& L, int step, uint64_t seed, double kT, double* xi);
+void launch_hnorm(const Launch& L, double clip);
+// losses
+void launch_rdf(const Launch& L, int nb, double rmax, double width, double window, double wgt,
+                bool grad, double* loss_dst);
+void launch_frame_copy(const Launch& L, int f);
+void launch_msd(const Launch& L, int nf, double lag, double w_msd, double target, double* loss_dst);
+void launch_msd_inject(const Launch& L, int f);
+void launch_zero(const Launch& L, double* a, size_t n);
+int pair_blocks(const SysParams& p);

@@ -1,0 +1,192 @@
+// list.hip — cell sort of molecules and the chunked molecule neighbour list.
+//
+// Role of Molly's find_neighbors / CellListMap (called at water/water.jl:1025,1038,1488,1504).
+// Membership is by molecule: j is listed for i when the O-O minimum-image distance at build time
+// is <= r_list = r_c + 2*h_ext + skin (FP64 rule, identical to the oracle's), which contains every
+// atom pair within r_c as long as each O moves less than skin/2 and each O-H stays below h_ext
+// until the next build (both checked every step in the pair kernel).
+//
+// The list is full (both i->j and j->i) so each molecule's forces are summed by its own lanes:
+// no atomics, deterministic. CH lanes cooperate on one molecule; lane c owns sub-list c, built
+// here by the same lane mapping the pair kernel uses (gtid = i*CH + c), in ELL-by-wave layout so
+// that the pair kernel's list reads are 256-byte coalesced.
+#include "internal.h"
+
+namespace {
+
+__device__ __forceinline__ int cell_index(const SysParams& p, double x, double y, double z) {
+  int c[3];
+  double w[3] = {x, y, z};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    int ck = (int)(wrap1(w[k], p.L[k]) * p.invL[k] * p.nc[k]);
+    c[k] = ck < 0 ? 0 : (ck >= p.nc[k] ? p.nc[k] - 1 : ck);
+  }
+  return (c[2] * p.nc[1] + c[1]) * p.nc[0] + c[0];
+}
+
+__global__ void k_zero_int(int* a, int n) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) a[t] = 0;
+}
+
+__global__ void k_cells_assign(SysParams p, const double* __restrict__ x, int* cell, int* count) {
+  int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= p.M) return;
+  int c = p.brute ? 0 : cell_index(p, x[9 * m], x[9 * m + 1], x[9 * m + 2]);
+  cell[m] = c;
+  atomicAdd(&count[c], 1);
+}
+
+// Exclusive scan of cell counts (one block); also clears the fill counters.
+__global__ void k_cells_scan(int ncell, const int* count, int* start, int* fill) {
+  __shared__ int part[RS_BLOCK];
+  int t = threadIdx.x;
+  int per = (ncell + RS_BLOCK - 1) / RS_BLOCK;
+  int b0 = t * per, b1 = min(ncell, b0 + per);
+  int s = 0;
+  for (int c = b0; c < b1; ++c) s += count[c];
+  part[t] = s;
+  __syncthreads();
+  if (t == 0) {
+    int acc = 0;
+    for (int u = 0; u < RS_BLOCK; ++u) {
+      int v = part[u];
+      part[u] = acc;
+      acc += v;
+    }
+    start[ncell] = acc;
+  }
+  __syncthreads();
+  int acc = part[t];
+  for (int c = b0; c < b1; ++c) {
+    start[c] = acc;
+    fill[c] = 0;
+    acc += count[c];
+  }
+}
+
+__global__ void k_cells_scatter(SysParams p, const int* cell, const int* start, int* fill, int* tmp) {
+  int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= p.M) return;
+  int c = cell[m];
+  int slot = atomicAdd(&fill[c], 1);
+  tmp[start[c] + slot] = m;
+}
+
+// Orders each cell's members by molecule index (one wave per cell): deterministic sorted order.
+__global__ void k_cells_order(SysParams p, const int* start, const int* tmp, int* order, int* rank) {
+  int wave = (blockIdx.x * blockDim.x + threadIdx.x) / RS_WAVE;
+  int lane = threadIdx.x & (RS_WAVE - 1);
+  if (wave >= p.ncell) return;
+  int s = start[wave], n = start[wave + 1] - s;
+  for (int t = lane; t < n; t += RS_WAVE) {
+    int m = tmp[s + t];
+    int r = 0;
+    for (int u = 0; u < n; ++u) r += (tmp[s + u] < m);
+    order[s + r] = m;
+    rank[m] = s + r;
+  }
+}
+
+// Sorted copies of the pair inputs for the current permutation.
+__global__ void k_pack(SysParams p, int precision, const int* __restrict__ order,
+                       const double* __restrict__ x, const double* __restrict__ C, int4* posq,
+                       double4* pos64, float4* uf, double4* ud, double4* O64) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;  // sorted atom slot 3*i + a
+  if (t >= p.N) return;
+  int i = t / 3, a = t - 3 * i;
+  int m = order[i];
+  const double* xa = x + 9 * m + 3 * a;
+  double x0 = xa[0], x1 = xa[1], x2 = xa[2];
+  if (precision == RS_PRECISION_FP32) {
+    posq[t] = make_int4((int)quant1(x0, p.qscale[0]), (int)quant1(x1, p.qscale[1]),
+                        (int)quant1(x2, p.qscale[2]), 0);
+  } else {
+    pos64[t] = make_double4(x0, x1, x2, 0.0);
+  }
+  if (a == 0) O64[i] = make_double4(x0, x1, x2, 0.0);
+  if (C) {
+    double im = 1.0 / (a == 0 ? p.mO : p.mH);
+    const double* ca = C + 9 * m + 3 * a;
+    if (precision == RS_PRECISION_FP32)
+      uf[t] = make_float4((float)(ca[0] * im), (float)(ca[1] * im), (float)(ca[2] * im), 0.f);
+    else
+      ud[t] = make_double4(ca[0] * im, ca[1] * im, ca[2] * im, 0.0);
+  }
+}
+
+template <int CH>
+__global__ __launch_bounds__(RS_BLOCK) void k_list_build(SysParams p, const int* __restrict__ start,
+                                                         const double4* __restrict__ O64, int* nbr,
+                                                         int* ncnt, int* flags) {
+  const int gtid = blockIdx.x * RS_BLOCK + threadIdx.x;
+  const int i = gtid / CH, c = gtid % CH;
+  if (i >= p.M) return;
+  const int lane = gtid & (RS_WAVE - 1);
+  int* out = nbr + (size_t)(gtid / RS_WAVE) * p.capc * RS_WAVE + lane;
+  const double4 oi = O64[i];
+  int n = 0;
+  auto test = [&](int j) {
+    if (j == i) return;
+    double4 oj = O64[j];
+    double dx = mic1(oj.x - oi.x, p.L[0], p.invL[0]);
+    double dy = mic1(oj.y - oi.y, p.L[1], p.invL[1]);
+    double dz = mic1(oj.z - oi.z, p.L[2], p.invL[2]);
+    if (r2_of(dx, dy, dz) <= p.rlist2) {
+      if (n < p.capc) out[(size_t)n * RS_WAVE] = j;
+      ++n;
+    }
+  };
+  if (p.brute) {
+    for (int j = c; j < p.M; j += CH) test(j);
+  } else {
+    int ci = cell_index(p, oi.x, oi.y, oi.z);
+    int cx = ci % p.nc[0], cy = (ci / p.nc[0]) % p.nc[1], cz = ci / (p.nc[0] * p.nc[1]);
+    int off = 0;
+    for (int dz = -1; dz <= 1; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          int ex = cx + dx, ey = cy + dy, ez = cz + dz;
+          ex += (ex < 0) ? p.nc[0] : (ex >= p.nc[0] ? -p.nc[0] : 0);
+          ey += (ey < 0) ? p.nc[1] : (ey >= p.nc[1] ? -p.nc[1] : 0);
+          ez += (ez < 0) ? p.nc[2] : (ez >= p.nc[2] ? -p.nc[2] : 0);
+          int e = (ez * p.nc[1] + ey) * p.nc[0] + ex;
+          int s = start[e], cnt = start[e + 1] - s;
+          int t0 = ((c - off) % CH + CH) % CH;
+          for (int t = t0; t < cnt; t += CH) test(s + t);
+          off += cnt;
+        }
+  }
+  if (n > p.capc) {
+    atomicOr(&flags[FLAG_LIST_OVERFLOW], 1);
+    n = p.capc;
+  }
+  ncnt[gtid] = n;
+}
+
+}  // namespace
+
+int pair_blocks(const SysParams& p) { return (p.M * p.CH + RS_BLOCK - 1) / RS_BLOCK; }
+
+void launch_list_build(const Launch& L, bool with_u) {
+  const SysParams& p = L.p;
+  const DevBufs& b = L.b;
+  int gm = (p.M + RS_BLOCK - 1) / RS_BLOCK;
+  k_zero_int<<<(p.ncell + 1 + RS_BLOCK - 1) / RS_BLOCK, RS_BLOCK, 0, L.s>>>(b.cell_count, p.ncell + 1);
+  k_cells_assign<<<gm, RS_BLOCK, 0, L.s>>>(p, b.x, b.cell, b.cell_count);
+  k_cells_scan<<<1, RS_BLOCK, 0, L.s>>>(p.ncell, b.cell_count, b.cell_start, b.cell_fill);
+  k_cells_scatter<<<gm, RS_BLOCK, 0, L.s>>>(p, b.cell, b.cell_start, b.cell_fill, b.tmp);
+  int gw = (p.ncell * RS_WAVE + RS_BLOCK - 1) / RS_BLOCK;
+  k_cells_order<<<gw, RS_BLOCK, 0, L.s>>>(p, b.cell_start, b.tmp, b.order, b.rank);
+  int ga = (p.N + RS_BLOCK - 1) / RS_BLOCK;
+  k_pack<<<ga, RS_BLOCK, 0, L.s>>>(p, L.precision, b.order, b.x, with_u ? b.C : nullptr, b.posq,
+                                    b.pos64, b.uf, b.ud, b.O64);
+  int gl = pair_blocks(p);
+  switch (p.CH) {
+    case 4: k_list_build<4><<<gl, RS_BLOCK, 0, L.s>>>(p, b.cell_start, b.O64, b.nbr, b.ncnt, b.flags); break;
+    case 8: k_list_build<8><<<gl, RS_BLOCK, 0, L.s>>>(p, b.cell_start, b.O64, b.nbr, b.ncnt, b.flags); break;
+    case 16: k_list_build<16><<<gl, RS_BLOCK, 0, L.s>>>(p, b.cell_start, b.O64, b.nbr, b.ncnt, b.flags); break;
+    default: k_list_build<32><<<gl, RS_BLOCK, 0, L.s>>>(p, b.cell_start, b.O64, b.nbr, b.ncnt, b.flags); break;
+  }
+}

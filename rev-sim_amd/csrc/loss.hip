@@ -1,0 +1,319 @@
+// loss.hip — O-O RDF loss and O mean-squared-displacement loss with their position gradients.
+//
+// RDF (rdf_differentiable / loss_rdf_part / rrule, water/water.jl:631-736), re-binned per the
+// config: bins x_b = (b+1/2)Δ, Δ = rmax/bins; each O-O distance r spreads weights
+// e_b = exp(-(x_b - r)^2/width) over the bins within ±window, normalised by their sum (Wang 2023
+// eq. 4/6 as restated by the reference); g_b = 2 pre/x_b^2 Σ_{i<j} φ_b(r_ij),
+// pre = V/(4π N_O^2 Δ); L = Σ_b (g_b - target_b)^2.
+// Bin sums are 64-bit fixed point (2^-32 per unit weight), accumulated with LDS and global integer
+// atomics: order-independent and identical to the oracle bit for bit (same r, same exp sequence).
+//
+// MSD (calculate_D / loss_fn_final, diffusion/diffusion.jl:861-911) on the O atoms of 2K frames:
+// D = (1/6K) Σ_s MSD(s, s+K)/(K Δt_frame) with unwrapped displacements from summed minimum-image
+// increments; L = w 1e18 (1e-6 D - target)^2; dL/dx_{frame} from the telescoping increments.
+#include "internal.h"
+
+namespace {
+
+struct RdfArgs {
+  int nb;
+  double rmax, width, window, wgt, pre;
+};
+
+// Window and per-bin weights for one distance, FP32 rule (bit-identical with the oracle).
+__device__ __forceinline__ void window32(const RdfArgs& r, float rf, int& lo, int& hi) {
+#pragma clang fp contract(off)
+  const float invd = (float)(r.nb / r.rmax), c = (float)r.window;
+  lo = (int)ceilf((rf - c) * invd - 0.5f);
+  hi = (int)floorf((rf + c) * invd - 0.5f);
+  if (lo < 0) lo = 0;
+  if (hi > r.nb - 1) hi = r.nb - 1;
+}
+__device__ __forceinline__ void window64(const RdfArgs& r, double rd, int& lo, int& hi) {
+#pragma clang fp contract(off)
+  const double invd = r.nb / r.rmax, c = r.window;
+  lo = (int)ceil((rd - c) * invd - 0.5);
+  hi = (int)floor((rd + c) * invd - 0.5);
+  if (lo < 0) lo = 0;
+  if (hi > r.nb - 1) hi = r.nb - 1;
+}
+
+__device__ __forceinline__ void contrib32(const RdfArgs& r, float rf, unsigned long long* lh) {
+#pragma clang fp contract(off)
+  int lo, hi;
+  window32(r, rf, lo, hi);
+  if (lo > hi) return;
+  const float dx = (float)(r.rmax / r.nb), iw = (float)(1.0 / r.width);
+  float den = 0.0f;
+  for (int b = lo; b <= hi; ++b) {
+    float t = ((float)b + 0.5f) * dx - rf;
+    den += rs_expf(-(t * t) * iw);
+  }
+  for (int b = lo; b <= hi; ++b) {
+    float t = ((float)b + 0.5f) * dx - rf;
+    float e = rs_expf(-(t * t) * iw);
+    long long v = __float2ll_rn((e / den) * 4294967296.0f);
+    atomicAdd(&lh[b], (unsigned long long)v);
+  }
+}
+
+__device__ __forceinline__ void contrib64(const RdfArgs& r, double rd, unsigned long long* lh) {
+#pragma clang fp contract(off)
+  int lo, hi;
+  window64(r, rd, lo, hi);
+  if (lo > hi) return;
+  const double dx = r.rmax / r.nb, iw = 1.0 / r.width;
+  double den = 0.0;
+  for (int b = lo; b <= hi; ++b) {
+    double t = ((double)b + 0.5) * dx - rd;
+    den += rs_exp(-(t * t) * iw);
+  }
+  for (int b = lo; b <= hi; ++b) {
+    double t = ((double)b + 0.5) * dx - rd;
+    double e = rs_exp(-(t * t) * iw);
+    long long v = __double2ll_rn((e / den) * 4294967296.0);
+    atomicAdd(&lh[b], (unsigned long long)v);
+  }
+}
+
+#define RDF_MAX_BINS 1024
+
+template <int CH>
+__global__ __launch_bounds__(RS_BLOCK) void k_rdf_hist(SysParams p, int precision, RdfArgs r,
+                                                        const int* __restrict__ nbr,
+                                                        const int* __restrict__ ncnt,
+                                                        const int4* __restrict__ posq,
+                                                        const double4* __restrict__ pos64,
+                                                        long long* hist) {
+  __shared__ unsigned long long lh[RDF_MAX_BINS];
+  for (int b = threadIdx.x; b < r.nb; b += RS_BLOCK) lh[b] = 0ull;
+  __syncthreads();
+  const int gtid = blockIdx.x * RS_BLOCK + threadIdx.x;
+  const int i = gtid / CH;
+  if (i < p.M) {
+    const int n = ncnt[gtid];
+    const int* L = nbr + (size_t)(gtid / RS_WAVE) * p.capc * RS_WAVE + (gtid & (RS_WAVE - 1));
+    for (int k = 0; k < n; ++k) {
+      const int j = L[(size_t)k * RS_WAVE];
+      if (j <= i) continue;
+      if (precision == RS_PRECISION_FP32) {
+        const int4 a = posq[3 * i], b = posq[3 * j];
+        float dx = qdiff((uint32_t)a.x, (uint32_t)b.x, p.unitf[0]);
+        float dy = qdiff((uint32_t)a.y, (uint32_t)b.y, p.unitf[1]);
+        float dz = qdiff((uint32_t)a.z, (uint32_t)b.z, p.unitf[2]);
+        contrib32(r, sqrtf(r2f_of(dx, dy, dz)), lh);
+      } else {
+        const double4 a = pos64[3 * i], b = pos64[3 * j];
+        double dx = mic1(b.x - a.x, p.L[0], p.invL[0]);
+        double dy = mic1(b.y - a.y, p.L[1], p.invL[1]);
+        double dz = mic1(b.z - a.z, p.L[2], p.invL[2]);
+        contrib64(r, sqrt(r2_of(dx, dy, dz)), lh);
+      }
+    }
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < r.nb; b += RS_BLOCK)
+    if (lh[b]) atomicAdd((unsigned long long*)&hist[b], lh[b]);
+}
+
+// g_b, L and dL/dg_b (one block). The loss sum runs over bins in order, as on the host.
+__global__ void k_rdf_final(RdfArgs r, const long long* hist, const double* target, double* g,
+                            double* G, double* scal, double* loss_dst) {
+#pragma clang fp contract(off)
+  const double dx = r.rmax / r.nb;
+  for (int b = threadIdx.x; b < r.nb; b += blockDim.x) {
+    double xb = ((double)b + 0.5) * dx;
+    double gb = 2.0 * r.pre / (xb * xb) * ((double)hist[b] / 4294967296.0);
+    g[b] = gb;
+    G[b] = 2.0 * (gb - target[b]) * 2.0 * r.pre / (xb * xb);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double L = 0.0;
+    for (int b = 0; b < r.nb; ++b) {
+      double d = g[b] - target[b];
+      L += d * d;
+    }
+    scal[1] = L;
+    if (loss_dst) *loss_dst += r.wgt * L;
+  }
+}
+
+// dL/dx_O (times the block weight) by owner-computes over the full list, FP64.
+template <int CH>
+__global__ __launch_bounds__(RS_BLOCK) void k_rdf_grad(SysParams p, int precision, RdfArgs r,
+                                                        const int* __restrict__ nbr,
+                                                        const int* __restrict__ ncnt,
+                                                        const int* __restrict__ order,
+                                                        const double* __restrict__ x,
+                                                        const int4* __restrict__ posq,
+                                                        const double* __restrict__ G,
+                                                        double* grad) {
+  const int gtid = blockIdx.x * RS_BLOCK + threadIdx.x;
+  const int i = gtid / CH, c = gtid % CH;
+  const bool act = i < p.M;
+  double acc[3] = {0.0, 0.0, 0.0};
+  const int mi = act ? order[i] : 0;
+  if (act) {
+    const double* xi = x + 9 * (size_t)mi;
+    const int n = ncnt[gtid];
+    const int* L = nbr + (size_t)(gtid / RS_WAVE) * p.capc * RS_WAVE + (gtid & (RS_WAVE - 1));
+    const double dxb = r.rmax / r.nb, iw = 1.0 / r.width, rmaxw = r.rmax + r.window;
+    for (int k = 0; k < n; ++k) {
+      const int j = L[(size_t)k * RS_WAVE];
+      const double* xj = x + 9 * (size_t)order[j];
+      double d[3];
+      for (int t = 0; t < 3; ++t) d[t] = mic1(xj[t] - xi[t], p.L[t], p.invL[t]);
+      const double rr = sqrt(r2_of(d[0], d[1], d[2]));
+      if (rr > rmaxw + 0.01) continue;
+      int lo, hi;
+      if (precision == RS_PRECISION_FP32) {
+        const int4 a = posq[3 * i], b = posq[3 * j];
+        float ex = qdiff((uint32_t)a.x, (uint32_t)b.x, p.unitf[0]);
+        float ey = qdiff((uint32_t)a.y, (uint32_t)b.y, p.unitf[1]);
+        float ez = qdiff((uint32_t)a.z, (uint32_t)b.z, p.unitf[2]);
+        window32(r, sqrtf(r2f_of(ex, ey, ez)), lo, hi);
+      } else {
+        window64(r, rr, lo, hi);
+      }
+      if (lo > hi) continue;
+      double den = 0.0, dden = 0.0, s1 = 0.0, s2 = 0.0;
+      for (int bb = lo; bb <= hi; ++bb) {
+        double t = ((double)bb + 0.5) * dxb - rr;
+        double e = rs_exp(-(t * t) * iw);
+        double de = e * 2.0 * t * iw;
+        den += e;
+        dden += de;
+        s1 += G[bb] * de;
+        s2 += G[bb] * e;
+      }
+      const double dLdr = s1 / den - dden * s2 / (den * den);
+      for (int t = 0; t < 3; ++t) acc[t] -= dLdr * d[t] / rr;
+    }
+  }
+  for (int t = 0; t < 3; ++t) acc[t] = chunk_sum<CH>(acc[t]);
+  if (act && c == 0) {
+    for (int t = 0; t < 3; ++t) {
+      grad[9 * (size_t)mi + t] = r.wgt * acc[t];
+      grad[9 * (size_t)mi + 3 + t] = 0.0;
+      grad[9 * (size_t)mi + 6 + t] = 0.0;
+    }
+  }
+}
+
+__global__ void k_zero_ll(long long* a, int n) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) a[t] = 0;
+}
+__global__ void k_zero_d(double* a, int n) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) a[t] = 0.0;
+}
+
+__global__ void k_frame_copy(SysParams p, const double* x, double* frames, int f) {
+  int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= p.M) return;
+  for (int k = 0; k < 3; ++k) frames[(size_t)f * 3 * p.M + 3 * m + k] = x[9 * (size_t)m + k];
+}
+
+__global__ void k_msd_del(SysParams p, int K, const double* frames, double* del, long long* acc) {
+  int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= p.M) return;
+  const size_t F = 3 * (size_t)p.M;
+  for (int s = 0; s < K; ++s) {
+    double a[3] = {0.0, 0.0, 0.0};
+    for (int f = s; f < s + K; ++f)
+      for (int k = 0; k < 3; ++k)
+        a[k] += mic1(frames[(f + 1) * F + 3 * m + k] - frames[f * F + 3 * m + k], p.L[k], p.invL[k]);
+    double sq = 0.0;
+    for (int k = 0; k < 3; ++k) {
+      del[((size_t)s * p.M + m) * 3 + k] = a[k];
+      sq += a[k] * a[k];
+    }
+    atomicAdd((unsigned long long*)&acc[s], (unsigned long long)__double2ll_rn(sq * 1099511627776.0));
+  }
+}
+
+__global__ void k_msd_final(SysParams p, int K, double lag, double w, double target,
+                            const long long* acc, double* scal, double* loss_dst) {
+  if (threadIdx.x != 0) return;
+  double Dsum = 0.0;
+  for (int s = 0; s < K; ++s) {
+    double msd = ((double)acc[s] / 1099511627776.0) / p.M;
+    Dsum += msd / (K * lag);
+  }
+  double D = Dsum / (6.0 * K);
+  double diff = 1e-6 * D - target;
+  double L = w * 1e18 * diff * diff;
+  scal[2] = L;
+  scal[3] = D;
+  scal[5] = (w * 1e18 * 2.0 * diff * 1e-6) * 2.0 / (6.0 * K * K * lag * p.M);
+  if (loss_dst) *loss_dst += L;
+}
+
+__global__ void k_msd_grad(SysParams p, int K, const double* del, const double* scal, double* out) {
+  const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;  // (f, m, k)
+  const size_t F = 3 * (size_t)p.M;
+  if (t >= 2 * (size_t)K * F) return;
+  const int f = (int)(t / F);
+  const size_t r = t - f * F;
+  const double c = scal[5];
+  out[t] = f >= K ? c * del[(size_t)(f - K) * F + r] : -(c * del[(size_t)f * F + r]);
+}
+
+__global__ void k_msd_inject(SysParams p, const double* mg, int f, double* grad) {
+  int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= p.M) return;
+  for (int k = 0; k < 3; ++k) grad[9 * (size_t)m + k] += mg[(size_t)f * 3 * p.M + 3 * m + k];
+}
+
+}  // namespace
+
+void launch_rdf(const Launch& L, int nb, double rmax, double width, double window, double wgt,
+                bool grad, double* loss_dst) {
+  const SysParams& p = L.p;
+  RdfArgs r;
+  r.nb = nb;
+  r.rmax = rmax;
+  r.width = width;
+  r.window = window;
+  r.wgt = wgt;
+  const double V = p.L[0] * p.L[1] * p.L[2];
+  const double dx = rmax / nb;
+  r.pre = V / (4.0 * M_PI * (double)p.M * (double)p.M * dx);
+  k_zero_ll<<<(nb + RS_BLOCK - 1) / RS_BLOCK, RS_BLOCK, 0, L.s>>>(L.b.hist, nb);
+  int g = pair_blocks(p);
+  if (p.CH == 8)
+    k_rdf_hist<8><<<g, RS_BLOCK, 0, L.s>>>(p, L.precision, r, L.b.nbr, L.b.ncnt, L.b.posq, L.b.pos64, L.b.hist);
+  else
+    k_rdf_hist<16><<<g, RS_BLOCK, 0, L.s>>>(p, L.precision, r, L.b.nbr, L.b.ncnt, L.b.posq, L.b.pos64, L.b.hist);
+  k_rdf_final<<<1, RS_BLOCK, 0, L.s>>>(r, L.b.hist, L.b.target, L.b.rdf_g, L.b.rdfG, L.b.scal, loss_dst);
+  if (grad) {
+    if (p.CH == 8)
+      k_rdf_grad<8><<<g, RS_BLOCK, 0, L.s>>>(p, L.precision, r, L.b.nbr, L.b.ncnt, L.b.order, L.b.x, L.b.posq, L.b.rdfG, L.b.grad);
+    else
+      k_rdf_grad<16><<<g, RS_BLOCK, 0, L.s>>>(p, L.precision, r, L.b.nbr, L.b.ncnt, L.b.order, L.b.x, L.b.posq, L.b.rdfG, L.b.grad);
+  }
+}
+
+void launch_zero(const Launch& L, double* a, size_t n) {
+  k_zero_d<<<(unsigned)((n + RS_BLOCK - 1) / RS_BLOCK), RS_BLOCK, 0, L.s>>>(a, (int)n);
+}
+
+void launch_frame_copy(const Launch& L, int f) {
+  k_frame_copy<<<(L.p.M + RS_BLOCK - 1) / RS_BLOCK, RS_BLOCK, 0, L.s>>>(L.p, L.b.x, L.b.frames, f);
+}
+
+void launch_msd(const Launch& L, int nf, double lag, double w_msd, double target, double* loss_dst) {
+  const SysParams& p = L.p;
+  const int K = nf / 2;
+  k_zero_ll<<<1, 64, 0, L.s>>>(L.b.msd_acc, K);
+  k_msd_del<<<(p.M + RS_BLOCK - 1) / RS_BLOCK, RS_BLOCK, 0, L.s>>>(p, K, L.b.frames, L.b.msd_del, L.b.msd_acc);
+  k_msd_final<<<1, 64, 0, L.s>>>(p, K, lag, w_msd, target, L.b.msd_acc, L.b.scal, loss_dst);
+  size_t tot = (size_t)nf * 3 * p.M;
+  k_msd_grad<<<(unsigned)((tot + RS_BLOCK - 1) / RS_BLOCK), RS_BLOCK, 0, L.s>>>(p, K, L.b.msd_del, L.b.scal, L.b.msd_grad);
+}
+
+void launch_msd_inject(const Launch& L, int f) {
+  k_msd_inject<<<(L.p.M + RS_BLOCK - 1) / RS_BLOCK, RS_BLOCK, 0, L.s>>>(L.p, L.b.msd_grad, f, L.b.grad);
+}

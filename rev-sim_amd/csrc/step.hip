@@ -1,0 +1,248 @@
+// step.hip — streaming kernels of the forward Langevin step and of the reverse-time adjoint.
+//
+// Forward step n (forward_sim loop, water/water.jl:1030-1037), FP64 state, one thread per atom:
+//   v += F dt/m; x += v dt/2; v = e v + s ξ_n; x += v dt/2; wrap       e = exp(-γdt), s = √(1-e²)
+// Reverse step (reverse_sim, water/water.jl:1496-1521), fused as [part 2 of step n+1][part 1 of n]:
+//   part 2: v -= F dt/m; B = e B + dt² h; A += B; dL/dθ += Σ_blocks VJP partials
+//   part 1: x -= v dt/2; v = (v - s ξ_n)/e; x -= v dt/2; wrap; D = e D + dt² h (1+e)/2; C += D
+// ξ_n is regenerated from the Philox counter (step n, atom) — the same bits forward and reverse.
+// Every thread also writes its atom's slot of the sorted pair inputs for the next pair kernel.
+// Expressions follow the reference's broadcast order and are evaluated without contraction.
+#include "internal.h"
+
+namespace {
+
+__device__ __forceinline__ void write_sorted(const SysParams& p, int precision, int slot,
+                                             const double x[3], int4* posq, double4* pos64) {
+  if (precision == RS_PRECISION_FP32)
+    posq[slot] = make_int4((int)quant1(x[0], p.qscale[0]), (int)quant1(x[1], p.qscale[1]),
+                           (int)quant1(x[2], p.qscale[2]), 0);
+  else
+    pos64[slot] = make_double4(x[0], x[1], x[2], 0.0);
+}
+
+__global__ void k_wrap(SysParams p, double* x) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 3 * p.N) return;
+  x[t] = wrap1(x[t], p.L[t % 3]);
+}
+
+__global__ __launch_bounds__(RS_BLOCK) void k_step_fwd(SysParams p, int precision, DevBufs b,
+                                                        int step, uint64_t seed, double dt,
+                                                        double e, double s, double kT) {
+#pragma clang fp contract(off)
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= p.N) return;
+  const int m = t / 3, a = t - 3 * m;
+  const double mass = a == 0 ? p.mO : p.mH;
+  double xi[3];
+  if (b.ext_noise) {
+    const double* src = b.ext_noise + (size_t)(step - 1) * 3 * p.N + 3 * t;
+    xi[0] = src[0];
+    xi[1] = src[1];
+    xi[2] = src[2];
+  } else {
+    double nn[3];
+    normals3(seed, step, t, nn);
+    const double sd = sqrt(kT / mass);
+    xi[0] = sd * nn[0];
+    xi[1] = sd * nn[1];
+    xi[2] = sd * nn[2];
+  }
+  double xn[3];
+  bool fin = true;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const size_t o = 3 * (size_t)t + k;
+    double v = b.v[o] + (b.F[o] * dt) / mass;
+    double x = b.x[o] + (v * dt) / 2.0;
+    v = v * e + xi[k] * s;
+    x = x + (v * dt) / 2.0;
+    x = wrap1(x, p.L[k]);
+    b.v[o] = v;
+    b.x[o] = x;
+    xn[k] = x;
+    fin = fin && isfinite(x) && isfinite(v);
+  }
+  if (!fin) atomicOr(&b.flags[FLAG_NONFINITE], 1);
+  write_sorted(p, precision, 3 * b.rank[m] + a, xn, b.posq, b.pos64);
+}
+
+__global__ __launch_bounds__(RS_BLOCK) void k_step_rev(SysParams p, int precision, DevBufs b,
+                                                        int do_r2, int do_r1, int step,
+                                                        uint64_t seed, double dt, double e,
+                                                        double s, double kT, int nslots, int clip) {
+#pragma clang fp contract(off)
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const double dt2 = dt * dt;
+  if (do_r2 && blockIdx.x == 0) {
+    // dL/dθ += per-block VJP partials, summed in block order (deterministic)
+    __shared__ double red[RS_BLOCK];
+    for (int k = 0; k < 6; ++k) {
+      double acc = 0.0;
+      for (int q = threadIdx.x; q < nslots; q += RS_BLOCK) acc += b.slots[(size_t)q * 8 + k];
+      red[threadIdx.x] = acc;
+      __syncthreads();
+      for (int w = RS_BLOCK / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+      }
+      if (threadIdx.x == 0) b.dth[k] += red[0];
+      __syncthreads();
+    }
+  }
+  if (t >= p.N) return;
+  const int m = t / 3, a = t - 3 * m;
+  const double mass = a == 0 ? p.mO : p.mH;
+  const double cs = clip ? b.scal[4] : 1.0;
+  double xi[3] = {0.0, 0.0, 0.0};
+  if (do_r1) {
+    if (b.ext_noise) {
+      const double* src = b.ext_noise + (size_t)(step - 1) * 3 * p.N + 3 * t;
+      xi[0] = src[0];
+      xi[1] = src[1];
+      xi[2] = src[2];
+    } else {
+      double nn[3];
+      normals3(seed, step, t, nn);
+      const double sd = sqrt(kT / mass);
+      xi[0] = sd * nn[0];
+      xi[1] = sd * nn[1];
+      xi[2] = sd * nn[2];
+    }
+  }
+  double xn[3], un[3];
+  bool fin = true;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const size_t o = 3 * (size_t)t + k;
+    double x = b.x[o], v = b.v[o], h = b.h[o];
+    if (clip) {
+      h = h * cs;
+      b.h[o] = h;
+    }
+    if (do_r2) {
+      v = v - (b.F[o] * dt) / mass;
+      double B = e * b.B[o] + dt2 * h;
+      b.B[o] = B;
+      b.A[o] = b.A[o] + B;
+    }
+    if (do_r1) {
+      x = x - (v * dt) / 2.0;
+      v = (v - xi[k] * s) / e;
+      x = x - (v * dt) / 2.0;
+      x = wrap1(x, p.L[k]);
+      double D = e * b.D[o] + ((dt2 * h) * (1.0 + e)) / 2.0;
+      double C = b.C[o] + D;
+      b.D[o] = D;
+      b.C[o] = C;
+      un[k] = C / mass;
+    }
+    b.x[o] = x;
+    b.v[o] = v;
+    xn[k] = x;
+    fin = fin && isfinite(x) && isfinite(v);
+  }
+  if (!fin) atomicOr(&b.flags[FLAG_NONFINITE], 1);
+  if (do_r1) {
+    const int slot = 3 * b.rank[m] + a;
+    write_sorted(p, precision, slot, xn, b.posq, b.pos64);
+    if (precision == RS_PRECISION_FP32)
+      b.uf[slot] = make_float4((float)un[0], (float)un[1], (float)un[2], 0.f);
+    else
+      b.ud[slot] = make_double4(un[0], un[1], un[2], 0.0);
+  }
+}
+
+// Gradient seed g (b.grad) into the accumulators: reset (A = B = g(1+e), C = 0,
+// D = g(1+2e+e²)/(2e), h = 0; water/water.jl:1480-1483,1540-1545) or additive (same terms added).
+__global__ void k_seed(SysParams p, DevBufs b, int reset, double e) {
+#pragma clang fp contract(off)
+  const int o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= 3 * p.N) return;
+  const double g = b.grad[o];
+  const double ab = g * (1.0 + e);
+  const double d = g * (1.0 + 2.0 * e + e * e) / (2.0 * e);
+  if (reset) {
+    b.A[o] = ab;
+    b.B[o] = ab;
+    b.C[o] = 0.0;
+    b.D[o] = d;
+    b.h[o] = 0.0;
+  } else {
+    b.A[o] += ab;
+    b.B[o] += ab;
+    b.D[o] += d;
+  }
+}
+
+__global__ void k_noise(SysParams p, int step, uint64_t seed, double kT, const double* ext,
+                        double* xi) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= p.N) return;
+  if (ext) {
+    for (int k = 0; k < 3; ++k) xi[3 * t + k] = ext[(size_t)(step - 1) * 3 * p.N + 3 * t + k];
+    return;
+  }
+  const double mass = (t % 3 == 0) ? p.mO : p.mH;
+  double nn[3];
+  normals3(seed, step, t, nn);
+  const double sd = sqrt(kT / mass);
+  for (int k = 0; k < 3; ++k) xi[3 * t + k] = sd * nn[k];
+}
+
+// ||h||: per-block partial sums, then one block sums them in order and sets the clip scale.
+__global__ void k_hnorm_part(SysParams p, const double* h, double* part) {
+  __shared__ double red[RS_BLOCK];
+  double acc = 0.0;
+  for (int o = blockIdx.x * RS_BLOCK + threadIdx.x; o < 3 * p.N; o += gridDim.x * RS_BLOCK)
+    acc += h[o] * h[o];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = RS_BLOCK / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void k_hnorm_final(int nparts, const double* part, double clip, double* scal) {
+  if (threadIdx.x != 0) return;
+  double s = 0.0;
+  for (int q = 0; q < nparts; ++q) s += part[q];
+  double nn = sqrt(s);
+  scal[4] = nn > clip ? clip / nn : 1.0;
+}
+
+}  // namespace
+
+void launch_wrap_state(const Launch& L) {
+  k_wrap<<<(3 * L.p.N + RS_BLOCK - 1) / RS_BLOCK, RS_BLOCK, 0, L.s>>>(L.p, L.b.x);
+}
+
+void launch_step_fwd(const Launch& L, int step, uint64_t seed, double dt, double e, double s,
+                     double kT) {
+  k_step_fwd<<<(L.p.N + RS_BLOCK - 1) / RS_BLOCK, RS_BLOCK, 0, L.s>>>(L.p, L.precision, L.b, step,
+                                                                       seed, dt, e, s, kT);
+}
+
+void launch_step_rev(const Launch& L, bool do_r2, bool do_r1, int step, uint64_t seed, double dt,
+                     double e, double s, double kT, int nslots, bool clip) {
+  k_step_rev<<<(L.p.N + RS_BLOCK - 1) / RS_BLOCK, RS_BLOCK, 0, L.s>>>(
+      L.p, L.precision, L.b, do_r2, do_r1, step, seed, dt, e, s, kT, nslots, clip);
+}
+
+void launch_seed(const Launch& L, bool reset, double e) {
+  k_seed<<<(3 * L.p.N + RS_BLOCK - 1) / RS_BLOCK, RS_BLOCK, 0, L.s>>>(L.p, L.b, reset, e);
+}
+
+void launch_noise(const Launch& L, int step, uint64_t seed, double kT, double* xi) {
+  k_noise<<<(L.p.N + RS_BLOCK - 1) / RS_BLOCK, RS_BLOCK, 0, L.s>>>(L.p, step, seed, kT,
+                                                                    L.b.ext_noise, xi);
+}
+
+void launch_hnorm(const Launch& L, double clip) {
+  const int parts = 256;
+  k_hnorm_part<<<parts, RS_BLOCK, 0, L.s>>>(L.p, L.b.h, L.b.hpart);
+  k_hnorm_final<<<1, 64, 0, L.s>>>(parts, L.b.hpart, clip, L.b.scal);
+}

@@ -1,0 +1,149 @@
+"""Simulator: a thin Python owner of one librevsim context (one GPU, one stream).
+
+Every method calls the HIP library through its C ABI; there is no CPU code path behind it. A
+missing library raises LibraryMissing, a failing call raises RevSimError with the library's
+message.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _abi
+from ._abi import RsConfig, dptr, fptr, lptr
+
+
+class RevSimError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{_abi.ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Simulator:
+    def __init__(self, cfg: RsConfig, device: int = 0, lib=None):
+        self.lib = lib or _abi.load_library()
+        self.cfg = cfg
+        h = ctypes.c_void_p()
+        rc = self.lib.rs_create(ctypes.byref(cfg), int(device), ctypes.byref(h))
+        if rc:
+            raise RevSimError(rc, self.lib.rs_last_error(None).decode())
+        self.h = h
+        self.M = cfg.n_molecules
+        self.N3 = 9 * self.M
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.rs_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _chk(self, rc):
+        if rc:
+            raise RevSimError(rc, self.lib.rs_last_error(self.h).decode())
+
+    def _vec(self, a, n=None):
+        a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+        if n is not None and a.size != n:
+            raise ValueError(f"expected {n} values, got {a.size}")
+        return a
+
+    # -- parameters -------------------------------------------------------------------------
+    def set_theta(self, theta):
+        self._chk(self.lib.rs_set_theta(self.h, dptr(self._vec(theta, 6))))
+
+    def set_target_rdf(self, g):
+        g = self._vec(g, self.cfg.rdf_bins)
+        self._chk(self.lib.rs_set_target_rdf(self.h, dptr(g), g.size))
+
+    def set_external_noise(self, xi, n_steps: int):
+        xi = None if xi is None else self._vec(xi, n_steps * self.N3)
+        self._chk(self.lib.rs_set_external_noise(self.h, dptr(xi), int(n_steps)))
+
+    # -- trajectories -----------------------------------------------------------------------
+    def forward(self, x0, v0, seed: int, n_steps: int):
+        x0, v0 = self._vec(x0, self.N3), self._vec(v0, self.N3)
+        x, v, loss = np.empty(self.N3), np.empty(self.N3), np.zeros(1)
+        self._chk(self.lib.rs_forward(self.h, dptr(x0), dptr(v0), seed, n_steps, dptr(x), dptr(v),
+                                      dptr(loss)))
+        return x, v, float(loss[0])
+
+    def reverse(self, xN, vN, seed: int, n_steps: int):
+        xN, vN = self._vec(xN, self.N3), self._vec(vN, self.N3)
+        loss, g = np.zeros(1), np.zeros(6)
+        x0, v0 = np.empty(self.N3), np.empty(self.N3)
+        self._chk(self.lib.rs_reverse(self.h, dptr(xN), dptr(vN), seed, n_steps, dptr(loss), dptr(g),
+                                      dptr(x0), dptr(v0)))
+        return float(loss[0]), g, x0, v0
+
+    def value_and_grad(self, x0, v0, seed: int, n_steps: int, mlp_grad_size: int = 0):
+        x0, v0 = self._vec(x0, self.N3), self._vec(v0, self.N3)
+        loss, g = np.zeros(1), np.zeros(6)
+        gm = np.zeros(mlp_grad_size, dtype=np.float32) if mlp_grad_size else None
+        self._chk(self.lib.rs_value_and_grad(self.h, dptr(x0), dptr(v0), seed, n_steps, dptr(loss),
+                                             dptr(g), fptr(gm)))
+        return (float(loss[0]), g) if gm is None else (float(loss[0]), g, gm)
+
+    # -- single evaluations -----------------------------------------------------------------
+    def forces(self, x):
+        x = self._vec(x, self.N3)
+        F = np.empty(self.N3)
+        self._chk(self.lib.rs_forces(self.h, dptr(x), dptr(F)))
+        return F
+
+    def adjoint_terms(self, x, c, w):
+        x, c, w = self._vec(x, self.N3), self._vec(c, self.N3), self._vec(w, self.N3)
+        F, h, v = np.empty(self.N3), np.empty(self.N3), np.zeros(6)
+        self._chk(self.lib.rs_adjoint_terms(self.h, dptr(x), dptr(c), dptr(w), dptr(F), dptr(h),
+                                            dptr(v)))
+        return F, h, v
+
+    def noise(self, seed: int, step: int):
+        xi = np.empty(self.N3)
+        self._chk(self.lib.rs_noise(self.h, seed, step, dptr(xi)))
+        return xi
+
+    def rdf(self, x):
+        x = self._vec(x, self.N3)
+        nb = self.cfg.rdf_bins
+        g, H, loss, d = np.empty(nb), np.empty(nb, dtype=np.int64), np.zeros(1), np.empty(self.N3)
+        self._chk(self.lib.rs_rdf(self.h, dptr(x), dptr(g), lptr(H), dptr(loss), dptr(d)))
+        return g, H, float(loss[0]), d
+
+    def msd(self, frames):
+        frames = self._vec(frames)
+        nf = frames.size // (3 * self.M)
+        D, loss, d = np.zeros(1), np.zeros(1), np.empty(frames.size)
+        self._chk(self.lib.rs_msd(self.h, dptr(frames), nf, dptr(D), dptr(loss), dptr(d)))
+        return float(D[0]), float(loss[0]), d
+
+    def molecule_pairs(self, x, capacity: int | None = None):
+        x = self._vec(x, self.N3)
+        cap = capacity or min(self.M * (self.M - 1) // 2, 8_000_000)
+        pairs = np.empty(2 * cap, dtype=np.int32)
+        n = ctypes.c_int64()
+        self._chk(self.lib.rs_molecule_pairs(self.h, dptr(x), pairs.ctypes.data_as(_abi._ip), cap,
+                                             ctypes.byref(n)))
+        return pairs[: 2 * n.value].reshape(-1, 2)
+
+    def list_radius(self) -> float:
+        return float(self.lib.rs_list_radius(self.h))
+
+    def enable_timing(self, on: bool = True):
+        self._chk(self.lib.rs_enable_timing(self.h, int(on)))
+
+    def timings(self) -> dict:
+        ms = np.zeros(len(_abi.PHASES))
+        self._chk(self.lib.rs_timings(self.h, dptr(ms), ms.size))
+        return dict(zip(_abi.PHASES, ms.tolist()))

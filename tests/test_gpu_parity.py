@@ -1,0 +1,137 @@
+"""GPU parity: librevsim.so (through its C ABI) against the FP64 CPU oracle on the same inputs.
+
+Tolerances: FP64 pair arithmetic (precision 0) within 1e-10 relative; FP32 pair arithmetic
+(precision 1) within 1e-5 relative for forces/adjoint terms; neighbour sets and RDF histograms
+bit-exact in both modes.
+"""
+import numpy as np
+import pytest
+
+from conftest import per_atom_rel, rel_err
+
+pytestmark = pytest.mark.gpu
+
+FP64, FP32 = 0, 1
+
+
+def _pair(prec, n=216, box=1.862, rc=0.8, **kw):
+    import oracle as O
+    import revsim
+
+    cfg = revsim.make_config(n, box, rc, precision=prec, **kw)
+    sim = revsim.Simulator(cfg, device=0)
+    sim.set_theta(revsim.DEFAULT_THETA)
+    orc = O.Oracle(revsim.make_config(n, box, rc, precision=prec, **kw))
+    orc.set_theta(revsim.DEFAULT_THETA)
+    return sim, orc
+
+
+def test_noise_matches_oracle(gpu_lib):
+    sim, orc = _pair(FP64)
+    for step in (1, 7, 1000):
+        a, b = sim.noise(13, step), orc.noise(13, step)
+        assert np.max(np.abs(a - b)) <= 1e-13 * np.max(np.abs(b))
+
+
+@pytest.mark.parametrize("prec", [FP64, FP32])
+def test_molecule_list_exact(gpu_lib, prec, jittered216, box3000):
+    for x, args in ((jittered216, (216, 1.862, 0.8)), (box3000.coords, (3000, 4.48, 1.0))):
+        sim, orc = _pair(prec, *args)
+        got = sim.molecule_pairs(x)
+        want = orc.molecule_pairs(x, sim.list_radius())
+        assert got.shape == want.shape
+        assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("prec,tol", [(FP64, 1e-11), (FP32, 1e-5)])
+def test_forces(gpu_lib, prec, tol, jittered216, box3000):
+    for x, args in ((jittered216, (216, 1.862, 0.8)), (box3000.coords, (3000, 4.48, 1.0))):
+        sim, orc = _pair(prec, *args)
+        F, Fo = sim.forces(x), orc.forces(x)
+        assert per_atom_rel(F, Fo) < tol, per_atom_rel(F, Fo)
+        # translation invariance: the net force vanishes up to the pair arithmetic's rounding
+        net_tol = (1e-12 if prec == FP64 else 1e-6) * np.abs(Fo).sum()
+        assert abs(F.reshape(-1, 3).sum(0)).max() < net_tol
+
+
+@pytest.mark.parametrize("prec,tol", [(FP64, 1e-10), (FP32, 1e-5)])
+def test_adjoint_terms(gpu_lib, prec, tol, jittered216):
+    sim, orc = _pair(prec)
+    rng = np.random.default_rng(5)
+    c = rng.standard_normal(jittered216.size)
+    w = rng.standard_normal(jittered216.size)
+    F, h, v = sim.adjoint_terms(jittered216, c, w)
+    Fo, ho, vo = orc.adjoint_terms(jittered216, c, w)
+    assert per_atom_rel(F, Fo) < tol
+    assert per_atom_rel(h, ho) < tol, per_atom_rel(h, ho)
+    # the bonded VJPs are FP64 in both modes; pair VJPs follow the pair precision
+    for k in range(6):
+        assert abs(v[k] - vo[k]) <= tol * 10 * (abs(vo[k]) + 1e-30), (k, v[k], vo[k])
+
+
+@pytest.mark.parametrize("prec", [FP64, FP32])
+def test_rdf_histogram_bit_exact(gpu_lib, prec, jittered216):
+    import revsim
+
+    sim, orc = _pair(prec)
+    g0, H0, _, _ = orc.rdf(jittered216)
+    tgt = revsim.perturbed_target(g0)
+    sim.set_target_rdf(tgt)
+    orc.set_target_rdf(tgt)
+    g, H, L, d = sim.rdf(jittered216)
+    go, Ho, Lo, do = orc.rdf(jittered216)
+    assert np.array_equal(H, Ho)
+    assert np.array_equal(g, go)
+    assert L == Lo
+    assert rel_err(d, do) < 1e-12
+
+
+def test_msd(gpu_lib, box216):
+    sim, orc = _pair(FP64, msd_window_steps=40, msd_every=5, w_msd=1e-3)
+    rng = np.random.default_rng(3)
+    base = box216.coords.reshape(-1, 9)[:, :3].reshape(-1)
+    fr = np.concatenate([base + 0.02 * k * rng.standard_normal(base.size) for k in range(8)])
+    D, L, d = sim.msd(fr)
+    Do, Lo, do = orc.msd(fr)
+    assert abs(D - Do) <= 1e-10 * abs(Do)
+    assert abs(L - Lo) <= 1e-9 * abs(Lo)
+    assert rel_err(d, do) < 1e-9
+
+
+def test_forward_trajectory_fp64(gpu_lib, box216):
+    sim, orc = _pair(FP64)
+    x, v, _ = sim.forward(box216.coords, box216.velocities, 13, 50)
+    xo, vo, _ = orc.forward(box216.coords, box216.velocities, 13, 50)
+    dx = x - xo
+    dx -= 1.862 * np.round(dx / 1.862)
+    assert np.max(np.abs(dx)) < 1e-10
+    assert rel_err(v, vo) < 1e-10
+
+
+@pytest.mark.parametrize("prec,tol", [(FP64, 1e-8), (FP32, 1e-4)])
+def test_value_and_grad(gpu_lib, prec, tol, box216):
+    import revsim
+
+    sim, orc = _pair(prec)
+    g0, _, _, _ = orc.rdf(box216.coords)
+    tgt = revsim.perturbed_target(g0)
+    sim.set_target_rdf(tgt)
+    orc.set_target_rdf(tgt)
+    L, g = sim.value_and_grad(box216.coords, box216.velocities, 13, 50)
+    Lo, go = orc.value_and_grad(box216.coords, box216.velocities, 13, 50)
+    assert abs(L - Lo) <= tol * abs(Lo)
+    assert rel_err(g, go) < tol, (g, go)
+
+
+def test_reversibility_fp64(gpu_lib, box216):
+    import revsim
+
+    sim, orc = _pair(FP64)
+    g0, _, _, _ = orc.rdf(box216.coords)
+    sim.set_target_rdf(revsim.perturbed_target(g0))
+    xN, vN, lf = sim.forward(box216.coords, box216.velocities, 13, 100)
+    lr, g, x0, v0 = sim.reverse(xN, vN, 13, 100)
+    d = x0 - np.mod(box216.coords, 1.862)
+    d -= 1.862 * np.round(d / 1.862)
+    assert np.max(np.abs(d)) < 1e-9  # the reference's own bound, water/water.jl:1613
+    assert abs(lr - lf) < 1e-7  # water/water.jl:1614
