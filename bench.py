@@ -1,0 +1,227 @@
+#!/usr/bin/env python3
+"""Benchmark: forward+reverse gradient steps/s of the reversible-simulation loop on one MI355X.
+
+Workload (default): 32,768 flexible waters (98,304 atoms) in a 9.95 nm box, r_c = 1.0 nm, list
+rebuilt every 10 steps, 1,000 forward + 1,000 reverse steps per trajectory, loss = O-O RDF (200
+bins, 0-0.8 nm) + O MSD slope over the last 500 fs vs 2.3e-9 m^2/s. FP32 pair arithmetic, FP64
+state and dL/dθ. One bench "step" is one full trajectory (rs_value_and_grad); `value` counts
+gradient steps (one forward + one reverse MD step) per second.
+
+Also reports: the dominant kernel's roofline (HIP events on the library's stream), the FP32-path
+dL/dθ relative error against the FP64 path, and the FP64 CPU oracle timed on this host on a
+bounded sample of the same system (CPU baseline).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "rev-sim_amd"))
+
+METRIC = "fwd+rev gradient steps/s at 98k atoms (plain vs +MLP); dL/dθ rel err vs FP64"
+PEAK_FP32 = 157.3  # TFLOP/s, MI355X vector FP32 (packed); non-packed issue rate is half
+PEAK_FP32_NONPACKED = 78.6
+PEAK_HBM = 8000.0  # GB/s
+# algorithmic FLOP per pair within r_c (add/sub/mul 1, FMA 2, sqrt/rcp/exp 1): base, extra for O-O LJ
+FLOP_FWD = (36, 15)
+FLOP_REV = (72, 33)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def count_pairs(x, box, rc):
+    """Atom pairs within r_c in different molecules, and O-O pairs, by a periodic k-d tree."""
+    from scipy.spatial import cKDTree
+
+    pos = np.mod(x.reshape(-1, 3), box)
+    tree = cKDTree(pos, boxsize=box)
+    n_all = int(tree.count_neighbors(tree, rc)) - pos.shape[0]  # ordered pairs incl. self
+    n_all //= 2
+    n_mol = pos.shape[0] // 3
+    n_pairs = n_all - 3 * n_mol  # the 3 intramolecular pairs of each water are within r_c
+    O = pos[0::3]
+    to = cKDTree(O, boxsize=box)
+    n_oo = (int(to.count_neighbors(to, rc)) - O.shape[0]) // 2
+    return n_pairs, n_oo
+
+
+def cpu_baseline(wl, x, v, theta, target, steps):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    import revsim
+
+    kw = dict(precision=revsim.RS_PRECISION_FP64)
+    if wl.get("msd_window_steps"):
+        kw.update(msd_window_steps=steps, msd_every=steps // 2, w_msd=wl.get("w_msd", 1.0))
+    cfg = revsim.make_config(wl["n_molecules"], wl["box"], wl["cutoff"], **kw)
+    orc = O.Oracle(cfg)
+    orc.set_theta(theta)
+    orc.set_target_rdf(target)
+    t0 = time.perf_counter()
+    orc.value_and_grad(x, v, 17, steps)
+    dt = time.perf_counter() - t0
+    return {
+        "value": steps / dt,
+        "unit": "gradient steps/s",
+        "cores": O.num_threads(),
+        "kind": "port",
+        "sample": f"FP64 C restatement of the reference CPU loop (OpenMP), same {wl['n_molecules']}-water "
+                  f"system, {steps} forward + {steps} reverse steps (RDF + MSD loss over those steps), "
+                  f"{dt:.1f} s wall",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3, help="timed trajectories")
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="water32768")
+    ap.add_argument("--n-steps", type=int, default=None, help="MD steps per trajectory")
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--equil", type=int, default=200, help="untimed equilibration steps")
+    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64-check", action="store_true")
+    ap.add_argument("--pair-chunks", type=int, default=0)
+    a = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if a.gpus != 1 or world != 1:
+        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": a.gpus}))
+        sys.exit(2)
+
+    import revsim
+
+    wl = dict(revsim.WORKLOADS[a.workload])
+    n_steps = a.n_steps or wl.pop("n_steps")
+    wl.pop("n_steps", None)
+    prec = revsim.RS_PRECISION_FP32 if a.precision == "fp32" else revsim.RS_PRECISION_FP64
+    extra = {k: wl[k] for k in ("msd_window_steps", "msd_every", "w_msd", "use_mlp") if k in wl}
+    cfg = revsim.make_config(wl["n_molecules"], wl["box"], wl["cutoff"], precision=prec,
+                             pair_chunks=a.pair_chunks, **extra)
+    wb = revsim.water_box(wl["n_molecules"], wl["box"])
+    theta = revsim.DEFAULT_THETA
+    log(f"[bench] workload {a.workload}: {wb.n_atoms} atoms, {n_steps}+{n_steps} steps, {a.precision}")
+    sim = revsim.Simulator(cfg, device=0)
+    sim.set_theta(theta)
+
+    # untimed equilibration from the lattice, then the synthetic target RDF
+    x, v = wb.coords, wb.velocities
+    if a.equil > 0:
+        cfg_eq = revsim.make_config(wl["n_molecules"], wl["box"], wl["cutoff"], precision=prec, w_rdf=0.0)
+        with revsim.Simulator(cfg_eq, device=0) as eq:
+            eq.set_theta(theta)
+            x, v, _ = eq.forward(x, v, revsim.system.SEED_NOISE + 100, a.equil)
+    g0, _, _, _ = sim.rdf(x)
+    target = revsim.perturbed_target(g0)
+    sim.set_target_rdf(target)
+    seed = revsim.system.SEED_NOISE
+
+    for w in range(a.warmup):
+        t = time.perf_counter()
+        sim.value_and_grad(x, v, seed, n_steps)
+        log(f"[bench] warmup {w}: {time.perf_counter() - t:.3f} s")
+
+    sim.enable_timing(True)
+    sim.timings()
+    t0 = time.perf_counter()
+    for k in range(a.steps):
+        loss, grad = sim.value_and_grad(x, v, seed, n_steps)
+    t1 = time.perf_counter()
+    tim = sim.timings()
+    sim.enable_timing(False)
+    elapsed = t1 - t0
+    value = a.steps * n_steps / elapsed
+    log(f"[bench] timed {a.steps} trajectories in {elapsed:.3f} s -> {value:.1f} gradient steps/s")
+    log(f"[bench] phases: {json.dumps(tim)}")
+
+    # roofline of the dominant kernel (the adjoint pair kernel)
+    xN, _, _ = sim.forward(x, v, seed, n_steps)
+    p0 = count_pairs(x, wl["box"], wl["cutoff"])
+    p1 = count_pairs(xN, wl["box"], wl["cutoff"])
+    n_pairs, n_oo = (p0[0] + p1[0]) / 2, (p0[1] + p1[1]) / 2
+    dom = max(("fwd_pair", "rev_pair", "fwd_step", "rev_step", "list", "loss"), key=lambda k: tim[k]["ms"])
+    rp = tim["rev_pair"]
+    avg_ms = rp["ms"] / max(rp["launches"], 1)
+    flop = FLOP_REV[0] * n_pairs + FLOP_REV[1] * n_oo
+    achieved = flop / (avg_ms * 1e-3) / 1e12
+    fp = tim["fwd_pair"]
+    fwd_avg = fp["ms"] / max(fp["launches"], 1)
+    roof = {
+        "kernel": "k_pair<P32,CH,adjoint> (reverse pair + HVP + θ-VJP)",
+        "bound": "valu",
+        "achieved": round(achieved, 3),
+        "peak": PEAK_FP32,
+        "unit": "TFLOP/s",
+        "frac": round(achieved / PEAK_FP32, 4),
+        "frac_of_nonpacked_peak": round(achieved / PEAK_FP32_NONPACKED, 4),
+        "traffic": None,
+        "avg_launch_ms": round(avg_ms, 5),
+        "flop_per_launch": flop,
+        "pairs_within_rc": n_pairs,
+        "oo_pairs_within_rc": n_oo,
+        "dominant_phase": dom,
+        "fwd_pair_avg_launch_ms": round(fwd_avg, 5),
+        "fwd_pair_tflops": round((FLOP_FWD[0] * n_pairs + FLOP_FWD[1] * n_oo) / (fwd_avg * 1e-3) / 1e12, 3),
+    }
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "gradient steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(1e3 * elapsed / a.steps, 3),
+        "ms_per_gradient_step": round(1e3 * elapsed / (a.steps * n_steps), 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32" if prec == revsim.RS_PRECISION_FP32 else "fp64",
+        "state_dtype": "fp64",
+        "data": "synthetic",
+        "config": {"workload": f"{a.workload}: {wl['n_molecules']} waters / {wb.n_atoms} atoms, "
+                               f"{wl['box']} nm box, r_c {wl['cutoff']} nm, {n_steps}+{n_steps} steps, "
+                               f"RDF{'+MSD' if extra.get('w_msd') else ''} loss",
+                   "atoms": wb.n_atoms, "n_steps": n_steps, "parallelism": "single GPU"},
+        "loss": loss,
+        "dL_dtheta": grad.tolist(),
+        "phases_ms": {k: round(vv["ms"], 3) for k, vv in tim.items()},
+        "roofline": roof,
+    }
+
+    if not a.no_fp64_check and prec == revsim.RS_PRECISION_FP32:
+        cfg64 = revsim.make_config(wl["n_molecules"], wl["box"], wl["cutoff"],
+                                   precision=revsim.RS_PRECISION_FP64, **extra)
+        with revsim.Simulator(cfg64, device=0) as s64:
+            s64.set_theta(theta)
+            s64.set_target_rdf(target)
+            t = time.perf_counter()
+            l64, g64 = s64.value_and_grad(x, v, seed, n_steps)
+            log(f"[bench] fp64 path: {time.perf_counter() - t:.2f} s")
+        rel = np.abs(grad - g64) / np.maximum(np.abs(g64), 1e-300)
+        out["grad_rel_err_vs_fp64"] = {
+            "normwise": float(np.linalg.norm(grad - g64) / np.linalg.norm(g64)),
+            "per_param": rel.tolist(),
+            "reference": "same trajectory with FP64 pair arithmetic (parity-tested against the FP64 CPU oracle)",
+        }
+
+    if not a.no_cpu_baseline:
+        log("[bench] cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(wl, x, v, theta, target, a.cpu_steps)
+        out["speedup_vs_cpu_baseline"] = round(value / out["cpu_baseline"]["value"], 1)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -151,6 +151,8 @@ double rs_list_radius(const rs_ctx* ctx);
 enum { RS_PHASE_FWD_PAIR = 0, RS_PHASE_FWD_STEP, RS_PHASE_REV_PAIR, RS_PHASE_REV_STEP,
        RS_PHASE_LIST, RS_PHASE_LOSS, RS_PHASE_MLP, RS_PHASE_COUNT };
 int rs_timings(rs_ctx* ctx, double* ms_by_phase, int32_t n);
+/* Number of timed launches per phase accumulated since the last rs_timings call (read it first). */
+int rs_timing_counts(rs_ctx* ctx, int64_t* counts, int32_t n);
 /* Enable/disable per-phase event timing (off by default; adds event records per launch). */
 int rs_enable_timing(rs_ctx* ctx, int32_t on);
 

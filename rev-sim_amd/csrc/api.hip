@@ -53,9 +53,13 @@ struct rs_ctx {
   int ext_steps = 0;
   int since_build = 0;
   std::string err;
-  // timing
+  // timing: events recorded around launches on the context's stream
   bool timing = false;
   double ms[RS_PHASE_COUNT] = {0};
+  int64_t cnt[RS_PHASE_COUNT] = {0};
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  std::vector<int> mark_phase;
   std::vector<double> slot_host;
 };
 
@@ -228,8 +232,42 @@ int ensure_frames(rs_ctx* c, int nf) {
   return 0;
 }
 
+// Phase timer: tbegin records an event before a launch, tend after it; the elapsed times are
+// collected after the stream has been synchronised (collect_timing).
+hipEvent_t next_event(rs_ctx* c) {
+  if (c->ev_used == c->ev_pool.size()) {
+    hipEvent_t e;
+    hipEventCreate(&e);
+    c->ev_pool.push_back(e);
+  }
+  return c->ev_pool[c->ev_used++];
+}
+void tbegin(rs_ctx* c, int phase) {
+  if (!c->timing) return;
+  hipEventRecord(next_event(c), c->s);
+  c->mark_phase.push_back(phase);
+}
+void tend(rs_ctx* c) {
+  if (!c->timing) return;
+  hipEventRecord(next_event(c), c->s);
+}
+void collect_timing(rs_ctx* c) {
+  if (!c->timing) return;
+  for (size_t k = 0; k < c->mark_phase.size(); ++k) {
+    float t = 0.f;
+    if (hipEventElapsedTime(&t, c->ev_pool[2 * k], c->ev_pool[2 * k + 1]) == hipSuccess) {
+      c->ms[c->mark_phase[k]] += t;
+      c->cnt[c->mark_phase[k]] += 1;
+    }
+  }
+  c->mark_phase.clear();
+  c->ev_used = 0;
+}
+
 void rebuild(rs_ctx* c, bool with_u) {
+  tbegin(c, RS_PHASE_LIST);
   launch_list_build(c->L, with_u);
+  tend(c);
   c->since_build = 0;
 }
 
@@ -241,6 +279,7 @@ int reset_flags(rs_ctx* c) {
 // Synchronise and translate the device flags into a status. Sets *overflow for a retry.
 int finish(rs_ctx* c, bool* overflow) {
   HIPCHK(c, hipStreamSynchronize(c->s));
+  collect_timing(c);
   int f[FLAG_COUNT];
   HIPCHK(c, hipMemcpy(f, c->L.b.flags, sizeof(f), hipMemcpyDeviceToHost));
   if (overflow) *overflow = f[FLAG_LIST_OVERFLOW] != 0;
@@ -298,17 +337,28 @@ int forward_impl(rs_ctx* c, const double* x0, const double* v0, uint64_t seed, i
     };
     take_snap(0);
     for (int n = 1; n <= n_steps; ++n) {
+      tbegin(c, RS_PHASE_FWD_PAIR);
       launch_pair(c->L, false, nullptr, 0, 0, nullptr, 0, 0);
+      tend(c);
+      tbegin(c, RS_PHASE_FWD_STEP);
       launch_step_fwd(c->L, n, seed, dt, e, s, kT);
+      tend(c);
       if (n % g.nl_every == 0) rebuild(c, false);
       take_snap(n);
-      if (n % P.n_loss == 0 && g.w_rdf != 0.0)
+      if (n % P.n_loss == 0 && g.w_rdf != 0.0) {
+        tbegin(c, RS_PHASE_LOSS);
         launch_rdf(c->L, g.rdf_bins, g.rdf_rmax, g.rdf_width, g.rdf_window, g.w_rdf / P.n_blocks,
                    false, b.scal + 0);
+        tend(c);
+      }
       int f = P.frame_of(n);
       if (f >= 0) launch_frame_copy(c->L, f);
     }
-    if (P.msd_on) launch_msd(c->L, P.nf, (P.nf / 2) * g.msd_every * dt, g.w_msd, g.msd_target, b.scal + 0);
+    if (P.msd_on) {
+      tbegin(c, RS_PHASE_LOSS);
+      launch_msd(c->L, P.nf, (P.nf / 2) * g.msd_every * dt, g.w_msd, g.msd_target, b.scal + 0);
+      tend(c);
+    }
     bool ovf = false;
     if ((r = finish(c, &ovf))) return r;
     if (ovf) {
@@ -335,6 +385,7 @@ int forward_impl(rs_ctx* c, const double* x0, const double* v0, uint64_t seed, i
 void seed_at(rs_ctx* c, const Plan& P, int t, bool with_rdf, bool reset, double e) {
   const rs_config& g = c->cfg;
   DevBufs& b = c->L.b;
+  tbegin(c, RS_PHASE_LOSS);
   launch_zero(c->L, b.grad, c->N3);
   if (with_rdf && g.w_rdf != 0.0)
     launch_rdf(c->L, g.rdf_bins, g.rdf_rmax, g.rdf_width, g.rdf_window, g.w_rdf / P.n_blocks, true,
@@ -342,6 +393,7 @@ void seed_at(rs_ctx* c, const Plan& P, int t, bool with_rdf, bool reset, double 
   int f = P.frame_of(t);
   if (f >= 0) launch_msd_inject(c->L, f);
   launch_seed(c->L, reset, e);
+  tend(c);
 }
 
 int restore(rs_ctx* c, int t) {
@@ -381,10 +433,14 @@ int reverse_impl(rs_ctx* c, const double* xN, const double* vN, uint64_t seed, i
     for (int n = n_steps; n >= 1; --n) {
       ++trunc_count;
       if (trunc_count <= P.n_trunc) {
+        tbegin(c, RS_PHASE_REV_STEP);
         launch_step_rev(c->L, pending, true, n, seed, dt, e, s, kT, nslots, pending && clip);
+        tend(c);
         pending = false;
         if (++c->since_build >= g.nl_every) rebuild(c, true);
+        tbegin(c, RS_PHASE_REV_PAIR);
         launch_pair(c->L, true, b.C, cs[0], cs[1], b.A, ws[0], ws[1]);
+        tend(c);
         if (clip) launch_hnorm(c->L, g.clip_norm);
         pending = true;
       }
@@ -395,7 +451,9 @@ int reverse_impl(rs_ctx* c, const double* xN, const double* vN, uint64_t seed, i
       const bool frame = P.frame_of(t) >= 0;
       if (!(blk || logr || frame)) continue;
       if (pending) {
+        tbegin(c, RS_PHASE_REV_STEP);
         launch_step_rev(c->L, true, false, n, seed, dt, e, s, kT, nslots, clip);
+        tend(c);
         pending = false;
       }
       bool restored = false;
@@ -442,6 +500,7 @@ int single_eval(rs_ctx* c, const double* x, bool with_u, Body body) {
     body();
     bool ovf = false;
     HIPCHK(c, hipStreamSynchronize(c->s));
+    collect_timing(c);
     int f[FLAG_COUNT];
     HIPCHK(c, hipMemcpy(f, b.flags, sizeof(f), hipMemcpyDeviceToHost));
     ovf = f[FLAG_LIST_OVERFLOW] != 0;
@@ -568,6 +627,7 @@ int rs_destroy(rs_ctx* c) {
                   b.hpart};
   for (void* q : ptrs)
     if (q) hipFree(q);
+  for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
   if (c->s) hipStreamDestroy(c->s);
   delete c;
   return 0;
@@ -769,7 +829,15 @@ int rs_timings(rs_ctx* c, double* ms, int32_t n) {
   for (int k = 0; k < n && k < RS_PHASE_COUNT; ++k) {
     ms[k] = c->ms[k];
     c->ms[k] = 0.0;
+    c->cnt[k] = 0;
   }
+  return 0;
+}
+
+int rs_timing_counts(rs_ctx* c, int64_t* counts, int32_t n) {
+  int r = check_ctx(c);
+  if (r) return r;
+  for (int k = 0; k < n && k < RS_PHASE_COUNT; ++k) counts[k] = c->cnt[k];
   return 0;
 }
 

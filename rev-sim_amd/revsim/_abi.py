@@ -160,6 +160,7 @@ SIGNATURES = [
     ("rs_molecule_pairs", ctypes.c_int, [_vp, _dp, _ip, ctypes.c_int64, _lp]),
     ("rs_list_radius", ctypes.c_double, [_vp]),
     ("rs_timings", ctypes.c_int, [_vp, _dp, ctypes.c_int32]),
+    ("rs_timing_counts", ctypes.c_int, [_vp, _lp, ctypes.c_int32]),
     ("rs_enable_timing", ctypes.c_int, [_vp, ctypes.c_int32]),
 ]
 

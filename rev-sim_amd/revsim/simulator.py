@@ -144,6 +144,9 @@ class Simulator:
         self._chk(self.lib.rs_enable_timing(self.h, int(on)))
 
     def timings(self) -> dict:
+        """Per-phase device time (ms) and launch counts since the last call, from HIP events."""
         ms = np.zeros(len(_abi.PHASES))
+        cnt = np.zeros(len(_abi.PHASES), dtype=np.int64)
+        self._chk(self.lib.rs_timing_counts(self.h, lptr(cnt), cnt.size))
         self._chk(self.lib.rs_timings(self.h, dptr(ms), ms.size))
-        return dict(zip(_abi.PHASES, ms.tolist()))
+        return {k: {"ms": float(m), "launches": int(n)} for k, m, n in zip(_abi.PHASES, ms, cnt)}
