@@ -912,7 +912,9 @@ int or_rdf(or_ctx* o, const double* x, double* g, int64_t* hist_out, double* los
 /* ---------------------------------------------------------------- MSD loss */
 
 /* calculate_D (diffusion/diffusion.jl:861-877) on the O atoms of frames f = 0..2K-1 spaced
-   msd_every*dt apart; unwrapped displacement by summed minimum-image increments; D in nm^2/ps;
+   msd_every*dt apart: D = (1/6K) sum_s MSD(s, s+K) / (K msd_every dt), i.e. the Einstein relation
+   at lag K frames averaged over K origins; unwrapped displacement by summed minimum-image
+   increments; D in nm^2/ps;
    L = w_msd 1e18 (1e-6 D - target)^2 (diffusion/diffusion.jl:897-911). dL/dframes analytic:
    the increments telescope, so dΔ_s/dx_{s+K} = I and dΔ_s/dx_s = -I. */
 int or_msd(or_ctx* o, const double* frames, int32_t nf, double* Dout, double* loss, double* dLdf) {
@@ -935,7 +937,7 @@ int or_msd(or_ctx* o, const double* frames, int32_t nf, double* Dout, double* lo
       msd += acc[0] * acc[0] + acc[1] * acc[1] + acc[2] * acc[2];
     }
     msd /= M;
-    Dsum += msd / (K * lag);
+    Dsum += msd / lag;
   }
   double Dv = Dsum / (6.0 * K);
   double diff = 1e-6 * Dv - o->cfg.msd_target;
@@ -944,7 +946,7 @@ int or_msd(or_ctx* o, const double* frames, int32_t nf, double* Dout, double* lo
   if (loss) *loss = L;
   if (dLdf) {
     double dLdD = o->cfg.w_msd * 1e18 * 2.0 * diff * 1e-6;
-    double c = dLdD * 2.0 / (6.0 * K * K * lag * M);
+    double c = dLdD * 2.0 / (6.0 * K * lag * M);
     memset(dLdf, 0, sizeof(double) * (size_t)nf * 3 * M);
     for (int s = 0; s < K; ++s)
       for (int i = 0; i < M; ++i)

