@@ -87,7 +87,7 @@ def main():
     ap.add_argument("--workload", default="water32768")
     ap.add_argument("--n-steps", type=int, default=None, help="MD steps per trajectory")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
-    ap.add_argument("--equil", type=int, default=200, help="untimed equilibration steps")
+    ap.add_argument("--equil", type=int, default=2000, help="untimed equilibration steps")
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-check", action="store_true")
@@ -107,6 +107,9 @@ def main():
     wl.pop("n_steps", None)
     prec = revsim.RS_PRECISION_FP32 if a.precision == "fp32" else revsim.RS_PRECISION_FP64
     extra = {k: wl[k] for k in ("msd_window_steps", "msd_every", "w_msd", "use_mlp") if k in wl}
+    if extra.get("msd_window_steps", 0) > n_steps:  # shorter trajectories: MSD over their 2nd half
+        every = extra["msd_every"]
+        extra["msd_window_steps"] = max(2 * every, (n_steps // 2) // (2 * every) * (2 * every))
     cfg = revsim.make_config(wl["n_molecules"], wl["box"], wl["cutoff"], precision=prec,
                              pair_chunks=a.pair_chunks, **extra)
     wb = revsim.water_box(wl["n_molecules"], wl["box"])
@@ -171,6 +174,7 @@ def main():
         "pairs_within_rc": n_pairs,
         "oo_pairs_within_rc": n_oo,
         "dominant_phase": dom,
+        "list_radius_nm": round(sim.list_radius(), 4),
         "fwd_pair_avg_launch_ms": round(fwd_avg, 5),
         "fwd_pair_tflops": round((FLOP_FWD[0] * n_pairs + FLOP_FWD[1] * n_oo) / (fwd_avg * 1e-3) / 1e12, 3),
     }

@@ -129,7 +129,7 @@ void or_default_config(rs_config* c) {
   c->n_molecules = 216;
   c->box[0] = c->box[1] = c->box[2] = 1.862;
   c->cutoff = 0.8;
-  c->skin = 0.1;
+  c->skin = 0.06;
   c->nl_every = 10;
   c->precision = RS_PRECISION_FP64;
   c->dt = 0.001;

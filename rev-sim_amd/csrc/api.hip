@@ -52,6 +52,7 @@ struct rs_ctx {
   double msd_loss = 0.0;
   int ext_steps = 0;
   int since_build = 0;
+  double hext_extra = 0.0;  // added to the O-H extent after an extent violation
   std::string err;
   // timing: events recorded around launches on the context's stream
   bool timing = false;
@@ -78,7 +79,7 @@ int fail(rs_ctx* c, int code, const std::string& msg) {
       return fail((c), RS_E_HIP, std::string(#expr ": ") + hipGetErrorString(e_));          \
   } while (0)
 
-const double H_EXTENT_FACTOR = 1.25;  // h_ext = 1.25 r0 + 0.005 nm (checked every step)
+const double H_EXTENT_FACTOR = 1.15;  // h_ext = 1.15 r0 + 0.002 nm (checked every step)
 
 void set_params(rs_ctx* c) {
   SysParams& p = c->L.p;
@@ -113,7 +114,7 @@ void set_params(rs_ctx* c) {
   p.eps24f = (float)(24.0 * p.eps);
   p.mO = g.mass_O;
   p.mH = g.mass_H;
-  p.hext = H_EXTENT_FACTOR * std::fabs(p.r0) + 0.005;
+  p.hext = H_EXTENT_FACTOR * std::fabs(p.r0) + 0.002 + c->hext_extra;
   c->rlist = rc + 2.0 * p.hext + g.skin;
   p.rlist2 = c->rlist * c->rlist;
   p.skin_half2 = 0.25 * g.skin * g.skin;
@@ -163,6 +164,7 @@ int alloc_all(rs_ctx* c) {
   const size_t lanes = (size_t)pair_blocks(p) * RS_BLOCK;
   int r = 0;
   double** st[] = {&b.x, &b.v, &b.F, &b.h, &b.A, &b.B, &b.C, &b.D, &b.grad};
+  if ((r = dalloc(c, &b.vjb, 3 * M))) return r;
   for (auto q : st)
     if ((r = dalloc(c, q, N3))) return r;
   if ((r = dalloc(c, &b.cell, M)) || (r = dalloc(c, &b.cell_count, p.ncell + 1)) ||
@@ -276,25 +278,32 @@ int reset_flags(rs_ctx* c) {
   return 0;
 }
 
-// Synchronise and translate the device flags into a status. Sets *overflow for a retry.
-int finish(rs_ctx* c, bool* overflow) {
+// Synchronise and translate the device flags into a status. *retry is set when the list was
+// too small or too short-sighted for this call; the caller widens it (grow_or_widen) and reruns.
+int finish(rs_ctx* c, int* retry) {
   HIPCHK(c, hipStreamSynchronize(c->s));
   collect_timing(c);
   int f[FLAG_COUNT];
   HIPCHK(c, hipMemcpy(f, c->L.b.flags, sizeof(f), hipMemcpyDeviceToHost));
-  if (overflow) *overflow = f[FLAG_LIST_OVERFLOW] != 0;
-  if (f[FLAG_LIST_OVERFLOW] && !overflow)
-    return fail(c, RS_E_OVERFLOW, "molecule neighbour list capacity exceeded");
   if (f[FLAG_NONFINITE]) return fail(c, RS_E_NONFINITE, "non-finite positions, velocities or forces");
-  if (f[FLAG_STALE])
-    return fail(c, RS_E_STALE, "a molecule moved more than skin/2 between neighbour-list builds");
-  if (f[FLAG_HEXTENT])
-    return fail(c, RS_E_STALE, "an O-H distance exceeded the extent assumed by the list radius");
+  *retry = (f[FLAG_LIST_OVERFLOW] ? 1 : 0) | (f[FLAG_STALE] ? 2 : 0) | (f[FLAG_HEXTENT] ? 4 : 0);
   return 0;
 }
 
-int grow_list(rs_ctx* c) {
-  c->L.p.capc *= 2;
+// After an overflow the per-lane capacity doubles; after a stale list the skin grows 1.5x (and the
+// O-H extent grows by 0.01 nm after an extent violation). The list radius and cell grid follow;
+// the set of interacting pairs does not depend on them.
+int grow_or_widen(rs_ctx* c, int why, int attempt) {
+  if (attempt >= 6)
+    return fail(c, (why & 1) ? RS_E_OVERFLOW : RS_E_STALE,
+                (why & 1) ? "neighbour list capacity exceeded after retries"
+                          : "neighbour list still stale after widening the skin");
+  const int capc = c->L.p.capc;
+  if (why & 2) c->cfg.skin = 1.5 * c->cfg.skin + 0.005;
+  if (why & 4) c->hext_extra += 0.01;
+  if (why & 6) set_params(c);  // a list radius beyond a third of the box falls back to all pairs
+  c->L.p.capc = std::max(capc, default_capc(c));
+  if (why & 1) c->L.p.capc = 2 * capc;
   return alloc_list(c);
 }
 
@@ -327,6 +336,7 @@ int forward_impl(rs_ctx* c, const double* x0, const double* v0, uint64_t seed, i
     HIPCHK(c, hipMemsetAsync(b.scal, 0, sizeof(double) * 16, c->s));
     launch_wrap_state(c->L);
     rebuild(c, false);
+    launch_bonded(c->L, false, nullptr, 0, 0, nullptr, 0, 0);
     size_t snap = 0;
     auto take_snap = [&](int t) {
       if (snap < c->snap_steps.size() && c->snap_steps[snap] == t) {
@@ -338,7 +348,7 @@ int forward_impl(rs_ctx* c, const double* x0, const double* v0, uint64_t seed, i
     take_snap(0);
     for (int n = 1; n <= n_steps; ++n) {
       tbegin(c, RS_PHASE_FWD_PAIR);
-      launch_pair(c->L, false, nullptr, 0, 0, nullptr, 0, 0);
+      launch_pair(c->L, false, nullptr, 0, 0);
       tend(c);
       tbegin(c, RS_PHASE_FWD_STEP);
       launch_step_fwd(c->L, n, seed, dt, e, s, kT);
@@ -359,11 +369,10 @@ int forward_impl(rs_ctx* c, const double* x0, const double* v0, uint64_t seed, i
       launch_msd(c->L, P.nf, (P.nf / 2) * g.msd_every * dt, g.w_msd, g.msd_target, b.scal + 0);
       tend(c);
     }
-    bool ovf = false;
-    if ((r = finish(c, &ovf))) return r;
-    if (ovf) {
-      if (attempt >= 4) return fail(c, RS_E_OVERFLOW, "neighbour list capacity exceeded after retries");
-      if ((r = grow_list(c))) return r;
+    int why = 0;
+    if ((r = finish(c, &why))) return r;
+    if (why) {
+      if ((r = grow_or_widen(c, why, attempt))) return r;
       continue;
     }
     break;
@@ -439,7 +448,8 @@ int reverse_impl(rs_ctx* c, const double* xN, const double* vN, uint64_t seed, i
         pending = false;
         if (++c->since_build >= g.nl_every) rebuild(c, true);
         tbegin(c, RS_PHASE_REV_PAIR);
-        launch_pair(c->L, true, b.C, cs[0], cs[1], b.A, ws[0], ws[1]);
+        launch_pair(c->L, true, b.A, ws[0], ws[1]);
+        launch_vjp_reduce(c->L);
         tend(c);
         if (clip) launch_hnorm(c->L, g.clip_norm);
         pending = true;
@@ -468,12 +478,10 @@ int reverse_impl(rs_ctx* c, const double* xN, const double* vN, uint64_t seed, i
       }
     }
     if (pending) launch_step_rev(c->L, true, false, 1, seed, dt, e, s, kT, nslots, clip);
-    bool ovf = false;
-    if ((r = finish(c, &ovf))) return r;
-    if (ovf) {
-      if (attempt >= 4) return fail(c, RS_E_OVERFLOW, "neighbour list capacity exceeded after retries");
-      if ((r = grow_list(c))) return r;
-      if (!xN || !vN) return fail(c, RS_E_OVERFLOW, "neighbour list overflow in a device-state reverse");
+    int why = 0;
+    if ((r = finish(c, &why))) return r;
+    if (why) {
+      if ((r = grow_or_widen(c, why, attempt))) return r;
       continue;
     }
     break;
@@ -498,16 +506,15 @@ int single_eval(rs_ctx* c, const double* x, bool with_u, Body body) {
     if ((r = reset_flags(c))) return r;
     rebuild(c, with_u);
     body();
-    bool ovf = false;
     HIPCHK(c, hipStreamSynchronize(c->s));
     collect_timing(c);
     int f[FLAG_COUNT];
     HIPCHK(c, hipMemcpy(f, b.flags, sizeof(f), hipMemcpyDeviceToHost));
-    ovf = f[FLAG_LIST_OVERFLOW] != 0;
     if (f[FLAG_NONFINITE]) return fail(c, RS_E_NONFINITE, "non-finite forces");
-    if (ovf) {
-      if (attempt >= 4) return fail(c, RS_E_OVERFLOW, "neighbour list capacity exceeded");
-      if ((r = grow_list(c))) return r;
+    // a single evaluation builds its list at x: only capacity and the O-H extent can fail
+    const int why = (f[FLAG_LIST_OVERFLOW] ? 1 : 0) | (f[FLAG_HEXTENT] ? 4 : 0);
+    if (why) {
+      if ((r = grow_or_widen(c, why, attempt))) return r;
       continue;
     }
     return 0;
@@ -535,7 +542,7 @@ void rs_default_config(rs_config* g) {
   g->n_molecules = 216;
   g->box[0] = g->box[1] = g->box[2] = 1.862;
   g->cutoff = 0.8;
-  g->skin = 0.1;
+  g->skin = 0.06;
   g->nl_every = 10;
   g->precision = RS_PRECISION_FP32;
   g->dt = 0.001;
@@ -622,7 +629,7 @@ int rs_destroy(rs_ctx* c) {
   DevBufs& b = c->L.b;
   void* ptrs[] = {b.x, b.v, b.F, b.h, b.A, b.B, b.C, b.D, b.grad, b.cell, b.cell_count,
                   b.cell_start, b.cell_fill, b.tmp, b.order, b.rank, b.posq, b.pos64, b.uf,
-                  b.ud, b.O64, b.nbr, b.ncnt, b.flags, b.slots, b.dth, b.scal, b.hist, b.rdfG,
+                  b.ud, b.O64, b.nbr, b.ncnt, b.flags, b.slots, b.vjb, b.dth, b.scal, b.hist, b.rdfG,
                   b.target, b.rdf_g, b.frames, b.msd_del, b.msd_acc, b.msd_grad, b.ext_noise,
                   b.hpart};
   for (void* q : ptrs)
@@ -706,7 +713,10 @@ int rs_forces(rs_ctx* c, const double* x, double* F) {
   int r = check_ctx(c);
   if (r) return r;
   if (!x || !F) return fail(c, RS_E_CONFIG, "null argument");
-  if ((r = single_eval(c, x, false, [&] { launch_pair(c->L, false, nullptr, 0, 0, nullptr, 0, 0); })))
+  if ((r = single_eval(c, x, false, [&] {
+         launch_bonded(c->L, false, nullptr, 0, 0, nullptr, 0, 0);
+         launch_pair(c->L, false, nullptr, 0, 0);
+       })))
     return r;
   HIPCHK(c, hipMemcpy(F, c->L.b.F, sizeof(double) * c->N3, hipMemcpyDeviceToHost));
   return 0;
@@ -721,7 +731,10 @@ int rs_adjoint_terms(rs_ctx* c, const double* x, const double* cc, const double*
   HIPCHK(c, hipMemcpy(b.C, cc, sizeof(double) * c->N3, hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(b.A, w, sizeof(double) * c->N3, hipMemcpyHostToDevice));
   const double cs[2] = {1.0 / c->cfg.mass_O, 1.0 / c->cfg.mass_H};
-  if ((r = single_eval(c, x, true, [&] { launch_pair(c->L, true, b.C, cs[0], cs[1], b.A, 1.0, 1.0); })))
+  if ((r = single_eval(c, x, true, [&] {
+         launch_bonded(c->L, true, b.C, cs[0], cs[1], b.A, 1.0, 1.0);
+         launch_pair(c->L, true, b.A, 1.0, 1.0);
+       })))
     return r;
   if (F) HIPCHK(c, hipMemcpy(F, b.F, sizeof(double) * c->N3, hipMemcpyDeviceToHost));
   if (h) HIPCHK(c, hipMemcpy(h, b.h, sizeof(double) * c->N3, hipMemcpyDeviceToHost));

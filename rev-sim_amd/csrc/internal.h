@@ -20,6 +20,7 @@ struct DevBufs {
   int *nbr, *ncnt;
   int* flags;
   double* slots;   // per-block θ-VJP partials [nblocks * 8]
+  double* vjb;     // bonded θ-VJP per molecule [3M]: (k_bond, r0, k_angle)
   double* dth;     // dL/dθ accumulator [8]
   double* scal;    // device scalars: [0] loss, [1] rdf loss (last), [2] msd loss, [3] D, [4] clip scale
   long long* hist; // RDF bins (fixed point 2^32)
@@ -42,8 +43,10 @@ struct Launch {
 // list
 void launch_list_build(const Launch& L, bool with_u);
 // pair
-void launch_pair(const Launch& L, bool adjoint, const double* csrc, double cscale_O, double cscale_H,
-                 const double* wsrc, double wscale_O, double wscale_H);
+void launch_pair(const Launch& L, bool adjoint, const double* wsrc, double wscale_O, double wscale_H);
+void launch_vjp_reduce(const Launch& L);
+void launch_bonded(const Launch& L, bool adjoint, const double* csrc, double cO, double cH,
+                   const double* wsrc, double wO, double wH);
 // integrator
 void launch_wrap_state(const Launch& L);
 void launch_step_fwd(const Launch& L, int step, uint64_t seed, double dt, double e, double s,

@@ -30,12 +30,28 @@ __global__ void k_zero_int(int* a, int n) {
   if (t < n) a[t] = 0;
 }
 
-__global__ void k_cells_assign(SysParams p, const double* __restrict__ x, int* cell, int* count) {
-  int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= p.M) return;
-  int c = p.brute ? 0 : cell_index(p, x[9 * m], x[9 * m + 1], x[9 * m + 2]);
-  cell[m] = c;
-  atomicAdd(&count[c], 1);
+// Cell of each molecule's O and the per-cell counts. Counts go through a block-private LDS
+// histogram (molecules of one block share few cells) and one global atomic per nonzero bin.
+constexpr int LDS_CELLS = 8192;
+
+__global__ __launch_bounds__(RS_BLOCK) void k_cells_assign(SysParams p, const double* __restrict__ x,
+                                                           int* cell, int* count) {
+  __shared__ int hist[LDS_CELLS];
+  const bool lds = p.ncell <= LDS_CELLS;
+  if (lds)
+    for (int c = threadIdx.x; c < p.ncell; c += RS_BLOCK) hist[c] = 0;
+  __syncthreads();
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m < p.M) {
+    const int c = p.brute ? 0 : cell_index(p, x[9 * m], x[9 * m + 1], x[9 * m + 2]);
+    cell[m] = c;
+    if (lds) atomicAdd(&hist[c], 1);
+    else atomicAdd(&count[c], 1);
+  }
+  __syncthreads();
+  if (lds)
+    for (int c = threadIdx.x; c < p.ncell; c += RS_BLOCK)
+      if (hist[c]) atomicAdd(&count[c], hist[c]);
 }
 
 // Exclusive scan of cell counts (one block); also clears the fill counters.
@@ -66,12 +82,30 @@ __global__ void k_cells_scan(int ncell, const int* count, int* start, int* fill)
   }
 }
 
-__global__ void k_cells_scatter(SysParams p, const int* cell, const int* start, int* fill, int* tmp) {
-  int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= p.M) return;
-  int c = cell[m];
-  int slot = atomicAdd(&fill[c], 1);
-  tmp[start[c] + slot] = m;
+// Places each molecule into its cell's range (any order inside the cell; k_cells_order then sorts
+// by molecule index). Block-private LDS offsets, one global atomic per nonzero bin.
+__global__ __launch_bounds__(RS_BLOCK) void k_cells_scatter(SysParams p, const int* cell,
+                                                            const int* start, int* fill, int* tmp) {
+  __shared__ int hist[LDS_CELLS];
+  __shared__ int base[LDS_CELLS];
+  const bool lds = p.ncell <= LDS_CELLS;
+  if (lds)
+    for (int c = threadIdx.x; c < p.ncell; c += RS_BLOCK) hist[c] = 0;
+  __syncthreads();
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = m < p.M ? cell[m] : 0;
+  int local = 0;
+  if (m < p.M) {
+    if (lds) local = atomicAdd(&hist[c], 1);
+    else tmp[start[c] + atomicAdd(&fill[c], 1)] = m;
+  }
+  __syncthreads();
+  if (lds) {
+    for (int q = threadIdx.x; q < p.ncell; q += RS_BLOCK)
+      if (hist[q]) base[q] = atomicAdd(&fill[q], hist[q]);
+    __syncthreads();
+    if (m < p.M) tmp[start[c] + base[c] + local] = m;
+  }
 }
 
 // Orders each cell's members by molecule index (one wave per cell): deterministic sorted order.

@@ -1,7 +1,9 @@
-// pair.hip — the fused force / adjoint kernel: reaction-field Coulomb + O-O Lennard-Jones over the
-// molecule list, plus the intramolecular harmonic bonds and angle, in one pass.
+// pair.hip — the fused pair force / adjoint kernel: reaction-field Coulomb + O-O Lennard-Jones over
+// the molecule list, forces, Hessian-vector product and θ-VJP in one pass.
 //
-// Forward variant (calc_F!, water/water.jl:877-975): F(x).
+// Forward variant (calc_F!, water/water.jl:877-975): F(x). The intramolecular terms (FP64, see
+// bonded.h) are evaluated by the preceding integrator kernel, which seeds F (and h and the bonded
+// θ-VJP in the adjoint variant) at the same positions; this kernel adds the pair terms.
 // Adjoint variant (calc_F_and_grads!, water/water.jl:1212-1433): F(x), the Hessian-vector product
 //   h = (dF/dx)^T u with u = accum_C / m, and the θ-VJP sum_dof w_dof dF_dof/dθ with
 //   w = accum_A dt^2/(2m) (water/water.jl:1517-1519), θ = (σ_O, ε_O, q_H, k_b, r0, k_a).
@@ -9,8 +11,8 @@
 // Work decomposition: CH consecutive lanes own one molecule i (sorted order); lane c walks
 // sub-list c of i's full molecule list, evaluates the 3x3 atom pairs of each listed molecule j and
 // accumulates only i's side. The CH partial sums are combined by xor-shuffles in a fixed order, so
-// the result is deterministic without atomics. Lane c == 0 then adds the bonded terms (FP64) and
-// writes the molecule's outputs in the caller's atom order.
+// the result is deterministic without atomics. Lane c == 0 then adds them to the molecule's
+// outputs in the caller's atom order.
 //
 // Pair arithmetic: FP32 on quantized positions (the int32 difference of 2^32-periodic positions is
 // the minimum image by construction) or FP64 (parity mode). Per pair of atoms a (own), b:
@@ -30,11 +32,9 @@ struct PairArgs {
   const int* __restrict__ order;
   const void* __restrict__ pos;
   const void* __restrict__ u;
-  const double* __restrict__ x;
-  const double* __restrict__ csrc;
   const double* __restrict__ wsrc;
-  double cscale[2], wscale[2];
-  const double4* __restrict__ O64;
+  double wscale[2];
+  const double* __restrict__ vjb;
   double* F;
   double* h;
   double* slots;
@@ -70,103 +70,6 @@ struct P64 {
   static __device__ __forceinline__ double rc2(const SysParams& p) { return p.rc2; }
   static __device__ __forceinline__ double rinv(double r2) { return 1.0 / sqrt(r2); }
 };
-
-// ------------------------------------------------------------------ bonded terms (FP64)
-
-struct dd {
-  double v, d;
-};
-__device__ __forceinline__ dd mk(double v, double d) { return dd{v, d}; }
-__device__ __forceinline__ dd operator+(dd a, dd b) { return mk(a.v + b.v, a.d + b.d); }
-__device__ __forceinline__ dd operator-(dd a, dd b) { return mk(a.v - b.v, a.d - b.d); }
-__device__ __forceinline__ dd operator-(dd a) { return mk(-a.v, -a.d); }
-__device__ __forceinline__ dd operator*(dd a, dd b) { return mk(a.v * b.v, a.d * b.v + a.v * b.d); }
-__device__ __forceinline__ dd operator/(dd a, dd b) {
-  return mk(a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v));
-}
-__device__ __forceinline__ dd dsqrt(dd a) {
-  double s = sqrt(a.v);
-  return mk(s, a.d / (2.0 * s));
-}
-__device__ __forceinline__ dd dacos(dd a) { return mk(acos(a.v), -a.d / sqrt(1.0 - a.v * a.v)); }
-__device__ __forceinline__ void dcross(const dd a[3], const dd b[3], dd c[3]) {
-  c[0] = a[1] * b[2] - a[2] * b[1];
-  c[1] = a[2] * b[0] - a[0] * b[2];
-  c[2] = a[0] * b[1] - a[1] * b[0];
-}
-__device__ __forceinline__ dd ddot(const dd a[3], const dd b[3]) {
-  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
-}
-
-// Bonds O-H1, O-H2 and angle H1-O-H2 of one molecule (water/water.jl:915-968, 1279-1422).
-// The angle's HVP is the tangent part of its force evaluated on dual numbers seeded with u:
-// (dF/dx) u = (dF/dx)^T u because dF/dx is minus the (symmetric) Hessian of the energy.
-template <bool ADJ>
-__device__ void bonded(const SysParams& p, const double* xm, const double* um, const double* wm,
-                       double F[9], double h[9], double vj[6], int* flags) {
-  double O[3] = {xm[0], xm[1], xm[2]};
-  for (int hh = 1; hh <= 2; ++hh) {
-    double d[3];
-    for (int k = 0; k < 3; ++k) d[k] = mic1(xm[3 * hh + k] - O[k], p.L[k], p.invL[k]);
-    double r = sqrt(r2_of(d[0], d[1], d[2]));
-    if (r > p.hext) atomicOr(&flags[FLAG_HEXTENT], 1);
-    double n[3] = {d[0] / r, d[1] / r, d[2] / r};
-    double disp = r - p.r0;
-    double Fv = -p.kb * disp;
-    for (int k = 0; k < 3; ++k) {
-      F[k] -= Fv * n[k];
-      F[3 * hh + k] += Fv * n[k];
-    }
-    if (ADJ) {
-      double du[3], dot = 0.0;
-      for (int k = 0; k < 3; ++k) {
-        du[k] = um[k] - um[3 * hh + k];
-        dot += n[k] * du[k];
-      }
-      double Fr = Fv / r, Fp = -p.kb;
-      for (int k = 0; k < 3; ++k) {
-        double hv = Fr * du[k] + (Fp - Fr) * dot * n[k];
-        h[k] += hv;
-        h[3 * hh + k] -= hv;
-        double dw = wm[k] - wm[3 * hh + k];
-        vj[3] += disp * n[k] * dw;
-        vj[4] += -p.kb * n[k] * dw;
-      }
-    }
-  }
-  // angle, i = H1, j = O (centre), k = H2
-  dd a[3], b[3];
-  for (int k = 0; k < 3; ++k) {
-    a[k] = mk(mic1(xm[3 + k] - O[k], p.L[k], p.invL[k]), ADJ ? um[3 + k] - um[k] : 0.0);
-    b[k] = mk(mic1(xm[6 + k] - O[k], p.L[k], p.invL[k]), ADJ ? um[6 + k] - um[k] : 0.0);
-  }
-  dd cr[3];
-  dcross(a, b, cr);
-  if (cr[0].v == 0.0 && cr[1].v == 0.0 && cr[2].v == 0.0) return;
-  dd rji = dsqrt(ddot(a, a)), rjk = dsqrt(ddot(b, b));
-  dd p1u[3], p3u[3], mb[3];
-  dcross(a, cr, p1u);
-  for (int k = 0; k < 3; ++k) mb[k] = -b[k];
-  dcross(mb, cr, p3u);
-  dd p1n = dsqrt(ddot(p1u, p1u)), p3n = dsqrt(ddot(p3u, p3u));
-  dd cs = ddot(a, b) / (rji * rjk);
-  dd th = dacos(cs);
-  dd disp = mk(p.theta0, 0.0) - th;
-  dd Fa = mk(p.ka, 0.0) * disp;
-  for (int k = 0; k < 3; ++k) {
-    dd p1d = p1u[k] / p1n / rji, p3d = p3u[k] / p3n / rjk;
-    dd gi = Fa * p1d, gk = Fa * p3d, gj = -(gi + gk);
-    F[3 + k] += gi.v;
-    F[6 + k] += gk.v;
-    F[k] += gj.v;
-    if (ADJ) {
-      h[3 + k] += gi.d;
-      h[6 + k] += gk.d;
-      h[k] += gj.d;
-      vj[5] += disp.v * (p1d.v * wm[3 + k] + p3d.v * wm[6 + k] - (p1d.v + p3d.v) * wm[k]);
-    }
-  }
-}
 
 // ------------------------------------------------------------------ the pair kernel
 
@@ -231,8 +134,9 @@ __global__ __launch_bounds__(RS_BLOCK) void k_pair(PairArgs A) {
         R d[3];
         P::diff(p, own[a], pj[b], d[0], d[1], d[2]);
         const R r2 = P::r2(d[0], d[1], d[2]);
-        const bool in = (r2 <= rc2) && (r2 > (R)0);
-        const R ri = P::rinv(in ? r2 : (R)1);
+        // r2 == 0 (coincident atoms) within the cutoff gives non-finite forces, which are flagged
+        const bool in = r2 <= rc2;
+        const R ri = P::rinv(r2);
         const R ri2 = ri * ri, ri3 = ri2 * ri;
         const R q = kq[(a > 0) + (b > 0)];
         const R fr = in ? q * (ri3 - krf2) : (R)0;
@@ -289,15 +193,11 @@ __global__ __launch_bounds__(RS_BLOCK) void k_pair(PairArgs A) {
     double vj[6] = {0, 0, 0, 0, 0, 0};
     if (act) {
       const int m = A.order[i];
-      const double* xm = A.x + 9 * (size_t)m;
-      double um[9], wm[9];
+      double* Fm = A.F + 9 * (size_t)m;
       if (ADJ) {
+        double wm[9];
 #pragma unroll
-        for (int t = 0; t < 9; ++t) {
-          const int ty = t < 3 ? 0 : 1;
-          um[t] = A.csrc[9 * (size_t)m + t] * A.cscale[ty];
-          wm[t] = A.wsrc[9 * (size_t)m + t] * A.wscale[ty];
-        }
+        for (int t = 0; t < 9; ++t) wm[t] = A.wsrc[9 * (size_t)m + t] * A.wscale[t < 3 ? 0 : 1];
         // pair-part VJPs on this molecule's atoms: F_RF = F_pair - F_LJ (LJ acts on O only)
         double sq = 0.0, se = 0.0, ss = 0.0;
 #pragma unroll
@@ -310,20 +210,20 @@ __global__ __launch_bounds__(RS_BLOCK) void k_pair(PairArgs A) {
         vj[0] = ss;
         vj[1] = p.eps != 0.0 ? se / p.eps : 0.0;
         vj[2] = 2.0 * sq / p.qH;
+        vj[3] = A.vjb[3 * (size_t)m];
+        vj[4] = A.vjb[3 * (size_t)m + 1];
+        vj[5] = A.vjb[3 * (size_t)m + 2];
+        double* hm = A.h + 9 * (size_t)m;
+#pragma unroll
+        for (int t = 0; t < 9; ++t) hm[t] += hd[t];
       }
-      bonded<ADJ>(p, xm, um, wm, Fd, hd, vj, A.flags);
-      // staleness of the list: O displacement since the build
-      const double4 ob = A.O64[i];
-      double dx = mic1(xm[0] - ob.x, p.L[0], p.invL[0]);
-      double dy = mic1(xm[1] - ob.y, p.L[1], p.invL[1]);
-      double dz = mic1(xm[2] - ob.z, p.L[2], p.invL[2]);
-      if (r2_of(dx, dy, dz) > p.skin_half2) atomicOr(&A.flags[FLAG_STALE], 1);
+      // F was seeded with the molecule's bonded forces; add the pair forces
       bool fin = true;
 #pragma unroll
       for (int t = 0; t < 9; ++t) {
-        A.F[9 * (size_t)m + t] = Fd[t];
-        fin = fin && isfinite(Fd[t]);
-        if (ADJ) A.h[9 * (size_t)m + t] = hd[t];
+        const double f = Fm[t] + Fd[t];
+        Fm[t] = f;
+        fin = fin && isfinite(f);
       }
       if (!fin) atomicOr(&A.flags[FLAG_NONFINITE], 1);
     }
@@ -342,6 +242,23 @@ __global__ __launch_bounds__(RS_BLOCK) void k_pair(PairArgs A) {
   }
 }
 
+// dL/dθ += Σ_blocks of the adjoint pair kernel's θ-VJP partials: one block per parameter, each
+// summing the partials in block order (deterministic; the kernel boundary orders it after them).
+__global__ __launch_bounds__(RS_BLOCK) void k_vjp_reduce(const double* __restrict__ slots, int nslots,
+                                                         double* dth) {
+  __shared__ double red[RS_BLOCK];
+  const int k = blockIdx.x;
+  double acc = 0.0;
+  for (int q = threadIdx.x; q < nslots; q += RS_BLOCK) acc += slots[(size_t)q * 8 + k];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = RS_BLOCK / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) dth[k] += red[0];
+}
+
 template <class P, bool ADJ>
 void dispatch(const Launch& L, const PairArgs& a) {
   int g = pair_blocks(L.p);
@@ -353,21 +270,16 @@ void dispatch(const Launch& L, const PairArgs& a) {
 
 }  // namespace
 
-void launch_pair(const Launch& L, bool adjoint, const double* csrc, double cscale_O, double cscale_H,
-                 const double* wsrc, double wscale_O, double wscale_H) {
+void launch_pair(const Launch& L, bool adjoint, const double* wsrc, double wscale_O, double wscale_H) {
   PairArgs a;
   a.p = L.p;
   a.nbr = L.b.nbr;
   a.ncnt = L.b.ncnt;
   a.order = L.b.order;
-  a.x = L.b.x;
-  a.csrc = csrc;
   a.wsrc = wsrc;
-  a.cscale[0] = cscale_O;
-  a.cscale[1] = cscale_H;
   a.wscale[0] = wscale_O;
   a.wscale[1] = wscale_H;
-  a.O64 = L.b.O64;
+  a.vjb = L.b.vjb;
   a.F = L.b.F;
   a.h = L.b.h;
   a.slots = L.b.slots;
@@ -383,4 +295,8 @@ void launch_pair(const Launch& L, bool adjoint, const double* csrc, double cscal
     if (adjoint) dispatch<P64, true>(L, a);
     else dispatch<P64, false>(L, a);
   }
+}
+
+void launch_vjp_reduce(const Launch& L) {
+  k_vjp_reduce<<<6, RS_BLOCK, 0, L.s>>>(L.b.slots, pair_blocks(L.p), L.b.dth);
 }

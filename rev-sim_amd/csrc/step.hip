@@ -1,16 +1,20 @@
 // step.hip — streaming kernels of the forward Langevin step and of the reverse-time adjoint.
 //
-// Forward step n (forward_sim loop, water/water.jl:1030-1037), FP64 state, one thread per atom:
+// Forward step n (forward_sim loop, water/water.jl:1030-1037), FP64 state, one thread per molecule:
 //   v += F dt/m; x += v dt/2; v = e v + s ξ_n; x += v dt/2; wrap       e = exp(-γdt), s = √(1-e²)
 // Reverse step (reverse_sim, water/water.jl:1496-1521), fused as [part 2 of step n+1][part 1 of n]:
-//   part 2: v -= F dt/m; B = e B + dt² h; A += B; dL/dθ += Σ_blocks VJP partials
+//   part 2: v -= F dt/m; B = e B + dt² h; A += B   (dL/dθ is accumulated by the pair kernel)
 //   part 1: x -= v dt/2; v = (v - s ξ_n)/e; x -= v dt/2; wrap; D = e D + dt² h (1+e)/2; C += D
 // ξ_n is regenerated from the Philox counter (step n, atom) — the same bits forward and reverse.
 // Every thread also writes its atom's slot of the sorted pair inputs for the next pair kernel.
 // Expressions follow the reference's broadcast order and are evaluated without contraction.
 #include "internal.h"
+#include "bonded.h"
 
 namespace {
+
+// Per-molecule kernels run one wave per block so that the few waves there are spread over all CUs.
+constexpr int MOL_BLOCK = 64;
 
 __device__ __forceinline__ void write_sorted(const SysParams& p, int precision, int slot,
                                              const double x[3], int4* posq, double4* pos64) {
@@ -27,130 +31,166 @@ __global__ void k_wrap(SysParams p, double* x) {
   x[t] = wrap1(x[t], p.L[t % 3]);
 }
 
-__global__ __launch_bounds__(RS_BLOCK) void k_step_fwd(SysParams p, int precision, DevBufs b,
-                                                        int step, uint64_t seed, double dt,
-                                                        double e, double s, double kT) {
-#pragma clang fp contract(off)
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= p.N) return;
-  const int m = t / 3, a = t - 3 * m;
-  const double mass = a == 0 ? p.mO : p.mH;
-  double xi[3];
+__device__ __forceinline__ void atom_noise(const DevBufs& b, const SysParams& p, int step,
+                                           uint64_t seed, int atom, double mass, double kT,
+                                           double xi[3]) {
   if (b.ext_noise) {
-    const double* src = b.ext_noise + (size_t)(step - 1) * 3 * p.N + 3 * t;
+    const double* src = b.ext_noise + (size_t)(step - 1) * 3 * p.N + 3 * (size_t)atom;
     xi[0] = src[0];
     xi[1] = src[1];
     xi[2] = src[2];
   } else {
     double nn[3];
-    normals3(seed, step, t, nn);
+    normals3(seed, step, atom, nn);
     const double sd = sqrt(kT / mass);
     xi[0] = sd * nn[0];
     xi[1] = sd * nn[1];
     xi[2] = sd * nn[2];
   }
-  double xn[3];
-  bool fin = true;
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    const size_t o = 3 * (size_t)t + k;
-    double v = b.v[o] + (b.F[o] * dt) / mass;
-    double x = b.x[o] + (v * dt) / 2.0;
-    v = v * e + xi[k] * s;
-    x = x + (v * dt) / 2.0;
-    x = wrap1(x, p.L[k]);
-    b.v[o] = v;
-    b.x[o] = x;
-    xn[k] = x;
-    fin = fin && isfinite(x) && isfinite(v);
-  }
-  if (!fin) atomicOr(&b.flags[FLAG_NONFINITE], 1);
-  write_sorted(p, precision, 3 * b.rank[m] + a, xn, b.posq, b.pos64);
 }
 
-__global__ __launch_bounds__(RS_BLOCK) void k_step_rev(SysParams p, int precision, DevBufs b,
+// Forward step n, one thread per molecule; then the bonded forces at the new positions, which
+// seed F for the next pair kernel (it adds the pair forces).
+__global__ __launch_bounds__(MOL_BLOCK) void k_step_fwd(SysParams p, int precision, DevBufs b,
+                                                        int step, uint64_t seed, double dt,
+                                                        double e, double s, double kT) {
+#pragma clang fp contract(off)
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= p.M) return;
+  double xm[9];
+  bool fin = true;
+  const int rk = b.rank[m];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const double mass = a == 0 ? p.mO : p.mH;
+    double xi[3];
+    atom_noise(b, p, step, seed, 3 * m + a, mass, kT, xi);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const size_t o = 9 * (size_t)m + 3 * a + k;
+      double v = b.v[o] + (b.F[o] * dt) / mass;
+      double x = b.x[o] + (v * dt) / 2.0;
+      v = v * e + xi[k] * s;
+      x = x + (v * dt) / 2.0;
+      x = wrap1(x, p.L[k]);
+      b.v[o] = v;
+      b.x[o] = x;
+      xm[3 * a + k] = x;
+      fin = fin && isfinite(x) && isfinite(v);
+    }
+    write_sorted(p, precision, 3 * rk + a, xm + 3 * a, b.posq, b.pos64);
+  }
+  if (!fin) atomicOr(&b.flags[FLAG_NONFINITE], 1);
+  double F[9], h[9], vj[3];
+  bonded_mol<false>(p, xm, nullptr, nullptr, F, h, vj, b.flags, b.O64[rk]);
+#pragma unroll
+  for (int t = 0; t < 9; ++t) b.F[9 * (size_t)m + t] = F[t];
+}
+
+// Reverse step, one thread per molecule: [part 2 of step n+1][part 1 of step n]; after part 1 the
+// bonded forces, HVP and θ-VJP at the reconstructed positions seed F, h and vjb for the adjoint
+// pair kernel of step n.
+__global__ __launch_bounds__(MOL_BLOCK) void k_step_rev(SysParams p, int precision, DevBufs b,
                                                         int do_r2, int do_r1, int step,
                                                         uint64_t seed, double dt, double e,
                                                         double s, double kT, int nslots, int clip) {
 #pragma clang fp contract(off)
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
   const double dt2 = dt * dt;
-  if (do_r2 && blockIdx.x == 0) {
-    // dL/dθ += per-block VJP partials, summed in block order (deterministic)
-    __shared__ double red[RS_BLOCK];
-    for (int k = 0; k < 6; ++k) {
-      double acc = 0.0;
-      for (int q = threadIdx.x; q < nslots; q += RS_BLOCK) acc += b.slots[(size_t)q * 8 + k];
-      red[threadIdx.x] = acc;
-      __syncthreads();
-      for (int w = RS_BLOCK / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
-      }
-      if (threadIdx.x == 0) b.dth[k] += red[0];
-      __syncthreads();
-    }
-  }
-  if (t >= p.N) return;
-  const int m = t / 3, a = t - 3 * m;
-  const double mass = a == 0 ? p.mO : p.mH;
+  if (m >= p.M) return;
   const double cs = clip ? b.scal[4] : 1.0;
-  double xi[3] = {0.0, 0.0, 0.0};
-  if (do_r1) {
-    if (b.ext_noise) {
-      const double* src = b.ext_noise + (size_t)(step - 1) * 3 * p.N + 3 * t;
-      xi[0] = src[0];
-      xi[1] = src[1];
-      xi[2] = src[2];
-    } else {
-      double nn[3];
-      normals3(seed, step, t, nn);
-      const double sd = sqrt(kT / mass);
-      xi[0] = sd * nn[0];
-      xi[1] = sd * nn[1];
-      xi[2] = sd * nn[2];
-    }
-  }
-  double xn[3], un[3];
+  const int rk = b.rank[m];
+  double xm[9], um[9], wm[9];
   bool fin = true;
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    const size_t o = 3 * (size_t)t + k;
-    double x = b.x[o], v = b.v[o], h = b.h[o];
-    if (clip) {
-      h = h * cs;
-      b.h[o] = h;
+  for (int a = 0; a < 3; ++a) {
+    const double mass = a == 0 ? p.mO : p.mH;
+    double xi[3] = {0.0, 0.0, 0.0};
+    if (do_r1) atom_noise(b, p, step, seed, 3 * m + a, mass, kT, xi);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int t = 3 * a + k;
+      const size_t o = 9 * (size_t)m + t;
+      double x = b.x[o], v = b.v[o], h = b.h[o], A = b.A[o];
+      if (clip) h = h * cs;
+      if (do_r2) {
+        v = v - (b.F[o] * dt) / mass;
+        double B = e * b.B[o] + dt2 * h;
+        b.B[o] = B;
+        A = A + B;
+        b.A[o] = A;
+      }
+      if (do_r1) {
+        x = x - (v * dt) / 2.0;
+        v = (v - xi[k] * s) / e;
+        x = x - (v * dt) / 2.0;
+        x = wrap1(x, p.L[k]);
+        double D = e * b.D[o] + ((dt2 * h) * (1.0 + e)) / 2.0;
+        double C = b.C[o] + D;
+        b.D[o] = D;
+        b.C[o] = C;
+        um[t] = C / mass;
+        wm[t] = A * (dt2 / (2.0 * mass));
+      } else if (clip) {
+        b.h[o] = h;
+      }
+      b.x[o] = x;
+      b.v[o] = v;
+      xm[t] = x;
+      fin = fin && isfinite(x) && isfinite(v);
     }
-    if (do_r2) {
-      v = v - (b.F[o] * dt) / mass;
-      double B = e * b.B[o] + dt2 * h;
-      b.B[o] = B;
-      b.A[o] = b.A[o] + B;
-    }
-    if (do_r1) {
-      x = x - (v * dt) / 2.0;
-      v = (v - xi[k] * s) / e;
-      x = x - (v * dt) / 2.0;
-      x = wrap1(x, p.L[k]);
-      double D = e * b.D[o] + ((dt2 * h) * (1.0 + e)) / 2.0;
-      double C = b.C[o] + D;
-      b.D[o] = D;
-      b.C[o] = C;
-      un[k] = C / mass;
-    }
-    b.x[o] = x;
-    b.v[o] = v;
-    xn[k] = x;
-    fin = fin && isfinite(x) && isfinite(v);
   }
   if (!fin) atomicOr(&b.flags[FLAG_NONFINITE], 1);
   if (do_r1) {
-    const int slot = 3 * b.rank[m] + a;
-    write_sorted(p, precision, slot, xn, b.posq, b.pos64);
-    if (precision == RS_PRECISION_FP32)
-      b.uf[slot] = make_float4((float)un[0], (float)un[1], (float)un[2], 0.f);
-    else
-      b.ud[slot] = make_double4(un[0], un[1], un[2], 0.0);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      write_sorted(p, precision, 3 * rk + a, xm + 3 * a, b.posq, b.pos64);
+      if (precision == RS_PRECISION_FP32)
+        b.uf[3 * rk + a] = make_float4((float)um[3 * a], (float)um[3 * a + 1], (float)um[3 * a + 2], 0.f);
+      else
+        b.ud[3 * rk + a] = make_double4(um[3 * a], um[3 * a + 1], um[3 * a + 2], 0.0);
+    }
+    double F[9], h[9], vj[3];
+    bonded_mol<true>(p, xm, um, wm, F, h, vj, b.flags, b.O64[rk]);
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+      b.F[9 * (size_t)m + t] = F[t];
+      b.h[9 * (size_t)m + t] = h[t];
+    }
+    b.vjb[3 * (size_t)m] = vj[0];
+    b.vjb[3 * (size_t)m + 1] = vj[1];
+    b.vjb[3 * (size_t)m + 2] = vj[2];
+  }
+}
+
+// Bonded terms of the current state alone (start of a trajectory and single evaluations):
+// F (and h, vjb) are seeded as in the step kernels. u = csrc*cscale, w = wsrc*wscale per atom type.
+template <bool ADJ>
+__global__ __launch_bounds__(MOL_BLOCK) void k_bonded(SysParams p, DevBufs b, const double* csrc,
+                                                      double cO, double cH, const double* wsrc,
+                                                      double wO, double wH) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= p.M) return;
+  double xm[9], um[9], wm[9];
+#pragma unroll
+  for (int t = 0; t < 9; ++t) {
+    xm[t] = b.x[9 * (size_t)m + t];
+    if (ADJ) {
+      um[t] = csrc[9 * (size_t)m + t] * (t < 3 ? cO : cH);
+      wm[t] = wsrc[9 * (size_t)m + t] * (t < 3 ? wO : wH);
+    }
+  }
+  double F[9], h[9], vj[3];
+  bonded_mol<ADJ>(p, xm, um, wm, F, h, vj, b.flags, b.O64[b.rank[m]]);
+#pragma unroll
+  for (int t = 0; t < 9; ++t) {
+    b.F[9 * (size_t)m + t] = F[t];
+    if (ADJ) b.h[9 * (size_t)m + t] = h[t];
+  }
+  if (ADJ) {
+    b.vjb[3 * (size_t)m] = vj[0];
+    b.vjb[3 * (size_t)m + 1] = vj[1];
+    b.vjb[3 * (size_t)m + 2] = vj[2];
   }
 }
 
@@ -222,13 +262,22 @@ void launch_wrap_state(const Launch& L) {
 
 void launch_step_fwd(const Launch& L, int step, uint64_t seed, double dt, double e, double s,
                      double kT) {
-  k_step_fwd<<<(L.p.N + RS_BLOCK - 1) / RS_BLOCK, RS_BLOCK, 0, L.s>>>(L.p, L.precision, L.b, step,
-                                                                       seed, dt, e, s, kT);
+  k_step_fwd<<<(L.p.M + MOL_BLOCK - 1) / MOL_BLOCK, MOL_BLOCK, 0, L.s>>>(L.p, L.precision, L.b, step,
+                                                                         seed, dt, e, s, kT);
+}
+
+void launch_bonded(const Launch& L, bool adjoint, const double* csrc, double cO, double cH,
+                   const double* wsrc, double wO, double wH) {
+  const int g = (L.p.M + MOL_BLOCK - 1) / MOL_BLOCK;
+  if (adjoint)
+    k_bonded<true><<<g, MOL_BLOCK, 0, L.s>>>(L.p, L.b, csrc, cO, cH, wsrc, wO, wH);
+  else
+    k_bonded<false><<<g, MOL_BLOCK, 0, L.s>>>(L.p, L.b, csrc, cO, cH, wsrc, wO, wH);
 }
 
 void launch_step_rev(const Launch& L, bool do_r2, bool do_r1, int step, uint64_t seed, double dt,
                      double e, double s, double kT, int nslots, bool clip) {
-  k_step_rev<<<(L.p.N + RS_BLOCK - 1) / RS_BLOCK, RS_BLOCK, 0, L.s>>>(
+  k_step_rev<<<(L.p.M + MOL_BLOCK - 1) / MOL_BLOCK, MOL_BLOCK, 0, L.s>>>(
       L.p, L.precision, L.b, do_r2, do_r1, step, seed, dt, e, s, kT, nslots, clip);
 }
 

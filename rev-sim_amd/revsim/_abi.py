@@ -77,7 +77,7 @@ def make_config(n_molecules: int, box, cutoff: float, **kw) -> RsConfig:
     for k in range(3):
         c.box[k] = b[k]
     c.cutoff = float(cutoff)
-    c.skin = 0.1
+    c.skin = 0.06
     c.nl_every = 10
     c.precision = RS_PRECISION_FP64
     c.dt = 0.001
