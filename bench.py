@@ -205,19 +205,33 @@ def main():
     }
 
     if not a.no_fp64_check and prec == revsim.RS_PRECISION_FP32:
-        cfg64 = revsim.make_config(wl["n_molecules"], wl["box"], wl["cutoff"],
-                                   precision=revsim.RS_PRECISION_FP64, **extra)
-        with revsim.Simulator(cfg64, device=0) as s64:
-            s64.set_theta(theta)
-            s64.set_target_rdf(target)
-            t = time.perf_counter()
-            l64, g64 = s64.value_and_grad(x, v, seed, n_steps)
-            log(f"[bench] fp64 path: {time.perf_counter() - t:.2f} s")
-        rel = np.abs(grad - g64) / np.maximum(np.abs(g64), 1e-300)
+        # dL/dθ of the FP32 path against the FP64 path (itself parity-tested against the FP64 CPU
+        # oracle) from the same start, at a well-conditioned horizon and at the full length.
+        errs = {}
+        for horizon in sorted({min(100, n_steps), n_steps}):
+            ex = dict(extra)
+            if ex.get("msd_window_steps", 0) > horizon:
+                ev = ex["msd_every"]
+                ex["msd_window_steps"] = max(2 * ev, (horizon // 2) // (2 * ev) * (2 * ev))
+            g = {}
+            for p_ in (revsim.RS_PRECISION_FP32, revsim.RS_PRECISION_FP64):
+                c_ = revsim.make_config(wl["n_molecules"], wl["box"], wl["cutoff"], precision=p_, **ex)
+                with revsim.Simulator(c_, device=0) as s_:
+                    s_.set_theta(theta)
+                    s_.set_target_rdf(target)
+                    g[p_] = s_.value_and_grad(x, v, seed, horizon)[1]
+            g32, g64 = g[revsim.RS_PRECISION_FP32], g[revsim.RS_PRECISION_FP64]
+            errs[str(horizon)] = {
+                "normwise": float(np.linalg.norm(g32 - g64) / np.linalg.norm(g64)),
+                "per_param": (np.abs(g32 - g64) / np.maximum(np.abs(g64), 1e-300)).tolist(),
+                "grad_norm_fp64": float(np.linalg.norm(g64)),
+            }
+            log(f"[bench] dL/dθ FP32 vs FP64 at {horizon} steps: {errs[str(horizon)]['normwise']:.3e}")
         out["grad_rel_err_vs_fp64"] = {
-            "normwise": float(np.linalg.norm(grad - g64) / np.linalg.norm(g64)),
-            "per_param": rel.tolist(),
-            "reference": "same trajectory with FP64 pair arithmetic (parity-tested against the FP64 CPU oracle)",
+            "by_horizon_steps": errs,
+            "reference": "same start, FP64 pair arithmetic (parity-tested against the FP64 CPU oracle)",
+            "note": "beyond ~200 steps the adjoint grows like exp(lambda t) (chaotic MD) and FP32/FP64 "
+                    "gradients decorrelate; the reference trains with n_steps_trunc = 200",
         }
 
     if not a.no_cpu_baseline:

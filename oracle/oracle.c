@@ -79,8 +79,22 @@ typedef struct or_ctx {
   int nframes;
   int32_t rec_steps;
   uint64_t rec_seed;
+  /* radial-basis MLP on O-O pairs (BF16-valued weights held as doubles) */
+  int use_mlp;
+  double* mlp;      /* NW weights: W1[64][16], b1[64], W2[64][64], b2[64], w3[64], b3 */
+  double* dW;       /* dL/dW of the last reverse pass */
   char err[256];
 } or_ctx;
+
+#define MLP_NF 16
+#define MLP_NH 64
+#define MLP_NW (MLP_NH * MLP_NF + MLP_NH + MLP_NH * MLP_NH + MLP_NH + MLP_NH + 1)
+#define MLP_W1 0
+#define MLP_B1 (MLP_NH * MLP_NF)
+#define MLP_W2 (MLP_B1 + MLP_NH)
+#define MLP_B2 (MLP_W2 + MLP_NH * MLP_NH)
+#define MLP_W3 (MLP_B2 + MLP_NH)
+#define MLP_B3 (MLP_W3 + MLP_NH)
 
 static void set_err(or_ctx* o, const char* m) {
   if (o) snprintf(o->err, sizeof(o->err), "%s", m);
@@ -163,6 +177,8 @@ static void free_records(or_ctx* o) {
 int or_destroy(or_ctx* o) {
   if (!o) return 0;
   free_records(o);
+  free(o->mlp);
+  free(o->dW);
   free(o->snaps);
   free(o->mass3);
   free(o->pa);
@@ -449,6 +465,182 @@ static int angle_dual(const or_ctx* o, const double* xi, const double* xj, const
   return 1;
 }
 
+/* ---------------------------------------------------------------- radial-basis MLP (O-O pairs) */
+
+/* Round a float to bfloat16 (round to nearest, ties to even) and return it as a double: the MLP's
+   weights are BF16 values. */
+static double bf16_round(float f) {
+  uint32_t b;
+  memcpy(&b, &f, 4);
+  uint32_t lsb = (b >> 16) & 1u;
+  b += 0x7FFFu + lsb;
+  b &= 0xFFFF0000u;
+  float r;
+  memcpy(&r, &b, 4);
+  return (double)r;
+}
+
+int or_set_mlp(or_ctx* o, const float* w, int64_t n) {
+  if (n != MLP_NW) return RS_E_CONFIG;
+  if (!o->mlp) o->mlp = (double*)malloc(sizeof(double) * MLP_NW);
+  for (int i = 0; i < MLP_NW; ++i) o->mlp[i] = bf16_round(w[i]);
+  o->use_mlp = 1;
+  return 0;
+}
+
+int or_mlp_size(void) { return MLP_NW; }
+
+/* Unrounded weights (finite-difference tests of the weight gradient). */
+int or_set_mlp_f64(or_ctx* o, const double* w, int64_t n) {
+  if (n != MLP_NW) return RS_E_CONFIG;
+  if (!o->mlp) o->mlp = (double*)malloc(sizeof(double) * MLP_NW);
+  memcpy(o->mlp, w, sizeof(double) * MLP_NW);
+  o->use_mlp = 1;
+  return 0;
+}
+
+/* Features φ_k(r) = exp(-(r-μ_k)^2/(2w^2)) fc(r), μ_k = 0.2 + k 0.8/15, w = 0.05 nm,
+   fc(r) = (cos(π r/r_c) + 1)/2, with their first and second r-derivatives. */
+static void rbf(double r, double rc, double p[MLP_NF], double p1[MLP_NF], double p2[MLP_NF]) {
+  const double w = 0.05, iw2 = 1.0 / (w * w);
+  const double a = M_PI / rc;
+  double fc = 0.5 * (cos(a * r) + 1.0), fc1 = -0.5 * a * sin(a * r), fc2 = -0.5 * a * a * cos(a * r);
+  for (int k = 0; k < MLP_NF; ++k) {
+    double mu = 0.2 + k * (0.8 / 15.0);
+    double d = r - mu;
+    double g = exp(-0.5 * d * d * iw2);
+    double g1 = -d * iw2 * g;
+    double g2 = (d * d * iw2 - 1.0) * iw2 * g;
+    p[k] = g * fc;
+    p1[k] = g1 * fc + g * fc1;
+    p2[k] = g2 * fc + 2.0 * g1 * fc1 + g * fc2;
+  }
+}
+
+/* SiLU σ(z) = z s(z) and its first three derivatives. */
+static void silu4(double z, double* f0, double* f1, double* f2, double* f3) {
+  double s = 1.0 / (1.0 + exp(-z));
+  double s1 = s * (1.0 - s), s2 = s1 * (1.0 - 2.0 * s), s3 = s2 * (1.0 - 2.0 * s) - 2.0 * s1 * s1;
+  *f0 = z * s;
+  *f1 = s + z * s1;
+  *f2 = 2.0 * s1 + z * s2;
+  *f3 = 3.0 * s2 + z * s3;
+}
+
+typedef struct {
+  double p[MLP_NF], p1[MLP_NF], p2[MLP_NF];
+  double z1[MLP_NH], z1a[MLP_NH], z1b[MLP_NH], s1[4][MLP_NH];
+  double a1[MLP_NH], a1a[MLP_NH], a1b[MLP_NH];
+  double z2[MLP_NH], z2a[MLP_NH], z2b[MLP_NH], s2[4][MLP_NH];
+  double a2[MLP_NH], a2a[MLP_NH], a2b[MLP_NH];
+  double u, u1, u2;
+} mlp_jet;
+
+/* u(r), u'(r), u''(r) by forward-mode (second-order jets) through 16 -> 64 -> 64 -> 1 SiLU. */
+static void mlp_forward(const double* W, double r, double rc, mlp_jet* J) {
+  rbf(r, rc, J->p, J->p1, J->p2);
+  for (int n = 0; n < MLP_NH; ++n) {
+    double z = W[MLP_B1 + n], za = 0.0, zb = 0.0;
+    for (int f = 0; f < MLP_NF; ++f) {
+      double wv = W[MLP_W1 + n * MLP_NF + f];
+      z += wv * J->p[f];
+      za += wv * J->p1[f];
+      zb += wv * J->p2[f];
+    }
+    J->z1[n] = z;
+    J->z1a[n] = za;
+    J->z1b[n] = zb;
+    silu4(z, &J->s1[0][n], &J->s1[1][n], &J->s1[2][n], &J->s1[3][n]);
+    J->a1[n] = J->s1[0][n];
+    J->a1a[n] = J->s1[1][n] * za;
+    J->a1b[n] = J->s1[2][n] * za * za + J->s1[1][n] * zb;
+  }
+  J->u = W[MLP_B3];
+  J->u1 = 0.0;
+  J->u2 = 0.0;
+  for (int m = 0; m < MLP_NH; ++m) {
+    double z = W[MLP_B2 + m], za = 0.0, zb = 0.0;
+    for (int n = 0; n < MLP_NH; ++n) {
+      double wv = W[MLP_W2 + m * MLP_NH + n];
+      z += wv * J->a1[n];
+      za += wv * J->a1a[n];
+      zb += wv * J->a1b[n];
+    }
+    J->z2[m] = z;
+    J->z2a[m] = za;
+    J->z2b[m] = zb;
+    silu4(z, &J->s2[0][m], &J->s2[1][m], &J->s2[2][m], &J->s2[3][m]);
+    J->a2[m] = J->s2[0][m];
+    J->a2a[m] = J->s2[1][m] * za;
+    J->a2b[m] = J->s2[2][m] * za * za + J->s2[1][m] * zb;
+    J->u += W[MLP_W3 + m] * J->a2[m];
+    J->u1 += W[MLP_W3 + m] * J->a2a[m];
+    J->u2 += W[MLP_W3 + m] * J->a2b[m];
+  }
+}
+
+/* dW += g1 ∂u'/∂W + g2 ∂u''/∂W (reverse mode through the jets of mlp_forward). */
+static void mlp_backward(const double* W, const mlp_jet* J, double g1, double g2, double* dW) {
+  double zb2[MLP_NH], zb2a[MLP_NH], zb2b[MLP_NH];
+  for (int m = 0; m < MLP_NH; ++m) {
+    double w3 = W[MLP_W3 + m];
+    dW[MLP_W3 + m] += g1 * J->a2a[m] + g2 * J->a2b[m];
+    double ab = g1 * w3, abb = g2 * w3; /* ā2', ā2'' */
+    double sp1 = J->s2[1][m], sp2 = J->s2[2][m], sp3 = J->s2[3][m];
+    double za = J->z2a[m], zbv = J->z2b[m];
+    zb2b[m] = abb * sp1;
+    zb2a[m] = abb * sp2 * 2.0 * za + ab * sp1;
+    zb2[m] = abb * (sp3 * za * za + sp2 * zbv) + ab * sp2 * za;
+  }
+  double ab1[MLP_NH], ab1a[MLP_NH], ab1b[MLP_NH];
+  for (int n = 0; n < MLP_NH; ++n) ab1[n] = ab1a[n] = ab1b[n] = 0.0;
+  for (int m = 0; m < MLP_NH; ++m) {
+    dW[MLP_B2 + m] += zb2[m];
+    for (int n = 0; n < MLP_NH; ++n) {
+      double wv = W[MLP_W2 + m * MLP_NH + n];
+      dW[MLP_W2 + m * MLP_NH + n] += zb2[m] * J->a1[n] + zb2a[m] * J->a1a[n] + zb2b[m] * J->a1b[n];
+      ab1[n] += wv * zb2[m];
+      ab1a[n] += wv * zb2a[m];
+      ab1b[n] += wv * zb2b[m];
+    }
+  }
+  for (int n = 0; n < MLP_NH; ++n) {
+    double sp1 = J->s1[1][n], sp2 = J->s1[2][n], sp3 = J->s1[3][n];
+    double za = J->z1a[n], zbv = J->z1b[n];
+    double zb1b = ab1b[n] * sp1;
+    double zb1a = ab1b[n] * sp2 * 2.0 * za + ab1a[n] * sp1;
+    double zb1 = ab1b[n] * (sp3 * za * za + sp2 * zbv) + ab1a[n] * sp2 * za + ab1[n] * sp1;
+    dW[MLP_B1 + n] += zb1;
+    for (int f = 0; f < MLP_NF; ++f)
+      dW[MLP_W1 + n * MLP_NF + f] += zb1 * J->p[f] + zb1a * J->p1[f] + zb1b * J->p2[f];
+  }
+}
+
+/* u, u', u'' at r (for tests of the library's tabulation). */
+int or_mlp_eval(or_ctx* o, const double* r, int32_t n, double* u, double* u1, double* u2) {
+  if (!o->use_mlp) return RS_E_CONFIG;
+  mlp_jet J;
+  for (int i = 0; i < n; ++i) {
+    mlp_forward(o->mlp, r[i], o->cfg.cutoff, &J);
+    if (u) u[i] = J.u;
+    if (u1) u1[i] = J.u1;
+    if (u2) u2[i] = J.u2;
+  }
+  return 0;
+}
+
+/* dW of Σ_i (g1_i u'(r_i) + g2_i u''(r_i)) (for tests of the library's weight backprop). */
+int or_mlp_vjp(or_ctx* o, const double* r, const double* g1, const double* g2, int32_t n, double* dW) {
+  if (!o->use_mlp) return RS_E_CONFIG;
+  memset(dW, 0, sizeof(double) * MLP_NW);
+  mlp_jet J;
+  for (int i = 0; i < n; ++i) {
+    mlp_forward(o->mlp, r[i], o->cfg.cutoff, &J);
+    mlp_backward(o->mlp, &J, g1[i], g2 ? g2[i] : 0.0, dW);
+  }
+  return 0;
+}
+
 /* ---------------------------------------------------------------- forces and adjoint terms */
 
 /* One pass over all interactions. With c == NULL only forces (and energy) are produced
@@ -457,7 +649,7 @@ static int angle_dual(const or_ctx* o, const double* xi, const double* xj, const
    θ = (σ_O, ε_O, q_H, k_b, r0, k_a). Per-thread buffers followed by a reduction, as in the
    reference (water/water.jl:879-885,970-972,1424-1430). */
 static void eval_terms(or_ctx* o, const double* x, const double* c, const double* w, double* F,
-                       double* h, double* vjp, double* energy) {
+                       double* h, double* vjp, double* energy, double* dW) {
   const int N3 = o->N3;
   const double sig = o->theta[0], eps = o->theta[1], qH = o->theta[2];
   const double kb = o->theta[3], r0 = o->theta[4], ka = o->theta[5];
@@ -472,6 +664,8 @@ static void eval_terms(or_ctx* o, const double* x, const double* c, const double
   double* hb = adj ? (double*)calloc((size_t)N3 * nth, sizeof(double)) : NULL;
   double* vb = (double*)calloc((size_t)NTH * nth, sizeof(double));
   double* eb = (double*)calloc((size_t)nth, sizeof(double));
+  const int mlp_on = o->use_mlp;
+  double* dWb = (adj && mlp_on && dW) ? (double*)calloc((size_t)MLP_NW * nth, sizeof(double)) : NULL;
   double* u = NULL;
   if (adj) {
     u = (double*)malloc(sizeof(double) * N3);
@@ -487,7 +681,9 @@ static void eval_terms(or_ctx* o, const double* x, const double* c, const double
     double* Ft = Fb + (size_t)N3 * t;
     double* ht = adj ? hb + (size_t)N3 * t : NULL;
     double* vt = vb + NTH * t;
+    double* dWt = dWb ? dWb + (size_t)MLP_NW * t : NULL;
     double et = 0.0;
+    mlp_jet J;
 #pragma omp for schedule(static)
     for (int64_t p = 0; p < np; ++p) {
       int a = o->pa[p], b = o->pb[p];
@@ -516,6 +712,18 @@ static void eval_terms(or_ctx* o, const double* x, const double* c, const double
         dFde = Flj / eps;                                            /* times ndr */
         Fv += Flj;
         if (energy) et += 4.0 * eps * (twelve - six);
+      }
+      if (aO && bO && mlp_on) {
+        /* MLP pair energy u(r): f = -u'(r), f' = -u''(r) */
+        mlp_forward(o->mlp, r, o->cfg.cutoff, &J);
+        Fv += -J.u1;
+        Fp += -J.u2;
+        if (energy) et += J.u;
+        if (dWt) {
+          double wd = 0.0;
+          for (int k = 0; k < 3; ++k) wd += ndr[k] * (w[3 * b + k] - w[3 * a + k]);
+          mlp_backward(o->mlp, &J, -wd, 0.0, dWt); /* dF/dW . w = -(du'/dW) r̂.(w_b - w_a) */
+        }
       }
       for (int k = 0; k < 3; ++k) {
         Ft[3 * a + k] -= Fv * ndr[k];
@@ -629,6 +837,14 @@ static void eval_terms(or_ctx* o, const double* x, const double* c, const double
     for (int t = 0; t < nth; ++t) s += eb[t];
     *energy = s;
   }
+  if (dWb) {
+    for (int k = 0; k < MLP_NW; ++k) {
+      double s = 0.0;
+      for (int t = 0; t < nth; ++t) s += dWb[(size_t)MLP_NW * t + k];
+      dW[k] += s;
+    }
+    free(dWb);
+  }
   free(Fb);
   free(hb);
   free(vb);
@@ -638,14 +854,23 @@ static void eval_terms(or_ctx* o, const double* x, const double* c, const double
 
 int or_forces(or_ctx* o, const double* x, double* F, double* energy) {
   build_list(o, x);
-  eval_terms(o, x, NULL, NULL, F, NULL, NULL, energy);
+  eval_terms(o, x, NULL, NULL, F, NULL, NULL, energy, NULL);
   return 0;
 }
 
 int or_adjoint_terms(or_ctx* o, const double* x, const double* c, const double* w, double* F,
                      double* h, double* vjp6) {
   build_list(o, x);
-  eval_terms(o, x, c, w, F, h, vjp6, NULL);
+  eval_terms(o, x, c, w, F, h, vjp6, NULL, NULL);
+  return 0;
+}
+
+/* As or_adjoint_terms, plus dW = sum_dof w_dof dF_dof/dW for the MLP weights. */
+int or_adjoint_terms_mlp(or_ctx* o, const double* x, const double* c, const double* w, double* F,
+                         double* h, double* vjp6, double* dW) {
+  build_list(o, x);
+  memset(dW, 0, sizeof(double) * MLP_NW);
+  eval_terms(o, x, c, w, F, h, vjp6, NULL, dW);
   return 0;
 }
 
@@ -1044,7 +1269,7 @@ static int forward_impl(or_ctx* o, const double* x0, const double* v0, uint64_t 
   build_list(o, x);
   double lrdf = 0.0;
   for (int n = 1; n <= n_steps; ++n) {
-    eval_terms(o, x, NULL, NULL, F, NULL, NULL, NULL);
+    eval_terms(o, x, NULL, NULL, F, NULL, NULL, NULL, NULL);
     or_noise(o, seed, n, xi);
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < N3; ++i) {
@@ -1147,6 +1372,10 @@ int or_reverse(or_ctx* o, const double* xN, const double* vN, uint64_t seed, int
     Dd[i] = g[i] * (1.0 + 2.0 * e + e * e) / (2.0 * e);
   }
   double th[NTH] = {0};
+  if (o->use_mlp) {
+    if (!o->dW) o->dW = (double*)malloc(sizeof(double) * MLP_NW);
+    memset(o->dW, 0, sizeof(double) * MLP_NW);
+  }
   build_list(o, x);
   int since_build = 0, trunc_count = 0;
   for (int n = n_steps; n >= 1; --n) {
@@ -1169,7 +1398,7 @@ int or_reverse(or_ctx* o, const double* xN, const double* vN, uint64_t seed, int
         since_build = 0;
       }
       double vj[NTH];
-      eval_terms(o, x, C, w, F, h, vj, NULL);
+      eval_terms(o, x, C, w, F, h, vj, NULL, o->dW);
       if (o->cfg.clip_norm > 0.0 && isfinite(o->cfg.clip_norm)) {
         double nn = 0.0;
         for (int i = 0; i < N3; ++i) nn += h[i] * h[i];
@@ -1281,6 +1510,10 @@ int or_value_and_grad_stored(or_ctx* o, const double* x0, const double* v0, uint
   }
   seed_grad(o, &p, n_steps, traj + (size_t)n_steps * N3, dLdf, lx, 1);
   double th[NTH] = {0};
+  if (o->use_mlp) {
+    if (!o->dW) o->dW = (double*)malloc(sizeof(double) * MLP_NW);
+    memset(o->dW, 0, sizeof(double) * MLP_NW);
+  }
   for (int n = n_steps; n >= 1; --n) {
     const double* xp = traj + (size_t)(n - 1) * N3;
     for (int i = 0; i < N3; ++i) {
@@ -1292,7 +1525,7 @@ int or_value_and_grad_stored(or_ctx* o, const double* x0, const double* v0, uint
     }
     double vj[NTH];
     build_list(o, xp);
-    eval_terms(o, xp, cc, hh, F, g, vj, NULL);
+    eval_terms(o, xp, cc, hh, F, g, vj, NULL, o->dW);
     for (int i = 0; i < N3; ++i) lx[i] += g[i];
     for (int k = 0; k < NTH; ++k) th[k] += vj[k];
     int t = n - 1;
@@ -1315,6 +1548,13 @@ int or_value_and_grad_stored(or_ctx* o, const double* x0, const double* v0, uint
 }
 
 double or_list_radius(const or_ctx* o) { return o->rlist; }
+
+/* dL/dW of the last reverse pass (or_reverse / or_value_and_grad / or_value_and_grad_stored). */
+int or_mlp_grad(const or_ctx* o, double* dW) {
+  if (!o->use_mlp || !o->dW) return RS_E_STATE;
+  memcpy(dW, o->dW, sizeof(double) * MLP_NW);
+  return 0;
+}
 int or_num_threads(void) {
 #ifdef _OPENMP
   return omp_get_max_threads();

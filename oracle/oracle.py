@@ -41,6 +41,13 @@ _SIGS = [
     ("or_value_and_grad", ctypes.c_int, [_vp, _dp, _dp, _u64, ctypes.c_int32, _dp, _dp]),
     ("or_value_and_grad_stored", ctypes.c_int, [_vp, _dp, _dp, _u64, ctypes.c_int32, _dp, _dp]),
     ("or_list_radius", ctypes.c_double, [_vp]),
+    ("or_set_mlp", ctypes.c_int, [_vp, _P(ctypes.c_float), ctypes.c_int64]),
+    ("or_set_mlp_f64", ctypes.c_int, [_vp, _dp, ctypes.c_int64]),
+    ("or_mlp_size", ctypes.c_int, []),
+    ("or_mlp_eval", ctypes.c_int, [_vp, _dp, ctypes.c_int32, _dp, _dp, _dp]),
+    ("or_mlp_vjp", ctypes.c_int, [_vp, _dp, _dp, _dp, ctypes.c_int32, _dp]),
+    ("or_adjoint_terms_mlp", ctypes.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
+    ("or_mlp_grad", ctypes.c_int, [_vp, _dp]),
     ("or_num_threads", ctypes.c_int, []),
     ("or_expf", ctypes.c_float, [ctypes.c_float]),
     ("or_exp", ctypes.c_double, [ctypes.c_double]),
@@ -188,6 +195,41 @@ class Oracle:
 
     def list_radius(self):
         return self.L.or_list_radius(self.h)
+
+    # -- radial-basis MLP on O-O pairs --------------------------------------------------------
+    def set_mlp(self, w, exact=False):
+        """Weights in the library's packed layout; rounded to bfloat16 unless exact=True."""
+        if exact:
+            w = np.ascontiguousarray(w, dtype=np.float64)
+            self._chk(self.L.or_set_mlp_f64(self.h, _d(w), w.size))
+        else:
+            w = np.ascontiguousarray(w, dtype=np.float32)
+            self._chk(self.L.or_set_mlp(self.h, w.ctypes.data_as(_P(ctypes.c_float)), w.size))
+
+    def mlp_eval(self, r):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        u, u1, u2 = (np.zeros(r.size) for _ in range(3))
+        self._chk(self.L.or_mlp_eval(self.h, _d(r), r.size, _d(u), _d(u1), _d(u2)))
+        return u, u1, u2
+
+    def mlp_vjp(self, r, g1, g2=None):
+        r, g1 = (np.ascontiguousarray(a, dtype=np.float64) for a in (r, g1))
+        g2 = None if g2 is None else np.ascontiguousarray(g2, dtype=np.float64)
+        dW = np.zeros(self.L.or_mlp_size())
+        self._chk(self.L.or_mlp_vjp(self.h, _d(r), _d(g1), _d(g2), r.size, _d(dW)))
+        return dW
+
+    def adjoint_terms_mlp(self, x, c, w):
+        x, c, w = (np.ascontiguousarray(a, dtype=np.float64) for a in (x, c, w))
+        F, h, v = np.zeros(self.N3), np.zeros(self.N3), np.zeros(6)
+        dW = np.zeros(self.L.or_mlp_size())
+        self._chk(self.L.or_adjoint_terms_mlp(self.h, _d(x), _d(c), _d(w), _d(F), _d(h), _d(v), _d(dW)))
+        return F, h, v, dW
+
+    def mlp_grad(self):
+        dW = np.zeros(self.L.or_mlp_size())
+        self._chk(self.L.or_mlp_grad(self.h, _d(dW)))
+        return dW
 
 
 def philox4x64(ctr, key):

@@ -1,0 +1,220 @@
+"""The reference's Julia call surface, restated over librevsim.
+
+forward_sim / reverse_sim / train mirror water/water.jl:977-979 (forward_sim), :1435-1436
+(reverse_sim) and :1623-1698 (train): same argument order and meaning, same return tuples, same
+argument errors. Differences, all at the boundary:
+
+* `params` is the reference's LJ vector (σ, ε, q_O, k_b, r0, k_a, θ0) (water/water.jl:1572); the
+  library's θ is (σ, ε, q_H, k_b, r0, k_a) with q_H = -q_O/2 and θ0 a fixed configuration value, so
+  gradients come back as dL/dq_O = -dL/dq_H/2 and dL/dθ0 = 0.
+* `noise_seeds`: the reference draws one Xoshiro seed per step; the library's noise is a pure
+  function of (seed, step, atom), so one 64-bit seed replaces the vector (the first element is used
+  when a vector is passed).
+* `boundary` is a box-edge length or 3-vector (nm); `coords`/`velocities` are (N,3) or 3N arrays.
+* `n_threads` is accepted and ignored (one GPU per context).
+* `fwd_loggers` is the handle returned in the forward System: the library keeps the forward
+  snapshots and MSD frames on the device.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+from . import _abi
+from .simulator import Simulator
+from .system import DEFAULT_THETA, THETA0
+
+INTER_TYPES = ("lj",)
+LOSS_TYPES = ("rdf", "msd", "rdf_msd")
+
+
+@dataclasses.dataclass
+class System:
+    coords: np.ndarray
+    velocities: np.ndarray
+    boundary: np.ndarray
+    loggers: object = None
+
+
+def params_to_theta(params) -> np.ndarray:
+    """(σ, ε, q_O, k_b, r0, k_a[, θ0]) -> (σ, ε, q_H, k_b, r0, k_a)."""
+    p = np.asarray(params, dtype=np.float64)
+    if p.size not in (6, 7):
+        raise ValueError("params must be (σ, ε, q_O, k_b, r0, k_a, θ0)")
+    return np.array([p[0], p[1], -p[2] / 2.0, p[3], p[4], p[5]])
+
+
+def theta_grad_to_params(g) -> np.ndarray:
+    """dL/d(σ, ε, q_H, k_b, r0, k_a) -> dL/d(σ, ε, q_O, k_b, r0, k_a, θ0) with θ0 held fixed."""
+    g = np.asarray(g, dtype=np.float64)
+    return np.array([g[0], g[1], -g[2] / 2.0, g[3], g[4], g[5], 0.0])
+
+
+def _box(boundary, default=None):
+    if boundary is None:
+        if default is None:
+            raise ValueError("boundary must be given")
+        return default
+    b = np.asarray(boundary, dtype=np.float64).reshape(-1)
+    return np.repeat(b, 3) if b.size == 1 else b
+
+
+def _seed(noise_seeds) -> int:
+    if np.isscalar(noise_seeds):
+        return int(noise_seeds) & 0xFFFFFFFFFFFFFFFF
+    return int(np.asarray(noise_seeds).reshape(-1)[0]) & 0xFFFFFFFFFFFFFFFF
+
+
+def _check_types(inter_type, loss_type):
+    if str(inter_type).lstrip(":") not in INTER_TYPES:
+        raise ValueError("inter_type not recognised")  # water/water.jl:148
+    if str(loss_type).lstrip(":") not in LOSS_TYPES:
+        raise ValueError("loss_type not recognised")  # water/water.jl:766
+
+
+class Context:
+    """Owns the Simulator that forward_sim and reverse_sim share (the reference rebuilds its System
+    on each call; here the device context persists so the forward records stay resident)."""
+
+    def __init__(self, n_molecules: int, boundary, cutoff: float = 1.0, precision=_abi.RS_PRECISION_FP32,
+                 device: int = 0, **cfg):
+        self.box = _box(boundary)
+        self.base = dict(n_molecules=n_molecules, cutoff=cutoff, precision=precision, **cfg)
+        self.device = device
+        self.sim = None
+        self.key = None
+
+    def simulator(self, **overrides) -> Simulator:
+        kw = dict(self.base)
+        kw.update(overrides)
+        key = tuple(sorted(kw.items()))
+        if self.sim is None or key != self.key:
+            if self.sim is not None:
+                self.sim.close()
+            n, rc = kw.pop("n_molecules"), kw.pop("cutoff")
+            cfg = _abi.make_config(n, list(self.box), rc, **kw)
+            self.sim = Simulator(cfg, device=self.device)
+            self.key = key
+        return self.sim
+
+
+def _loss_cfg(loss_type, n_steps, n_steps_loss, n_steps_trunc=None, clip=math.inf, msd_window=500,
+              msd_every=50):
+    lt = str(loss_type).lstrip(":")
+    kw = dict(n_steps_loss=int(n_steps_loss) if n_steps_loss and n_steps_loss != n_steps else 0,
+              w_rdf=1.0 if lt in ("rdf", "rdf_msd") else 0.0,
+              clip_norm=float(clip))
+    if n_steps_trunc is not None:
+        kw["n_steps_trunc"] = 0 if n_steps_trunc == (n_steps_loss or n_steps) else int(n_steps_trunc)
+    if lt in ("msd", "rdf_msd"):
+        w = min(msd_window, n_steps)
+        w -= w % (2 * msd_every)
+        if w < 2 * msd_every:
+            raise ValueError("trajectory too short for the MSD loss window")
+        kw.update(msd_window_steps=w, msd_every=msd_every, w_msd=1.0)
+    return kw
+
+
+def forward_sim(coords, velocities, boundary, noise_seeds, gamma, n_steps, n_steps_loss, params, dt,
+                n_threads=None, inter_type="lj", loss_type="rdf", run_grads=False, calc_loss=True,
+                use_barostat=False, *, context: Context, target_rdf=None):
+    """water/water.jl:977-1105 with run_grads=false: returns (System, loss, grads=None)."""
+    _check_types(inter_type, loss_type)
+    if run_grads:
+        raise ValueError("run_grads (the direct-differentiation gradient) is not part of this library")
+    if use_barostat:
+        raise ValueError("the Monte Carlo barostat is not part of this library (NVT only)")
+    if n_steps_loss and n_steps % n_steps_loss:
+        raise ValueError("n_steps_loss must divide n_steps")
+    kw = _loss_cfg(loss_type, n_steps, n_steps_loss)
+    if not calc_loss:
+        kw.update(w_rdf=0.0, w_msd=0.0, msd_window_steps=0, msd_every=0)
+    sim = context.simulator(gamma=float(gamma), dt=float(dt), **kw)
+    sim.set_theta(params_to_theta(params))
+    if target_rdf is not None:
+        sim.set_target_rdf(target_rdf)
+    x, v, loss = sim.forward(np.asarray(coords, dtype=np.float64).reshape(-1),
+                             np.asarray(velocities, dtype=np.float64).reshape(-1), _seed(noise_seeds), n_steps)
+    sys = System(x.reshape(-1, 3), v.reshape(-1, 3), context.box.copy(), loggers=(sim, _seed(noise_seeds), n_steps))
+    return sys, loss, None
+
+
+def reverse_sim(coords, velocities, boundary, noise_seeds, gamma, n_steps, n_steps_loss, n_steps_trunc, params,
+                dt, n_threads, fwd_loggers, clip_norm_threshold, inter_type="lj", loss_type="rdf"):
+    """water/water.jl:1435-1558: returns (System, loss, dl_dp, l_rdf, l_msd) with dl_dp in the
+    reference's 7-parameter layout."""
+    _check_types(inter_type, loss_type)
+    n_loss = n_steps_loss or n_steps
+    if n_steps_trunc > n_loss:
+        raise ValueError("n_steps_trunc must not be greater than n_steps_loss")  # water/water.jl:1437
+    if fwd_loggers is None:
+        raise ValueError("fwd_loggers from forward_sim are required")
+    sim, seed, fwd_steps = fwd_loggers
+    if fwd_steps != n_steps or seed != _seed(noise_seeds):
+        raise ValueError("fwd_loggers belong to a different trajectory")
+    sim.set_theta(params_to_theta(params))
+    # the forward call fixed the loss configuration; truncation and clipping apply to the reverse
+    if (sim.cfg.n_steps_trunc != (0 if n_steps_trunc == n_loss else n_steps_trunc)
+            or sim.cfg.clip_norm != float(clip_norm_threshold)):
+        raise ValueError("n_steps_trunc and clip_norm_threshold must match the context's configuration")
+    loss, g, x0, v0 = sim.reverse(np.asarray(coords, dtype=np.float64).reshape(-1),
+                                  np.asarray(velocities, dtype=np.float64).reshape(-1), seed, n_steps)
+    sys = System(x0.reshape(-1, 3), v0.reshape(-1, 3), _box(boundary, sim.cfg.box[:]))
+    return sys, loss, theta_grad_to_params(g), None, None
+
+
+class Adam:
+    """Adam (Kingma & Ba 2015) with Optimisers.jl's defaults (β = (0.9, 0.999), ε = 1e-8)."""
+
+    def __init__(self, lr: float, n: int, beta1=0.9, beta2=0.999, eps=1e-8):
+        self.lr, self.b1, self.b2, self.eps = lr, beta1, beta2, eps
+        self.m = np.zeros(n)
+        self.v = np.zeros(n)
+        self.t = 0
+
+    def step(self, p: np.ndarray, g: np.ndarray) -> np.ndarray:
+        self.t += 1
+        self.m = self.b1 * self.m + (1 - self.b1) * g
+        self.v = self.b2 * self.v + (1 - self.b2) * g * g
+        mh = self.m / (1 - self.b1 ** self.t)
+        vh = self.v / (1 - self.b2 ** self.t)
+        return p - self.lr * mh / (np.sqrt(vh) + self.eps)
+
+
+def train(sim: Simulator, x0, v0, params_start, n_epochs: int, n_steps: int, n_steps_equil: int = 0,
+          learning_rate: float = 2e-3, grad_clamp_val: float = 1e3, scale_params: bool = True,
+          seed: int = 1, log=None):
+    """The epoch loop of water/water.jl:1623-1698 on one context: fresh noise seed per epoch,
+    optional equilibration, forward + reverse, a log line per epoch, Adam on parameters scaled by
+    their starting values with the gradient clamped to ±grad_clamp_val (water/water.jl:1682-1696).
+    Returns the parameter trajectory and the per-epoch (loss, grads)."""
+    params = np.asarray(params_start, dtype=np.float64).copy()
+    p0 = params.copy()
+    opt = Adam(learning_rate, params.size)
+    scaled = np.ones_like(params)
+    rng = np.random.Generator(np.random.Philox(seed))
+    history = []
+    x, v = np.asarray(x0, dtype=np.float64).reshape(-1), np.asarray(v0, dtype=np.float64).reshape(-1)
+    for epoch in range(1, n_epochs + 1):
+        s_eq, s_run = (int(k) for k in rng.integers(0, 2 ** 63, size=2))
+        sim.set_theta(params_to_theta(params))
+        xs, vs = x, v
+        if n_steps_equil:
+            xs, vs, _ = sim.forward(x, v, s_eq, n_steps_equil)
+        loss, g = sim.value_and_grad(xs, vs, s_run, n_steps)
+        grads = theta_grad_to_params(g)[: params.size]
+        history.append((loss, grads.copy()))
+        if log is not None:
+            log(f"{epoch} {loss} " + " ".join(map(str, params)) + " " + " ".join(map(str, grads)))
+        if scale_params:
+            scaled = opt.step(scaled, np.clip(grads * p0, -grad_clamp_val, grad_clamp_val))
+            params = scaled * p0
+        else:
+            params = opt.step(params, np.clip(grads, -grad_clamp_val, grad_clamp_val))
+    return params, history
+
+
+__all__ = ["System", "Context", "forward_sim", "reverse_sim", "train", "Adam", "params_to_theta",
+           "theta_grad_to_params", "DEFAULT_THETA", "THETA0"]

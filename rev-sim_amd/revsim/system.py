@@ -109,6 +109,36 @@ WORKLOADS = {
 }
 
 
+MLP_NF, MLP_NH = 16, 64
+MLP_SIZES = (("W1", (MLP_NH, MLP_NF)), ("b1", (MLP_NH,)), ("W2", (MLP_NH, MLP_NH)), ("b2", (MLP_NH,)),
+             ("w3", (MLP_NH,)), ("b3", (1,)))
+MLP_NW = sum(int(np.prod(s)) for _, s in MLP_SIZES)
+
+
+def mlp_init(seed: int = SEED_MLP) -> np.ndarray:
+    """Xavier-uniform weights (zero biases) of the 16 -> 64 -> 64 -> 1 SiLU network, packed as
+    W1 (row-major 64x16), b1, W2 (64x64), b2, w3, b3 — the layout rs_set_mlp takes."""
+    rng = np.random.Generator(np.random.Philox(seed))
+    parts = []
+    for name, shape in MLP_SIZES:
+        if name.startswith("b"):
+            parts.append(np.zeros(shape))
+        else:
+            fan_out, fan_in = (shape if len(shape) == 2 else (1, shape[0]))
+            a = np.sqrt(6.0 / (fan_in + fan_out))
+            parts.append(rng.uniform(-a, a, size=shape))
+    return np.ascontiguousarray(np.concatenate([p.reshape(-1) for p in parts]), dtype=np.float32)
+
+
+def mlp_unpack(w: np.ndarray) -> dict:
+    out, o = {}, 0
+    for name, shape in MLP_SIZES:
+        n = int(np.prod(shape))
+        out[name] = np.asarray(w[o:o + n]).reshape(shape)
+        o += n
+    return out
+
+
 def perturbed_target(g: np.ndarray, seed: int = SEED_TARGET, rel: float = 0.05) -> np.ndarray:
     """Synthetic target RDF: g * (1 + rel * N(0,1)) per bin with a seeded Philox stream."""
     rng = np.random.Generator(np.random.Philox(seed))
