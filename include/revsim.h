@@ -126,6 +126,21 @@ int rs_reverse(rs_ctx* ctx, const double* xN, const double* vN, uint64_t seed, i
 int rs_value_and_grad(rs_ctx* ctx, const double* x0, const double* v0, uint64_t seed,
                       int32_t n_steps, double* loss_out, double* dL_dtheta6, float* dL_dmlp);
 
+/* --- radial-basis MLP correction on O-O pairs (rs_config.use_mlp = 1) ---------------------
+   u(r) = w3.SiLU(W2 SiLU(W1 φ(r) + b1) + b2) + b3 per O-O pair within the cutoff, 16 Gaussian
+   features (centres 0.2-1.0 nm, width 0.05 nm) with a cosine cutoff. No reference counterpart:
+   the reference's learned-potential variants evaluate a network through PythonCall
+   (water/water_maceoff.jl:147-163 HVP/VJP by double backward); this is the MI355X version for a
+   pair network. Weights are packed W1[64][16], b1[64], W2[64][64], b2[64], w3[64], b3 (5313
+   floats) and rounded to bfloat16 on entry. */
+#define RS_MLP_NWEIGHTS 5313
+int rs_set_mlp(rs_ctx* ctx, const float* weights, int64_t n);
+/* dL/dW of the last rs_reverse / rs_value_and_grad / rs_adjoint_terms call (n = 5313). */
+int rs_mlp_grad(rs_ctx* ctx, double* dL_dW, int64_t n);
+/* The tabulated pair force f(r_k) = -u'(r_k) and f'(r_k) = -u''(r_k) on the library's n_nodes
+   nodes r_k = r_lo + k (r_c - r_lo)/(n_nodes - 1); returns n_nodes and r_lo. */
+int rs_mlp_table(rs_ctx* ctx, double* f, double* df, int32_t capacity, int32_t* n_nodes, double* r_lo);
+
 /* --- single-evaluation entry points (parity and unit tests) ------------------------------ */
 /* Forces F(x) (3N doubles). */
 int rs_forces(rs_ctx* ctx, const double* x, double* F);

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "internal.h"
+#include "mlp.h"
 
 namespace {
 
@@ -53,6 +54,7 @@ struct rs_ctx {
   int ext_steps = 0;
   int since_build = 0;
   double hext_extra = 0.0;  // added to the O-H extent after an extent violation
+  bool mlp_set = false;     // weights uploaded and tabulated
   std::string err;
   // timing: events recorded around launches on the context's stream
   bool timing = false;
@@ -183,6 +185,13 @@ int alloc_all(rs_ctx* c) {
       (r = dalloc(c, &b.rdf_g, nb)))
     return r;
   if ((r = dalloc(c, &b.msd_acc, 64))) return r;
+  if (c->cfg.use_mlp) {
+    if ((r = dalloc(c, &b.mlp_w, MLP_NW)) || (r = dalloc(c, &b.mlp_wb, MLP_NW)) ||
+        (r = dalloc(c, (double2**)&b.mlp_tab, MLP_NT)) ||
+        (r = dalloc(c, &b.mlp_slot, (size_t)pair_blocks(p) * MLP_NG)) || (r = dalloc(c, &b.mlp_G, MLP_NG)) ||
+        (r = dalloc(c, &b.mlp_part, (size_t)(MLP_NT / 64) * MLP_NW)) || (r = dalloc(c, &b.mlp_dW, MLP_NW)))
+      return r;
+  }
   return alloc_list(c);
 }
 
@@ -286,6 +295,7 @@ int finish(rs_ctx* c, int* retry) {
   int f[FLAG_COUNT];
   HIPCHK(c, hipMemcpy(f, c->L.b.flags, sizeof(f), hipMemcpyDeviceToHost));
   if (f[FLAG_NONFINITE]) return fail(c, RS_E_NONFINITE, "non-finite positions, velocities or forces");
+  if (f[FLAG_MLP_RANGE]) return fail(c, RS_E_NONFINITE, "an O-O pair came closer than the MLP table's range");
   *retry = (f[FLAG_LIST_OVERFLOW] ? 1 : 0) | (f[FLAG_STALE] ? 2 : 0) | (f[FLAG_HEXTENT] ? 4 : 0);
   return 0;
 }
@@ -314,6 +324,7 @@ int forward_impl(rs_ctx* c, const double* x0, const double* v0, uint64_t seed, i
   Plan P;
   int r = make_plan(c, n_steps, &P);
   if (r) return r;
+  if (c->cfg.use_mlp && !c->mlp_set) return fail(c, RS_E_STATE, "use_mlp is set but rs_set_mlp was not called");
   if (c->ext_steps && c->ext_steps < n_steps)
     return fail(c, RS_E_CONFIG, "external noise holds fewer steps than requested");
   DevBufs& b = c->L.b;
@@ -429,7 +440,10 @@ int reverse_impl(rs_ctx* c, const double* xN, const double* vN, uint64_t seed, i
   const double ws[2] = {dt * dt / (2.0 * g.mass_O), dt * dt / (2.0 * g.mass_H)};
   const bool clip = g.clip_norm > 0.0 && std::isfinite(g.clip_norm);
   const int nslots = pair_blocks(c->L.p);
+  if (c->cfg.use_mlp && !c->mlp_set) return fail(c, RS_E_STATE, "use_mlp is set but rs_set_mlp was not called");
   for (int attempt = 0;; ++attempt) {
+    if (c->L.use_mlp)
+      HIPCHK(c, hipMemsetAsync(b.mlp_slot, 0, sizeof(double) * (size_t)pair_blocks(c->L.p) * MLP_NG, c->s));
     if (xN) HIPCHK(c, hipMemcpyAsync(b.x, xN, sizeof(double) * c->N3, hipMemcpyHostToDevice, c->s));
     if (vN) HIPCHK(c, hipMemcpyAsync(b.v, vN, sizeof(double) * c->N3, hipMemcpyHostToDevice, c->s));
     if ((r = reset_flags(c))) return r;
@@ -448,6 +462,7 @@ int reverse_impl(rs_ctx* c, const double* xN, const double* vN, uint64_t seed, i
         pending = false;
         if (++c->since_build >= g.nl_every) rebuild(c, true);
         tbegin(c, RS_PHASE_REV_PAIR);
+        c->L.parity = n & 1;
         launch_pair(c->L, true, b.A, ws[0], ws[1]);
         launch_vjp_reduce(c->L);
         tend(c);
@@ -582,7 +597,9 @@ int rs_create(const rs_config* cfg, int32_t device, rs_ctx** out) {
   if (g.rdf_bins < 1 || g.rdf_bins > 1024 || !(g.rdf_rmax > 0.0) || !(g.rdf_width > 0.0) ||
       !(g.rdf_window > 0.0))
     return fail(nullptr, RS_E_CONFIG, "rdf_bins in [1,1024] and positive rdf_rmax/width/window required");
-  if (g.use_mlp) return fail(nullptr, RS_E_CONFIG, "use_mlp is not available in this build");
+  if (g.use_mlp != 0 && g.use_mlp != 1) return fail(nullptr, RS_E_CONFIG, "use_mlp must be 0 or 1");
+  if (g.use_mlp && (g.cutoff <= MLP_RLO + 0.1))
+    return fail(nullptr, RS_E_CONFIG, "the MLP term needs a cutoff above 0.25 nm");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0)
     return fail(nullptr, RS_E_NODEVICE, "no HIP device with that index");
@@ -600,6 +617,8 @@ int rs_create(const rs_config* cfg, int32_t device, rs_ctx** out) {
   }
   c->L.s = c->s;
   c->L.precision = g.precision;
+  c->L.use_mlp = 0;
+  c->L.mlp_rc = g.cutoff;
   c->L.p.CH = g.pair_chunks > 0 ? g.pair_chunks : 16;
   if (c->L.p.CH != 8 && c->L.p.CH != 16) {
     rs_destroy(c);
@@ -631,7 +650,7 @@ int rs_destroy(rs_ctx* c) {
                   b.cell_start, b.cell_fill, b.tmp, b.order, b.rank, b.posq, b.pos64, b.uf,
                   b.ud, b.O64, b.nbr, b.ncnt, b.flags, b.slots, b.vjb, b.dth, b.scal, b.hist, b.rdfG,
                   b.target, b.rdf_g, b.frames, b.msd_del, b.msd_acc, b.msd_grad, b.ext_noise,
-                  b.hpart};
+                  b.hpart, b.mlp_w, b.mlp_wb, b.mlp_tab, b.mlp_slot, b.mlp_G, b.mlp_part, b.mlp_dW};
   for (void* q : ptrs)
     if (q) hipFree(q);
   for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
@@ -705,14 +724,91 @@ int rs_value_and_grad(rs_ctx* c, const double* x0, const double* v0, uint64_t se
   if (!x0 || !v0) return fail(c, RS_E_CONFIG, "null x0/v0");
   std::vector<double> xN(c->N3), vN(c->N3);
   if ((r = forward_impl(c, x0, v0, seed, n_steps, xN.data(), vN.data(), nullptr))) return r;
-  (void)dL_dmlp;
-  return reverse_impl(c, xN.data(), vN.data(), seed, n_steps, loss_out, dth, nullptr, nullptr);
+  if ((r = reverse_impl(c, xN.data(), vN.data(), seed, n_steps, loss_out, dth, nullptr, nullptr))) return r;
+  if (dL_dmlp) {
+    if (!c->L.use_mlp) return fail(c, RS_E_STATE, "dL_dmlp requested without the MLP term");
+    std::vector<double> d(MLP_NW);
+    if ((r = rs_mlp_grad(c, d.data(), MLP_NW))) return r;
+    for (int k = 0; k < MLP_NW; ++k) dL_dmlp[k] = (float)d[k];
+  }
+  return 0;
+}
+
+int rs_set_mlp(rs_ctx* c, const float* w, int64_t n) {
+  int r = check_ctx(c);
+  if (r) return r;
+  if (!c->cfg.use_mlp) return fail(c, RS_E_CONFIG, "the context was created with use_mlp = 0");
+  if (!w || n != MLP_NW) return fail(c, RS_E_CONFIG, "weights must hold 5313 values");
+  std::vector<double> wd(MLP_NW);
+  std::vector<unsigned short> wb(MLP_NW);
+  for (int k = 0; k < MLP_NW; ++k) {
+    if (!std::isfinite(w[k])) return fail(c, RS_E_CONFIG, "non-finite MLP weight");
+    uint32_t u;
+    std::memcpy(&u, &w[k], 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest bfloat16, ties to even
+    u &= 0xFFFF0000u;
+    float f;
+    std::memcpy(&f, &u, 4);
+    wd[k] = (double)f;
+    wb[k] = (unsigned short)(u >> 16);
+  }
+  HIPCHK(c, hipMemcpy(c->L.b.mlp_w, wd.data(), sizeof(double) * MLP_NW, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->L.b.mlp_wb, wb.data(), sizeof(unsigned short) * MLP_NW, hipMemcpyHostToDevice));
+  tbegin(c, RS_PHASE_MLP);
+  launch_mlp_tabulate(c->L, c->cfg.cutoff);
+  tend(c);
+  HIPCHK(c, hipStreamSynchronize(c->s));
+  collect_timing(c);
+  c->L.use_mlp = 1;
+  c->mlp_set = true;
+  c->rec_valid = false;
+  return 0;
+}
+
+int rs_mlp_grad(rs_ctx* c, double* dW, int64_t n) {
+  int r = check_ctx(c);
+  if (r) return r;
+  if (!c->L.use_mlp) return fail(c, RS_E_STATE, "no MLP weights set");
+  if (!dW || n != MLP_NW) return fail(c, RS_E_CONFIG, "dL_dW must hold 5313 values");
+  tbegin(c, RS_PHASE_MLP);
+  launch_mlp_grad(c->L, c->cfg.cutoff);
+  tend(c);
+  HIPCHK(c, hipStreamSynchronize(c->s));
+  collect_timing(c);
+  HIPCHK(c, hipMemcpy(dW, c->L.b.mlp_dW, sizeof(double) * MLP_NW, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int rs_mlp_table(rs_ctx* c, double* f, double* df, int32_t capacity, int32_t* n_nodes, double* r_lo) {
+  int r = check_ctx(c);
+  if (r) return r;
+  if (!c->L.use_mlp) return fail(c, RS_E_STATE, "no MLP weights set");
+  if (n_nodes) *n_nodes = MLP_NT;
+  if (r_lo) *r_lo = MLP_RLO;
+  if (capacity < MLP_NT) return capacity == 0 ? 0 : fail(c, RS_E_CONFIG, "capacity below the node count");
+  if (c->L.precision == RS_PRECISION_FP32) {
+    std::vector<float2> t(MLP_NT);
+    HIPCHK(c, hipMemcpy(t.data(), c->L.b.mlp_tab, sizeof(float2) * MLP_NT, hipMemcpyDeviceToHost));
+    for (int k = 0; k < MLP_NT; ++k) {
+      if (f) f[k] = t[k].x;
+      if (df) df[k] = t[k].y;
+    }
+  } else {
+    std::vector<double2> t(MLP_NT);
+    HIPCHK(c, hipMemcpy(t.data(), c->L.b.mlp_tab, sizeof(double2) * MLP_NT, hipMemcpyDeviceToHost));
+    for (int k = 0; k < MLP_NT; ++k) {
+      if (f) f[k] = t[k].x;
+      if (df) df[k] = t[k].y;
+    }
+  }
+  return 0;
 }
 
 int rs_forces(rs_ctx* c, const double* x, double* F) {
   int r = check_ctx(c);
   if (r) return r;
   if (!x || !F) return fail(c, RS_E_CONFIG, "null argument");
+  if (c->cfg.use_mlp && !c->mlp_set) return fail(c, RS_E_STATE, "use_mlp is set but rs_set_mlp was not called");
   if ((r = single_eval(c, x, false, [&] {
          launch_bonded(c->L, false, nullptr, 0, 0, nullptr, 0, 0);
          launch_pair(c->L, false, nullptr, 0, 0);
@@ -731,7 +827,12 @@ int rs_adjoint_terms(rs_ctx* c, const double* x, const double* cc, const double*
   HIPCHK(c, hipMemcpy(b.C, cc, sizeof(double) * c->N3, hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(b.A, w, sizeof(double) * c->N3, hipMemcpyHostToDevice));
   const double cs[2] = {1.0 / c->cfg.mass_O, 1.0 / c->cfg.mass_H};
+  if (c->cfg.use_mlp && !c->mlp_set) return fail(c, RS_E_STATE, "use_mlp is set but rs_set_mlp was not called");
   if ((r = single_eval(c, x, true, [&] {
+         c->L.parity = 0;
+         hipMemsetAsync(b.scal + 6, 0, sizeof(double) * 2, c->s);
+         if (c->L.use_mlp)
+           hipMemsetAsync(b.mlp_slot, 0, sizeof(double) * (size_t)pair_blocks(c->L.p) * MLP_NG, c->s);
          launch_bonded(c->L, true, b.C, cs[0], cs[1], b.A, 1.0, 1.0);
          launch_pair(c->L, true, b.A, 1.0, 1.0);
        })))

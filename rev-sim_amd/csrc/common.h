@@ -15,6 +15,7 @@ enum {
   FLAG_STALE = 1,          // an O moved more than skin/2 since the last list build
   FLAG_NONFINITE = 2,      // NaN/Inf in positions, velocities or forces
   FLAG_HEXTENT = 3,        // an O-H distance exceeded the extent assumed by the list radius
+  FLAG_MLP_RANGE = 4,      // an O-O pair closer than the MLP table's lower end
   FLAG_COUNT = 8
 };
 

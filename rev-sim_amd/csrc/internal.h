@@ -31,6 +31,12 @@ struct DevBufs {
   double* msd_grad;   // dL/dframes [nf * 3M]
   double* ext_noise;  // [steps * 3N] or null
   double* hpart;      // h-norm partials
+  // radial-basis MLP (mlp.h)
+  double* mlp_w;            // weights (BF16 values) as FP64
+  unsigned short* mlp_wb;   // weights as BF16 bits
+  void* mlp_tab;            // (f, f') per node: float2 (FP32 path) or double2
+  double* mlp_slot;         // per pair-block table gradient [blocks * MLP_NG]
+  double *mlp_G, *mlp_part, *mlp_dW;
 };
 
 struct Launch {
@@ -38,6 +44,9 @@ struct Launch {
   DevBufs b;
   int precision;
   hipStream_t s;
+  int use_mlp;     // MLP pair term on (weights set and tabulated)
+  double mlp_rc;   // cutoff of the MLP features (= pair cutoff)
+  int parity;      // step parity selecting the max|w_O| slot of the adjoint MLP accumulation
 };
 
 // list
@@ -64,3 +73,5 @@ void launch_msd(const Launch& L, int nf, double lag, double w_msd, double target
 void launch_msd_inject(const Launch& L, int f);
 void launch_zero(const Launch& L, double* a, size_t n);
 int pair_blocks(const SysParams& p);
+void launch_mlp_tabulate(const Launch& L, double rc);
+void launch_mlp_grad(const Launch& L, double rc);

@@ -22,6 +22,7 @@
 //          with (f' - f/r)/r^2 = -3 k q_a q_b / r^5   [- 24ε/r^4 (28 s^12 - 8 s^6) for O-O]
 //   q-VJP: dF/dq_H = 2 F_RF / q_H; ε-VJP: dF/dε = F_LJ/ε; σ-VJP: dF_a/dσ = -(144ε/(σ r^2))(4s^12-s^6) d
 #include "internal.h"
+#include "mlp.h"
 
 namespace {
 
@@ -39,7 +40,23 @@ struct PairArgs {
   double* h;
   double* slots;
   int* flags;
+  // MLP pair term on O-O pairs (mlp.h)
+  int mlp;
+  const void* __restrict__ mtab;
+  double mrlo, mh, minvh;
+  double* mslot;                    // per-block table gradient (adjoint)
+  const double* __restrict__ wmax;  // max|w_O| of this step (uint64 bits of a double)
 };
+
+// Fixed-point scale of the per-block table-gradient sums: 2^(50 - e) with max|w_O| < 2^e, so one
+// block's sum (≤ 2^11 pair contributions bounded by max|w_O|) stays below 2^61.
+__device__ __forceinline__ double mlp_scale(const double* wmax) {
+  const double m = *wmax;
+  if (!(m > 0.0)) return 1125899906842624.0;  // 2^50
+  int e;
+  frexp(m, &e);
+  return ldexp(1.0, 50 - e);
+}
 
 struct P32 {
   typedef int4 Pos;
@@ -87,7 +104,7 @@ __global__ __launch_bounds__(RS_BLOCK) void k_pair(PairArgs A) {
 
   Pos own[3];
   R uo[3][3];
-  R F[3][3], hh[3][3], Flj[3], Fs[3];
+  R F[3][3], hh[3][3], Flj[3], Fs[3], Fml[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     own[a] = act ? pos[3 * i + a] : pos[0];
@@ -98,6 +115,7 @@ __global__ __launch_bounds__(RS_BLOCK) void k_pair(PairArgs A) {
     }
     Flj[a] = 0;
     Fs[a] = 0;
+    Fml[a] = 0;
     if (ADJ) {
       U t = act ? uu[3 * i + a] : uu[0];
       uo[a][0] = t.x;
@@ -110,6 +128,23 @@ __global__ __launch_bounds__(RS_BLOCK) void k_pair(PairArgs A) {
   const R kq[3] = {(R)p.kqq[0], (R)p.kqq[1], (R)p.kqq[2]};
   const R sig2 = (R)(p.sig * p.sig), eps24 = (R)(24.0 * p.eps);
   const R epss = (R)(144.0 * p.eps / p.sig);
+
+  // MLP: own O's w (adjoint), table-gradient accumulator in LDS
+  __shared__ long long lg[ADJ ? MLP_NG : 1];
+  R wO[3] = {0, 0, 0};
+  double gscale = 0.0;
+  if (ADJ && A.mlp) {
+    for (int q = threadIdx.x; q < MLP_NG; q += RS_BLOCK) lg[q] = 0;
+    gscale = mlp_scale(A.wmax);
+    if (act) {
+      const size_t mo = 9 * (size_t)A.order[i];
+      wO[0] = (R)(A.wsrc[mo] * A.wscale[0]);
+      wO[1] = (R)(A.wsrc[mo + 1] * A.wscale[0]);
+      wO[2] = (R)(A.wsrc[mo + 2] * A.wscale[0]);
+    }
+    __syncthreads();
+  }
+  const R mrlo = (R)A.mrlo, mh = (R)A.mh, minvh = (R)A.minvh;
 
   const int n = act ? A.ncnt[gtid] : 0;
   const int* __restrict__ L = A.nbr + (size_t)(gtid / RS_WAVE) * p.capc * RS_WAVE + (gtid & (RS_WAVE - 1));
@@ -168,13 +203,59 @@ __global__ __launch_bounds__(RS_BLOCK) void k_pair(PairArgs A) {
               Fs[t] -= fs * d[t];
             }
           }
+          if (A.mlp) {
+            // f(r) = -u'(r) by cubic Hermite between table nodes k, k+1
+            const R rr = r2 * ri;
+            R tk = (rr - mrlo) * minvh;
+            if (in && tk < (R)0) atomicOr(&A.flags[FLAG_MLP_RANGE], 1);
+            tk = tk < (R)0 ? (R)0 : tk;
+            int kk = (int)tk;
+            kk = kk > MLP_NT - 2 ? MLP_NT - 2 : kk;
+            R tt = tk - (R)kk;
+            tt = tt > (R)1 ? (R)1 : tt;
+            R fk0, dk0, fk1, dk1;
+            if (sizeof(R) == 4) {
+              const float2* tb = (const float2*)A.mtab;
+              const float2 e0 = tb[kk], e1 = tb[kk + 1];
+              fk0 = (R)e0.x; dk0 = (R)e0.y; fk1 = (R)e1.x; dk1 = (R)e1.y;
+            } else {
+              const double2* tb = (const double2*)A.mtab;
+              const double2 e0 = tb[kk], e1 = tb[kk + 1];
+              fk0 = (R)e0.x; dk0 = (R)e0.y; fk1 = (R)e1.x; dk1 = (R)e1.y;
+            }
+            R hb[4], hdb[4];
+            hermite<R>(tt, hb, hdb);
+            const R f = hb[0] * fk0 + hb[1] * mh * dk0 + hb[2] * fk1 + hb[3] * mh * dk1;
+            const R fp = (hdb[0] * fk0 + hdb[1] * mh * dk0 + hdb[2] * fk1 + hdb[3] * mh * dk1) * minvh;
+            const R frm = in ? f * ri : (R)0;
+#pragma unroll
+            for (int t = 0; t < 3; ++t) F[0][t] -= frm * d[t];
+            if (ADJ) {
+              const R g2m = in ? (fp - f * ri) * ri2 : (R)0;
+#pragma unroll
+              for (int t = 0; t < 3; ++t) {
+                hh[0][t] += frm * du[t] + (g2m * dot) * d[t];
+                Fml[t] -= frm * d[t];
+              }
+              // weight VJP: w_O . dF_O/d(table) = -(w_O . r̂) ∂f/∂(table)
+              if (in) {
+                const double q = -(double)((wO[0] * d[0] + wO[1] * d[1] + wO[2] * d[2]) * ri) * gscale;
+                atomicAdd((unsigned long long*)&lg[2 * kk], (unsigned long long)__double2ll_rn(q * (double)hb[0]));
+                atomicAdd((unsigned long long*)&lg[2 * kk + 1],
+                          (unsigned long long)__double2ll_rn(q * (double)(mh * hb[1])));
+                atomicAdd((unsigned long long*)&lg[2 * kk + 2], (unsigned long long)__double2ll_rn(q * (double)hb[2]));
+                atomicAdd((unsigned long long*)&lg[2 * kk + 3],
+                          (unsigned long long)__double2ll_rn(q * (double)(mh * hb[3])));
+              }
+            }
+          }
         }
       }
     }
   }
 
   // combine the CH lanes of the molecule (fixed xor tree, FP64)
-  double Fd[9], hd[9], Fljd[3], Fsd[3];
+  double Fd[9], hd[9], Fljd[3], Fsd[3], Fmld[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
 #pragma unroll
@@ -185,6 +266,7 @@ __global__ __launch_bounds__(RS_BLOCK) void k_pair(PairArgs A) {
     if (ADJ) {
       Fljd[a] = chunk_sum<CH>((double)Flj[a]);
       Fsd[a] = chunk_sum<CH>((double)Fs[a]);
+      Fmld[a] = A.mlp ? chunk_sum<CH>((double)Fml[a]) : 0.0;
     }
   }
 
@@ -198,10 +280,11 @@ __global__ __launch_bounds__(RS_BLOCK) void k_pair(PairArgs A) {
         double wm[9];
 #pragma unroll
         for (int t = 0; t < 9; ++t) wm[t] = A.wsrc[9 * (size_t)m + t] * A.wscale[t < 3 ? 0 : 1];
-        // pair-part VJPs on this molecule's atoms: F_RF = F_pair - F_LJ (LJ acts on O only)
+        // pair-part VJPs on this molecule's atoms: F_RF = F_pair - F_LJ - F_MLP (LJ and the MLP
+        // act on O only)
         double sq = 0.0, se = 0.0, ss = 0.0;
 #pragma unroll
-        for (int t = 0; t < 9; ++t) sq += wm[t] * (Fd[t] - (t < 3 ? Fljd[t] : 0.0));
+        for (int t = 0; t < 9; ++t) sq += wm[t] * (Fd[t] - (t < 3 ? Fljd[t] + Fmld[t] : 0.0));
 #pragma unroll
         for (int t = 0; t < 3; ++t) {
           se += wm[t] * Fljd[t];
@@ -239,6 +322,13 @@ __global__ __launch_bounds__(RS_BLOCK) void k_pair(PairArgs A) {
       for (int q = 0; q < RS_BLOCK / CH; ++q) s += vjs[q][threadIdx.x];
       A.slots[(size_t)blockIdx.x * 8 + threadIdx.x] = s;
     }
+    if (A.mlp) {
+      // this block's table gradient of the step, accumulated over the trajectory in its own slot
+      const double inv = 1.0 / gscale;
+      double* sl = A.mslot + (size_t)blockIdx.x * MLP_NG;
+      for (int q = threadIdx.x; q < MLP_NG; q += RS_BLOCK)
+        if (lg[q]) sl[q] += (double)lg[q] * inv;
+    }
   }
 }
 
@@ -272,6 +362,13 @@ void dispatch(const Launch& L, const PairArgs& a) {
 
 void launch_pair(const Launch& L, bool adjoint, const double* wsrc, double wscale_O, double wscale_H) {
   PairArgs a;
+  a.mlp = L.use_mlp;
+  a.mtab = L.b.mlp_tab;
+  a.mrlo = MLP_RLO;
+  a.mh = (L.mlp_rc - MLP_RLO) / (MLP_NT - 1);
+  a.minvh = 1.0 / a.mh;
+  a.mslot = L.b.mlp_slot;
+  a.wmax = L.b.scal + 6 + (L.parity & 1);
   a.p = L.p;
   a.nbr = L.b.nbr;
   a.ncnt = L.b.ncnt;

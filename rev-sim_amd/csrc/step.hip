@@ -16,6 +16,15 @@ namespace {
 // Per-molecule kernels run one wave per block so that the few waves there are spread over all CUs.
 constexpr int MOL_BLOCK = 64;
 
+// max over the block's molecules of |w| on O atoms, folded into *slot (a non-negative double kept
+// as its uint64 bit pattern, which orders like the value): the per-step fixed-point scale of the
+// MLP table gradient.
+__device__ __forceinline__ void wmax_fold(double v, double* slot) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  if ((threadIdx.x & 63) == 0 && v > 0.0)
+    atomicMax((unsigned long long*)slot, (unsigned long long)__double_as_longlong(v));
+}
+
 __device__ __forceinline__ void write_sorted(const SysParams& p, int precision, int slot,
                                              const double x[3], int4* posq, double4* pos64) {
   if (precision == RS_PRECISION_FP32)
@@ -93,11 +102,15 @@ __global__ __launch_bounds__(MOL_BLOCK) void k_step_fwd(SysParams p, int precisi
 __global__ __launch_bounds__(MOL_BLOCK) void k_step_rev(SysParams p, int precision, DevBufs b,
                                                         int do_r2, int do_r1, int step,
                                                         uint64_t seed, double dt, double e,
-                                                        double s, double kT, int nslots, int clip) {
+                                                        double s, double kT, int nslots, int clip,
+                                                        int mlp) {
 #pragma clang fp contract(off)
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   const double dt2 = dt * dt;
-  if (m >= p.M) return;
+  // MLP scale slots alternate by step parity: this step's pair kernel reads slot (step & 1)
+  if (mlp && do_r1 && blockIdx.x == 0 && threadIdx.x == 0) b.scal[6 + ((step + 1) & 1)] = 0.0;
+  double wv = 0.0;  // max |w| on this molecule's O (MLP scale)
+  if (m < p.M) {
   const double cs = clip ? b.scal[4] : 1.0;
   const int rk = b.rank[m];
   double xm[9], um[9], wm[9];
@@ -150,6 +163,7 @@ __global__ __launch_bounds__(MOL_BLOCK) void k_step_rev(SysParams p, int precisi
       else
         b.ud[3 * rk + a] = make_double4(um[3 * a], um[3 * a + 1], um[3 * a + 2], 0.0);
     }
+    wv = fmax(fabs(wm[0]), fmax(fabs(wm[1]), fabs(wm[2])));
     double F[9], h[9], vj[3];
     bonded_mol<true>(p, xm, um, wm, F, h, vj, b.flags, b.O64[rk]);
 #pragma unroll
@@ -161,6 +175,8 @@ __global__ __launch_bounds__(MOL_BLOCK) void k_step_rev(SysParams p, int precisi
     b.vjb[3 * (size_t)m + 1] = vj[1];
     b.vjb[3 * (size_t)m + 2] = vj[2];
   }
+  }
+  if (mlp && do_r1) wmax_fold(wv, b.scal + 6 + (step & 1));
 }
 
 // Bonded terms of the current state alone (start of a trajectory and single evaluations):
@@ -168,9 +184,10 @@ __global__ __launch_bounds__(MOL_BLOCK) void k_step_rev(SysParams p, int precisi
 template <bool ADJ>
 __global__ __launch_bounds__(MOL_BLOCK) void k_bonded(SysParams p, DevBufs b, const double* csrc,
                                                       double cO, double cH, const double* wsrc,
-                                                      double wO, double wH) {
+                                                      double wO, double wH, int mlp) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= p.M) return;
+  double wv = 0.0;
+  if (m < p.M) {
   double xm[9], um[9], wm[9];
 #pragma unroll
   for (int t = 0; t < 9; ++t) {
@@ -180,6 +197,7 @@ __global__ __launch_bounds__(MOL_BLOCK) void k_bonded(SysParams p, DevBufs b, co
       wm[t] = wsrc[9 * (size_t)m + t] * (t < 3 ? wO : wH);
     }
   }
+  if (ADJ) wv = fmax(fabs(wm[0]), fmax(fabs(wm[1]), fabs(wm[2])));
   double F[9], h[9], vj[3];
   bonded_mol<ADJ>(p, xm, um, wm, F, h, vj, b.flags, b.O64[b.rank[m]]);
 #pragma unroll
@@ -192,6 +210,8 @@ __global__ __launch_bounds__(MOL_BLOCK) void k_bonded(SysParams p, DevBufs b, co
     b.vjb[3 * (size_t)m + 1] = vj[1];
     b.vjb[3 * (size_t)m + 2] = vj[2];
   }
+  }
+  if (ADJ && mlp) wmax_fold(wv, b.scal + 6);
 }
 
 // Gradient seed g (b.grad) into the accumulators: reset (A = B = g(1+e), C = 0,
@@ -270,15 +290,15 @@ void launch_bonded(const Launch& L, bool adjoint, const double* csrc, double cO,
                    const double* wsrc, double wO, double wH) {
   const int g = (L.p.M + MOL_BLOCK - 1) / MOL_BLOCK;
   if (adjoint)
-    k_bonded<true><<<g, MOL_BLOCK, 0, L.s>>>(L.p, L.b, csrc, cO, cH, wsrc, wO, wH);
+    k_bonded<true><<<g, MOL_BLOCK, 0, L.s>>>(L.p, L.b, csrc, cO, cH, wsrc, wO, wH, L.use_mlp);
   else
-    k_bonded<false><<<g, MOL_BLOCK, 0, L.s>>>(L.p, L.b, csrc, cO, cH, wsrc, wO, wH);
+    k_bonded<false><<<g, MOL_BLOCK, 0, L.s>>>(L.p, L.b, csrc, cO, cH, wsrc, wO, wH, 0);
 }
 
 void launch_step_rev(const Launch& L, bool do_r2, bool do_r1, int step, uint64_t seed, double dt,
                      double e, double s, double kT, int nslots, bool clip) {
   k_step_rev<<<(L.p.M + MOL_BLOCK - 1) / MOL_BLOCK, MOL_BLOCK, 0, L.s>>>(
-      L.p, L.precision, L.b, do_r2, do_r1, step, seed, dt, e, s, kT, nslots, clip);
+      L.p, L.precision, L.b, do_r2, do_r1, step, seed, dt, e, s, kT, nslots, clip, L.use_mlp);
 }
 
 void launch_seed(const Launch& L, bool reset, double e) {

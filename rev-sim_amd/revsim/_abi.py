@@ -26,6 +26,7 @@ ERRORS = {
 }
 
 PHASES = ("fwd_pair", "fwd_step", "rev_pair", "rev_step", "list", "loss", "mlp")
+MLP_NW = 5313
 
 _i32 = ctypes.c_int32
 _f64 = ctypes.c_double
@@ -152,6 +153,9 @@ SIGNATURES = [
      [_vp, _dp, _dp, ctypes.c_uint64, ctypes.c_int32, _dp, _dp, _dp, _dp]),
     ("rs_value_and_grad", ctypes.c_int,
      [_vp, _dp, _dp, ctypes.c_uint64, ctypes.c_int32, _dp, _dp, _fp]),
+    ("rs_set_mlp", ctypes.c_int, [_vp, _fp, ctypes.c_int64]),
+    ("rs_mlp_grad", ctypes.c_int, [_vp, _dp, ctypes.c_int64]),
+    ("rs_mlp_table", ctypes.c_int, [_vp, _dp, _dp, ctypes.c_int32, _ip, _dp]),
     ("rs_forces", ctypes.c_int, [_vp, _dp, _dp]),
     ("rs_adjoint_terms", ctypes.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp]),
     ("rs_noise", ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.c_int32, _dp]),

@@ -71,6 +71,26 @@ class Simulator:
         xi = None if xi is None else self._vec(xi, n_steps * self.N3)
         self._chk(self.lib.rs_set_external_noise(self.h, dptr(xi), int(n_steps)))
 
+    # -- radial-basis MLP on O-O pairs -------------------------------------------------------
+    def set_mlp(self, w):
+        """Packed weights (W1, b1, W2, b2, w3, b3; 5313 values), rounded to bfloat16 on entry."""
+        w = np.ascontiguousarray(w, dtype=np.float32).reshape(-1)
+        self._chk(self.lib.rs_set_mlp(self.h, fptr(w), w.size))
+
+    def mlp_grad(self):
+        d = np.empty(_abi.MLP_NW)
+        self._chk(self.lib.rs_mlp_grad(self.h, dptr(d), d.size))
+        return d
+
+    def mlp_table(self):
+        n = ctypes.c_int32()
+        rlo = ctypes.c_double()
+        self._chk(self.lib.rs_mlp_table(self.h, None, None, 0, ctypes.byref(n), ctypes.byref(rlo)))
+        f, df = np.empty(n.value), np.empty(n.value)
+        self._chk(self.lib.rs_mlp_table(self.h, dptr(f), dptr(df), n.value, ctypes.byref(n), ctypes.byref(rlo)))
+        r = rlo.value + np.arange(n.value) * (self.cfg.cutoff - rlo.value) / (n.value - 1)
+        return r, f, df
+
     # -- trajectories -----------------------------------------------------------------------
     def forward(self, x0, v0, seed: int, n_steps: int):
         x0, v0 = self._vec(x0, self.N3), self._vec(v0, self.N3)
