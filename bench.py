@@ -79,6 +79,58 @@ def cpu_baseline(wl, x, v, theta, target, steps):
     }
 
 
+def timed_trajectories(sim, x, v, seed, n_steps, warmup, steps, tag):
+    """W untimed then K timed value_and_grad trajectories; per-phase HIP-event times of the timed
+    ones (events recorded on the library's stream)."""
+    for w in range(warmup):
+        t = time.perf_counter()
+        sim.value_and_grad(x, v, seed, n_steps)
+        log(f"[bench] {tag} warmup {w}: {time.perf_counter() - t:.3f} s")
+    sim.enable_timing(True)
+    sim.timings()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        loss, grad = sim.value_and_grad(x, v, seed, n_steps)[:2]
+    elapsed = time.perf_counter() - t0
+    tim = sim.timings()
+    sim.enable_timing(False)
+    log(f"[bench] {tag}: {steps} trajectories in {elapsed:.3f} s -> {steps * n_steps / elapsed:.1f} "
+        f"gradient steps/s; phases {json.dumps(tim)}")
+    return loss, grad, elapsed, tim
+
+
+def grad_error_by_horizon(wl, extra, x, v, theta, target, seed, horizons, mlp=None):
+    """dL/dθ of the FP32 path against the FP64 path from the same start (the FP64 path is
+    parity-tested against the FP64 CPU oracle)."""
+    import revsim
+
+    errs = {}
+    for horizon in horizons:
+        ex = dict(extra)
+        if ex.get("msd_window_steps", 0) > horizon:
+            ev = ex["msd_every"]
+            ex["msd_window_steps"] = max(2 * ev, (horizon // 2) // (2 * ev) * (2 * ev))
+        g = {}
+        for p_ in (revsim.RS_PRECISION_FP32, revsim.RS_PRECISION_FP64):
+            c_ = revsim.make_config(wl["n_molecules"], wl["box"], wl["cutoff"], precision=p_,
+                                    use_mlp=int(mlp is not None), **ex)
+            with revsim.Simulator(c_, device=0) as s_:
+                s_.set_theta(theta)
+                s_.set_target_rdf(target)
+                if mlp is not None:
+                    s_.set_mlp(mlp)
+                g[p_] = s_.value_and_grad(x, v, seed, horizon)[1]
+        g32, g64 = g[revsim.RS_PRECISION_FP32], g[revsim.RS_PRECISION_FP64]
+        errs[str(horizon)] = {
+            "normwise": float(np.linalg.norm(g32 - g64) / np.linalg.norm(g64)),
+            "per_param": (np.abs(g32 - g64) / np.maximum(np.abs(g64), 1e-300)).tolist(),
+            "grad_norm_fp64": float(np.linalg.norm(g64)),
+        }
+        log(f"[bench] dL/dθ FP32 vs FP64 at {horizon} steps{' (+MLP)' if mlp is not None else ''}: "
+            f"{errs[str(horizon)]['normwise']:.3e}")
+    return errs
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,6 +144,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-check", action="store_true")
     ap.add_argument("--pair-chunks", type=int, default=0)
+    ap.add_argument("--no-mlp", action="store_true", help="skip the +MLP leg")
     a = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -130,23 +183,8 @@ def main():
     sim.set_target_rdf(target)
     seed = revsim.system.SEED_NOISE
 
-    for w in range(a.warmup):
-        t = time.perf_counter()
-        sim.value_and_grad(x, v, seed, n_steps)
-        log(f"[bench] warmup {w}: {time.perf_counter() - t:.3f} s")
-
-    sim.enable_timing(True)
-    sim.timings()
-    t0 = time.perf_counter()
-    for k in range(a.steps):
-        loss, grad = sim.value_and_grad(x, v, seed, n_steps)
-    t1 = time.perf_counter()
-    tim = sim.timings()
-    sim.enable_timing(False)
-    elapsed = t1 - t0
+    loss, grad, elapsed, tim = timed_trajectories(sim, x, v, seed, n_steps, a.warmup, a.steps, "plain")
     value = a.steps * n_steps / elapsed
-    log(f"[bench] timed {a.steps} trajectories in {elapsed:.3f} s -> {value:.1f} gradient steps/s")
-    log(f"[bench] phases: {json.dumps(tim)}")
 
     # roofline of the dominant kernel (the adjoint pair kernel)
     xN, _, _ = sim.forward(x, v, seed, n_steps)
@@ -204,35 +242,44 @@ def main():
         "roofline": roof,
     }
 
+    if not a.no_mlp:
+        # the same workload with the radial-basis MLP correction on O-O pairs (Xavier weights)
+        from revsim.system import mlp_init
+
+        cfg_m = revsim.make_config(wl["n_molecules"], wl["box"], wl["cutoff"], precision=prec,
+                                   pair_chunks=a.pair_chunks, **dict(extra, use_mlp=1))
+        with revsim.Simulator(cfg_m, device=0) as sm:
+            sm.set_theta(theta)
+            sm.set_target_rdf(target)
+            sm.set_mlp(mlp_init())
+            loss_m, grad_m, el_m, tim_m = timed_trajectories(sm, x, v, seed, n_steps, a.warmup, a.steps, "+MLP")
+        vm = a.steps * n_steps / el_m
+        out["mlp"] = {
+            "value": round(vm, 2),
+            "unit": "gradient steps/s",
+            "ms_per_step": round(1e3 * el_m / a.steps, 3),
+            "cost_ratio_vs_plain": round(el_m / elapsed, 4),
+            "loss": loss_m,
+            "dL_dtheta": grad_m.tolist(),
+            "dL_dtheta_rel_diff_vs_plain": float(np.linalg.norm(grad_m - grad) / np.linalg.norm(grad)),
+            "phases_ms": {k: round(vv["ms"], 3) for k, vv in tim_m.items()},
+            "network": "16 RBF -> 64 -> 64 -> 1 SiLU, bf16 weights (Xavier, seed 15); table of f, f' on "
+                       "512 nodes built by bf16 MFMA per trajectory, cubic Hermite in the pair kernels",
+        }
+
     if not a.no_fp64_check and prec == revsim.RS_PRECISION_FP32:
-        # dL/dθ of the FP32 path against the FP64 path (itself parity-tested against the FP64 CPU
-        # oracle) from the same start, at a well-conditioned horizon and at the full length.
-        errs = {}
-        for horizon in sorted({min(100, n_steps), n_steps}):
-            ex = dict(extra)
-            if ex.get("msd_window_steps", 0) > horizon:
-                ev = ex["msd_every"]
-                ex["msd_window_steps"] = max(2 * ev, (horizon // 2) // (2 * ev) * (2 * ev))
-            g = {}
-            for p_ in (revsim.RS_PRECISION_FP32, revsim.RS_PRECISION_FP64):
-                c_ = revsim.make_config(wl["n_molecules"], wl["box"], wl["cutoff"], precision=p_, **ex)
-                with revsim.Simulator(c_, device=0) as s_:
-                    s_.set_theta(theta)
-                    s_.set_target_rdf(target)
-                    g[p_] = s_.value_and_grad(x, v, seed, horizon)[1]
-            g32, g64 = g[revsim.RS_PRECISION_FP32], g[revsim.RS_PRECISION_FP64]
-            errs[str(horizon)] = {
-                "normwise": float(np.linalg.norm(g32 - g64) / np.linalg.norm(g64)),
-                "per_param": (np.abs(g32 - g64) / np.maximum(np.abs(g64), 1e-300)).tolist(),
-                "grad_norm_fp64": float(np.linalg.norm(g64)),
-            }
-            log(f"[bench] dL/dθ FP32 vs FP64 at {horizon} steps: {errs[str(horizon)]['normwise']:.3e}")
+        horizons = sorted({min(100, n_steps), n_steps})
         out["grad_rel_err_vs_fp64"] = {
-            "by_horizon_steps": errs,
+            "by_horizon_steps": grad_error_by_horizon(wl, extra, x, v, theta, target, seed, horizons),
             "reference": "same start, FP64 pair arithmetic (parity-tested against the FP64 CPU oracle)",
             "note": "beyond ~200 steps the adjoint grows like exp(lambda t) (chaotic MD) and FP32/FP64 "
                     "gradients decorrelate; the reference trains with n_steps_trunc = 200",
         }
+        if not a.no_mlp:
+            from revsim.system import mlp_init
+
+            out["mlp"]["grad_rel_err_vs_fp64"] = grad_error_by_horizon(
+                wl, extra, x, v, theta, target, seed, [min(100, n_steps)], mlp=mlp_init())
 
     if not a.no_cpu_baseline:
         log("[bench] cpu baseline ...")

@@ -185,14 +185,30 @@ class Adam:
 
 def train(sim: Simulator, x0, v0, params_start, n_epochs: int, n_steps: int, n_steps_equil: int = 0,
           learning_rate: float = 2e-3, grad_clamp_val: float = 1e3, scale_params: bool = True,
-          seed: int = 1, log=None):
+          seed: int = 1, log=None, mlp_weights=None, eval_fn=None):
     """The epoch loop of water/water.jl:1623-1698 on one context: fresh noise seed per epoch,
     optional equilibration, forward + reverse, a log line per epoch, Adam on parameters scaled by
     their starting values with the gradient clamped to ±grad_clamp_val (water/water.jl:1682-1696).
-    Returns the parameter trajectory and the per-epoch (loss, grads)."""
+
+    With `mlp_weights` (packed 5313-value vector) the radial-basis MLP correction on O-O pairs is
+    trained jointly: one Adam state over (scaled θ, MLP weights), the weights kept in FP64 as the
+    master copy and uploaded (rounded to bfloat16 by the library) before every epoch. The context
+    must have been created with use_mlp = 1.
+
+    Returns (params, history) — or (params, mlp_weights, history) when training the MLP — with
+    history the per-epoch (loss, dL/dparams[, dL/dW]) and, if `eval_fn` is given, its value on the
+    epoch's parameters appended."""
     params = np.asarray(params_start, dtype=np.float64).copy()
     p0 = params.copy()
-    opt = Adam(learning_rate, params.size)
+    joint = mlp_weights is not None
+    if joint:
+        if not sim.cfg.use_mlp:
+            raise ValueError("MLP training needs a context created with use_mlp = 1")
+        W = np.asarray(mlp_weights, dtype=np.float64).reshape(-1).copy()
+        if W.size != _abi.MLP_NW:
+            raise ValueError(f"mlp_weights must hold {_abi.MLP_NW} values")
+    n_p = params.size
+    opt = Adam(learning_rate, n_p + (W.size if joint else 0))
     scaled = np.ones_like(params)
     rng = np.random.Generator(np.random.Philox(seed))
     history = []
@@ -200,20 +216,36 @@ def train(sim: Simulator, x0, v0, params_start, n_epochs: int, n_steps: int, n_s
     for epoch in range(1, n_epochs + 1):
         s_eq, s_run = (int(k) for k in rng.integers(0, 2 ** 63, size=2))
         sim.set_theta(params_to_theta(params))
+        if joint:
+            sim.set_mlp(W.astype(np.float32))
         xs, vs = x, v
         if n_steps_equil:
             xs, vs, _ = sim.forward(x, v, s_eq, n_steps_equil)
-        loss, g = sim.value_and_grad(xs, vs, s_run, n_steps)
-        grads = theta_grad_to_params(g)[: params.size]
-        history.append((loss, grads.copy()))
+        if joint:
+            loss, g, gW = sim.value_and_grad(xs, vs, s_run, n_steps, mlp_grad_size=W.size)
+            gW = gW.astype(np.float64)
+        else:
+            loss, g = sim.value_and_grad(xs, vs, s_run, n_steps)
+        grads = theta_grad_to_params(g)[:n_p]
+        rec = [loss, grads.copy()] + ([gW.copy()] if joint else [])
+        if eval_fn is not None:
+            rec.append(eval_fn(params, W if joint else None))
+        history.append(tuple(rec))
         if log is not None:
             log(f"{epoch} {loss} " + " ".join(map(str, params)) + " " + " ".join(map(str, grads)))
+        gp = grads * p0 if scale_params else grads
+        gall = np.clip(np.concatenate([gp, gW]) if joint else gp, -grad_clamp_val, grad_clamp_val)
+        pall = np.concatenate([scaled if scale_params else params, W]) if joint else (
+            scaled if scale_params else params)
+        pall = opt.step(pall, gall)
         if scale_params:
-            scaled = opt.step(scaled, np.clip(grads * p0, -grad_clamp_val, grad_clamp_val))
+            scaled = pall[:n_p]
             params = scaled * p0
         else:
-            params = opt.step(params, np.clip(grads, -grad_clamp_val, grad_clamp_val))
-    return params, history
+            params = pall[:n_p].copy()
+        if joint:
+            W = pall[n_p:].copy()
+    return (params, W, history) if joint else (params, history)
 
 
 __all__ = ["System", "Context", "forward_sim", "reverse_sim", "train", "Adam", "params_to_theta",

@@ -1,0 +1,120 @@
+"""CPU tests of the training loop's host logic (revsim.interface.train, water/water.jl:1623-1698),
+run over the FP64 oracle through a small adapter with the Simulator's call surface.
+
+Pins: Adam's first step has the closed form p1 = p0 - lr * g / (|g| + eps) elementwise (m̂ = g,
+v̂ = g²), applied to the scaled parameters (scale p0, gradient g * p0) and to the MLP weights; the
+clamp acts before Adam; the gradients logged per epoch are the oracle's at that epoch's
+parameters. The training gain of the MLP correction is measured on the GPU (test_gpu_train.py)."""
+import numpy as np
+import pytest
+
+
+class OracleSim:
+    """The Simulator methods train() uses, over the oracle (tests only)."""
+
+    def __init__(self, cfg):
+        import oracle as O
+
+        self.cfg = cfg
+        self.o = O.Oracle(cfg)
+
+    def set_theta(self, th):
+        self.o.set_theta(th)
+
+    def set_mlp(self, w):
+        self.o.set_mlp(w)
+
+    def set_target_rdf(self, g):
+        self.o.set_target_rdf(g)
+
+    def forward(self, x, v, seed, n):
+        return self.o.forward(x, v, seed, n)
+
+    def value_and_grad(self, x, v, seed, n, mlp_grad_size=0):
+        L, g = self.o.value_and_grad(x, v, seed, n)
+        if mlp_grad_size:
+            return L, g, self.o.mlp_grad().astype(np.float32)
+        return L, g
+
+
+def _setup(use_mlp, n_steps_loss=0):
+    import revsim
+
+    wb = revsim.water_box(216, 1.862)
+    cfg = revsim.make_config(216, 1.862, 0.8, use_mlp=use_mlp, n_steps_loss=n_steps_loss)
+    sim = OracleSim(cfg)
+    sim.set_theta(revsim.DEFAULT_THETA)
+    g0 = sim.o.rdf(wb.coords)[0]
+    sim.set_target_rdf(revsim.perturbed_target(g0))
+    return sim, wb
+
+
+def test_adam_closed_form_first_step():
+    from revsim.interface import Adam
+
+    rng = np.random.default_rng(0)
+    p, g = rng.standard_normal(7), rng.standard_normal(7)
+    lr = 1e-3
+    opt = Adam(lr, 7)
+    p1 = opt.step(p, g)
+    assert np.allclose(p1, p - lr * g / (np.abs(g) + 1e-8), rtol=0, atol=1e-15)
+    # second step with the same gradient: m̂ = g, v̂ = g² again
+    p2 = opt.step(p1, g)
+    assert np.allclose(p2, p1 - lr * g / (np.abs(g) + 1e-8), rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("joint", [False, True])
+def test_train_first_epoch_update(joint):
+    import revsim
+    from revsim import interface as I
+    from revsim.system import mlp_init
+
+    sim, wb = _setup(int(joint))
+    p0 = np.array([0.315, 0.636, -0.834, 462750.0, 0.09572, 836.8, 1.82421813418])
+    W0 = mlp_init().astype(np.float64)
+    lr = 1e-3
+    kw = dict(mlp_weights=W0) if joint else {}
+    out = I.train(sim, wb.coords, wb.velocities, p0, 1, 10, learning_rate=lr, **kw)
+    hist = out[-1]
+    # the logged gradient is the oracle's at p0 with the epoch's seed
+    rng = np.random.Generator(np.random.Philox(1))
+    _, s_run = (int(k) for k in rng.integers(0, 2 ** 63, size=2))
+    sim.set_theta(I.params_to_theta(p0))
+    if joint:
+        sim.set_mlp(W0.astype(np.float32))
+        L, g, gW = sim.value_and_grad(wb.coords, wb.velocities, s_run, 10, mlp_grad_size=W0.size)
+    else:
+        L, g = sim.value_and_grad(wb.coords, wb.velocities, s_run, 10)
+    gp = I.theta_grad_to_params(g)
+    assert hist[0][0] == L and np.array_equal(hist[0][1], gp)
+    # one Adam step on the scaled parameters, gradient clamped to ±1e3 first
+    gs = np.clip(gp * p0, -1e3, 1e3)
+    assert np.allclose(out[0], p0 * (1 - lr * gs / (np.abs(gs) + 1e-8)), rtol=1e-14, atol=0)
+    if joint:
+        gw = np.clip(gW.astype(np.float64), -1e3, 1e3)
+        assert np.array_equal(hist[0][2], gW.astype(np.float64))
+        assert np.allclose(out[1], W0 - lr * gw / (np.abs(gw) + 1e-8), rtol=0, atol=1e-12)
+
+
+def test_train_rejects_mlp_without_use_mlp():
+    from revsim import interface as I
+    from revsim.system import mlp_init
+
+    sim, wb = _setup(0)
+    with pytest.raises(ValueError):
+        I.train(sim, wb.coords, wb.velocities, np.ones(7), 1, 4, mlp_weights=mlp_init())
+    sim.cfg.use_mlp = 1
+    with pytest.raises(ValueError):
+        I.train(sim, wb.coords, wb.velocities, np.ones(7), 1, 4, mlp_weights=np.zeros(10))
+
+
+def test_theta_param_mapping_round_trip():
+    from revsim import interface as I
+
+    p = np.array([0.3, 0.6, -0.8, 4e5, 0.0957, 800.0, 1.9])
+    th = I.params_to_theta(p)
+    assert th[2] == 0.4 and th.size == 6
+    g = np.arange(1.0, 7.0)
+    gp = I.theta_grad_to_params(g)
+    # L(q_O) = L(q_H = -q_O/2): dL/dq_O = -dL/dq_H / 2; θ0 held fixed
+    assert gp[2] == -1.5 and gp[6] == 0.0 and np.array_equal(gp[[0, 1, 3, 4, 5]], g[[0, 1, 3, 4, 5]])
