@@ -79,23 +79,38 @@ def cpu_baseline(wl, x, v, theta, target, steps):
     }
 
 
+def fit_msd_window(extra, n_steps):
+    """Shorter trajectories than the MSD window: the MSD over their second half, with at least two
+    frame intervals (window a multiple of 2 * msd_every, as the library requires)."""
+    ex = dict(extra)
+    if ex.get("msd_window_steps", 0) > n_steps:
+        every = ex["msd_every"]
+        if n_steps // 2 < 2 * every:
+            every = max(1, n_steps // 4)
+        ex["msd_every"] = every
+        ex["msd_window_steps"] = max(2 * every, (n_steps // 2) // (2 * every) * (2 * every))
+    return ex
+
+
 def timed_trajectories(sim, x, v, seed, n_steps, warmup, steps, tag):
-    """W untimed then K timed value_and_grad trajectories; per-phase HIP-event times of the timed
-    ones (events recorded on the library's stream)."""
+    """W untimed then K timed value_and_grad trajectories (no instrumentation inside the timed
+    region); then one more trajectory with per-phase HIP events recorded on the library's stream,
+    whose phase times are returned (milliseconds for that one trajectory)."""
     for w in range(warmup):
         t = time.perf_counter()
         sim.value_and_grad(x, v, seed, n_steps)
         log(f"[bench] {tag} warmup {w}: {time.perf_counter() - t:.3f} s")
-    sim.enable_timing(True)
-    sim.timings()
     t0 = time.perf_counter()
     for k in range(steps):
         loss, grad = sim.value_and_grad(x, v, seed, n_steps)[:2]
     elapsed = time.perf_counter() - t0
+    sim.enable_timing(True)
+    sim.timings()
+    sim.value_and_grad(x, v, seed, n_steps)
     tim = sim.timings()
     sim.enable_timing(False)
     log(f"[bench] {tag}: {steps} trajectories in {elapsed:.3f} s -> {steps * n_steps / elapsed:.1f} "
-        f"gradient steps/s; phases {json.dumps(tim)}")
+        f"gradient steps/s; phases of one instrumented trajectory {json.dumps(tim)}")
     return loss, grad, elapsed, tim
 
 
@@ -106,10 +121,7 @@ def grad_error_by_horizon(wl, extra, x, v, theta, target, seed, horizons, mlp=No
 
     errs = {}
     for horizon in horizons:
-        ex = dict(extra)
-        if ex.get("msd_window_steps", 0) > horizon:
-            ev = ex["msd_every"]
-            ex["msd_window_steps"] = max(2 * ev, (horizon // 2) // (2 * ev) * (2 * ev))
+        ex = fit_msd_window(extra, horizon)
         g = {}
         for p_ in (revsim.RS_PRECISION_FP32, revsim.RS_PRECISION_FP64):
             c_ = revsim.make_config(wl["n_molecules"], wl["box"], wl["cutoff"], precision=p_,
@@ -145,6 +157,9 @@ def main():
     ap.add_argument("--no-fp64-check", action="store_true")
     ap.add_argument("--pair-chunks", type=int, default=0)
     ap.add_argument("--no-mlp", action="store_true", help="skip the +MLP leg")
+    ap.add_argument("--calibrate-copy-mb", type=int, default=0,
+                    help="also run 5 device copies of this many MiB (a kernel of known HBM traffic, "
+                         "to scale FETCH_SIZE / WRITE_SIZE counters by)")
     a = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -160,9 +175,7 @@ def main():
     wl.pop("n_steps", None)
     prec = revsim.RS_PRECISION_FP32 if a.precision == "fp32" else revsim.RS_PRECISION_FP64
     extra = {k: wl[k] for k in ("msd_window_steps", "msd_every", "w_msd", "use_mlp") if k in wl}
-    if extra.get("msd_window_steps", 0) > n_steps:  # shorter trajectories: MSD over their 2nd half
-        every = extra["msd_every"]
-        extra["msd_window_steps"] = max(2 * every, (n_steps // 2) // (2 * every) * (2 * every))
+    extra = fit_msd_window(extra, n_steps)
     cfg = revsim.make_config(wl["n_molecules"], wl["box"], wl["cutoff"], precision=prec,
                              pair_chunks=a.pair_chunks, **extra)
     wb = revsim.water_box(wl["n_molecules"], wl["box"])
@@ -238,7 +251,7 @@ def main():
                    "atoms": wb.n_atoms, "n_steps": n_steps, "parallelism": "single GPU"},
         "loss": loss,
         "dL_dtheta": grad.tolist(),
-        "phases_ms": {k: round(vv["ms"], 3) for k, vv in tim.items()},
+        "phases_ms_one_trajectory": {k: round(vv["ms"], 3) for k, vv in tim.items()},
         "roofline": roof,
     }
 
@@ -262,7 +275,7 @@ def main():
             "loss": loss_m,
             "dL_dtheta": grad_m.tolist(),
             "dL_dtheta_rel_diff_vs_plain": float(np.linalg.norm(grad_m - grad) / np.linalg.norm(grad)),
-            "phases_ms": {k: round(vv["ms"], 3) for k, vv in tim_m.items()},
+            "phases_ms_one_trajectory": {k: round(vv["ms"], 3) for k, vv in tim_m.items()},
             "network": "16 RBF -> 64 -> 64 -> 1 SiLU, bf16 weights (Xavier, seed 15); table of f, f' on "
                        "512 nodes built by bf16 MFMA per trajectory, cubic Hermite in the pair kernels",
         }
@@ -280,6 +293,12 @@ def main():
 
             out["mlp"]["grad_rel_err_vs_fp64"] = grad_error_by_horizon(
                 wl, extra, x, v, theta, target, seed, [min(100, n_steps)], mlp=mlp_init())
+
+    if a.calibrate_copy_mb:
+        nb = a.calibrate_copy_mb << 20
+        sim.copy_calibration(nb, 5)
+        out["copy_calibration"] = {"bytes_read": nb, "bytes_written": nb, "launches": 5,
+                                   "kernel": "k_copy16 (16-byte vector copy, librevsim)"}
 
     if not a.no_cpu_baseline:
         log("[bench] cpu baseline ...")

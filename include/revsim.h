@@ -170,6 +170,11 @@ int rs_timings(rs_ctx* ctx, double* ms_by_phase, int32_t n);
 int rs_timing_counts(rs_ctx* ctx, int64_t* counts, int32_t n);
 /* Enable/disable per-phase event timing (off by default; adds event records per launch). */
 int rs_enable_timing(rs_ctx* ctx, int32_t on);
+/* Measurement aid, no counterpart in the reference: `reps` launches of a plain 16-byte vector copy
+   kernel (k_copy16) over `bytes` bytes between two scratch buffers allocated for the call, so
+   hardware traffic counters can be scaled against a kernel whose HBM traffic is known exactly
+   (bytes read + bytes written per launch). */
+int rs_copy_calibration(rs_ctx* ctx, int64_t bytes, int32_t reps);
 
 #ifdef __cplusplus
 }

@@ -153,9 +153,11 @@ int dalloc(rs_ctx* c, T** p, size_t n) {
 int alloc_list(rs_ctx* c) {
   DevBufs& b = c->L.b;
   if (b.nbr) hipFree(b.nbr);
-  b.nbr = nullptr;
+  if (b.nbr_tmp) hipFree(b.nbr_tmp);
+  b.nbr = b.nbr_tmp = nullptr;
   const size_t lanes = (size_t)pair_blocks(c->L.p) * RS_BLOCK;
-  return dalloc(c, &b.nbr, lanes * c->L.p.capc);
+  int r = dalloc(c, &b.nbr, lanes * c->L.p.capc);
+  return r ? r : dalloc(c, &b.nbr_tmp, lanes * c->L.p.capc);
 }
 
 int alloc_all(rs_ctx* c) {
@@ -536,6 +538,12 @@ int single_eval(rs_ctx* c, const double* x, bool with_u, Body body) {
   }
 }
 
+// Plain 16-byte vector copy: reads and writes exactly n * 16 bytes (grid-stride loop).
+__global__ void k_copy16(const uint4* __restrict__ a, uint4* __restrict__ b, size_t n) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) b[i] = a[i];
+}
+
 int check_ctx(rs_ctx* c) {
   if (!c) return fail(nullptr, RS_E_CONFIG, "null context");
   HIPCHK(c, hipSetDevice(c->device));
@@ -648,7 +656,7 @@ int rs_destroy(rs_ctx* c) {
   DevBufs& b = c->L.b;
   void* ptrs[] = {b.x, b.v, b.F, b.h, b.A, b.B, b.C, b.D, b.grad, b.cell, b.cell_count,
                   b.cell_start, b.cell_fill, b.tmp, b.order, b.rank, b.posq, b.pos64, b.uf,
-                  b.ud, b.O64, b.nbr, b.ncnt, b.flags, b.slots, b.vjb, b.dth, b.scal, b.hist, b.rdfG,
+                  b.ud, b.O64, b.nbr, b.nbr_tmp, b.ncnt, b.flags, b.slots, b.vjb, b.dth, b.scal, b.hist, b.rdfG,
                   b.target, b.rdf_g, b.frames, b.msd_del, b.msd_acc, b.msd_grad, b.ext_noise,
                   b.hpart, b.mlp_w, b.mlp_wb, b.mlp_tab, b.mlp_slot, b.mlp_G, b.mlp_part, b.mlp_dW};
   for (void* q : ptrs)
@@ -959,6 +967,31 @@ int rs_enable_timing(rs_ctx* c, int32_t on) {
   int r = check_ctx(c);
   if (r) return r;
   c->timing = on != 0;
+  return 0;
+}
+
+int rs_copy_calibration(rs_ctx* c, int64_t bytes, int32_t reps) {
+  int r = check_ctx(c);
+  if (r) return r;
+  if (bytes <= 0 || bytes % 16 || bytes > ((int64_t)1 << 34) || reps < 1 || reps > 100)
+    return fail(c, RS_E_CONFIG, "rs_copy_calibration: bytes must be a positive multiple of 16 up to 16 GiB, reps 1..100");
+  uint4 *a = nullptr, *b = nullptr;
+  HIPCHK(c, hipMalloc((void**)&a, (size_t)bytes));
+  if (hipMalloc((void**)&b, (size_t)bytes) != hipSuccess) {
+    hipFree(a);
+    return fail(c, RS_E_HIP, "rs_copy_calibration: hipMalloc failed");
+  }
+  const size_t n = (size_t)bytes / 16;
+  const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 16384);
+  hipError_t e = hipMemsetAsync(a, 0, (size_t)bytes, c->s);
+  for (int k = 0; k < reps && e == hipSuccess; ++k) {
+    k_copy16<<<blocks, 256, 0, c->s>>>(a, b, n);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(c->s);
+  hipFree(a);
+  hipFree(b);
+  if (e != hipSuccess) return fail(c, RS_E_HIP, std::string("rs_copy_calibration: ") + hipGetErrorString(e));
   return 0;
 }
 

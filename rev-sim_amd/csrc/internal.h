@@ -18,6 +18,7 @@ struct DevBufs {
   double4* O64;    // O position at the last list build, sorted
   // molecule list, ELL by wave: nbr[((gtid/64)*capc + k)*64 + gtid%64], gtid = i*CH + c
   int *nbr, *ncnt;
+  int* nbr_tmp;    // list-build scratch, same layout as nbr (per-lane discovery order)
   int* flags;
   double* slots;   // per-block θ-VJP partials [nblocks * 8]
   double* vjb;     // bonded θ-VJP per molecule [3M]: (k_bond, r0, k_angle)

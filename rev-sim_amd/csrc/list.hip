@@ -152,51 +152,75 @@ __global__ void k_pack(SysParams p, int precision, const int* __restrict__ order
 
 template <int CH>
 __global__ __launch_bounds__(RS_BLOCK) void k_list_build(SysParams p, const int* __restrict__ start,
-                                                         const double4* __restrict__ O64, int* nbr,
-                                                         int* ncnt, int* flags) {
+                                                         const double4* __restrict__ O64, int* tmp,
+                                                         int* nbr, int* ncnt, int* flags) {
+  // Pass 1: lane c tests every CH-th candidate of the 27-cell stencil (or of all molecules) and
+  // keeps the accepted ones in discovery order in its slot column of tmp.
   const int gtid = blockIdx.x * RS_BLOCK + threadIdx.x;
   const int i = gtid / CH, c = gtid % CH;
-  if (i >= p.M) return;
+  const bool act = i < p.M;
   const int lane = gtid & (RS_WAVE - 1);
-  int* out = nbr + (size_t)(gtid / RS_WAVE) * p.capc * RS_WAVE + lane;
-  const double4 oi = O64[i];
+  const size_t wbase = (size_t)(gtid / RS_WAVE) * p.capc * RS_WAVE;
+  int* tout = tmp + wbase + lane;
   int n = 0;
-  auto test = [&](int j) {
-    if (j == i) return;
-    double4 oj = O64[j];
-    double dx = mic1(oj.x - oi.x, p.L[0], p.invL[0]);
-    double dy = mic1(oj.y - oi.y, p.L[1], p.invL[1]);
-    double dz = mic1(oj.z - oi.z, p.L[2], p.invL[2]);
-    if (r2_of(dx, dy, dz) <= p.rlist2) {
-      if (n < p.capc) out[(size_t)n * RS_WAVE] = j;
-      ++n;
+  if (act) {
+    const double4 oi = O64[i];
+    auto test = [&](int j) {
+      if (j == i) return;
+      double4 oj = O64[j];
+      double dx = mic1(oj.x - oi.x, p.L[0], p.invL[0]);
+      double dy = mic1(oj.y - oi.y, p.L[1], p.invL[1]);
+      double dz = mic1(oj.z - oi.z, p.L[2], p.invL[2]);
+      if (r2_of(dx, dy, dz) <= p.rlist2) {
+        if (n < p.capc) tout[(size_t)n * RS_WAVE] = j;
+        ++n;
+      }
+    };
+    if (p.brute) {
+      for (int j = c; j < p.M; j += CH) test(j);
+    } else {
+      int ci = cell_index(p, oi.x, oi.y, oi.z);
+      int cx = ci % p.nc[0], cy = (ci / p.nc[0]) % p.nc[1], cz = ci / (p.nc[0] * p.nc[1]);
+      int off = 0;
+      for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+          for (int dx = -1; dx <= 1; ++dx) {
+            int ex = cx + dx, ey = cy + dy, ez = cz + dz;
+            ex += (ex < 0) ? p.nc[0] : (ex >= p.nc[0] ? -p.nc[0] : 0);
+            ey += (ey < 0) ? p.nc[1] : (ey >= p.nc[1] ? -p.nc[1] : 0);
+            ez += (ez < 0) ? p.nc[2] : (ez >= p.nc[2] ? -p.nc[2] : 0);
+            int e = (ez * p.nc[1] + ey) * p.nc[0] + ex;
+            int s = start[e], cnt = start[e + 1] - s;
+            int t0 = ((c - off) % CH + CH) % CH;
+            for (int t = t0; t < cnt; t += CH) test(s + t);
+            off += cnt;
+          }
     }
-  };
-  if (p.brute) {
-    for (int j = c; j < p.M; j += CH) test(j);
-  } else {
-    int ci = cell_index(p, oi.x, oi.y, oi.z);
-    int cx = ci % p.nc[0], cy = (ci / p.nc[0]) % p.nc[1], cz = ci / (p.nc[0] * p.nc[1]);
-    int off = 0;
-    for (int dz = -1; dz <= 1; ++dz)
-      for (int dy = -1; dy <= 1; ++dy)
-        for (int dx = -1; dx <= 1; ++dx) {
-          int ex = cx + dx, ey = cy + dy, ez = cz + dz;
-          ex += (ex < 0) ? p.nc[0] : (ex >= p.nc[0] ? -p.nc[0] : 0);
-          ey += (ey < 0) ? p.nc[1] : (ey >= p.nc[1] ? -p.nc[1] : 0);
-          ez += (ez < 0) ? p.nc[2] : (ez >= p.nc[2] ? -p.nc[2] : 0);
-          int e = (ez * p.nc[1] + ey) * p.nc[0] + ex;
-          int s = start[e], cnt = start[e + 1] - s;
-          int t0 = ((c - off) % CH + CH) % CH;
-          for (int t = t0; t < cnt; t += CH) test(s + t);
-          off += cnt;
-        }
   }
-  if (n > p.capc) {
-    atomicOr(&flags[FLAG_LIST_OVERFLOW], 1);
-    n = p.capc;
+  // Pass 2: the molecule's accepted candidates in (lane, discovery) order get consecutive global
+  // ranks g = (exclusive scan of the lane counts) + k and are dealt round-robin: rank g goes to
+  // sub-list g % CH, slot g / CH. Sub-lists then differ in length by at most one, so the lanes of a
+  // wave run the same number of pair iterations.
+  int incl = n;
+#pragma unroll
+  for (int o = 1; o < CH; o <<= 1) {
+    const int t = __shfl_up(incl, o, CH);
+    if (c >= o) incl += t;
   }
-  ncnt[gtid] = n;
+  const int T = __shfl(incl, CH - 1, CH);
+  const int first = incl - n;
+  const bool ovf = n > p.capc || T > CH * p.capc;
+  if (act) {
+    if (ovf) atomicOr(&flags[FLAG_LIST_OVERFLOW], 1);
+    const int nn = n < p.capc ? n : p.capc;
+    for (int k = 0; k < nn; ++k) {
+      const int g = first + k;
+      const int slot = g / CH;
+      if (slot < p.capc) nbr[wbase + (size_t)slot * RS_WAVE + (lane - c + g % CH)] = tout[(size_t)k * RS_WAVE];
+    }
+    const int mine = (T - c + CH - 1) / CH;
+    ncnt[gtid] = mine < p.capc ? mine : p.capc;
+  }
 }
 
 }  // namespace
@@ -218,9 +242,9 @@ void launch_list_build(const Launch& L, bool with_u) {
                                     b.pos64, b.uf, b.ud, b.O64);
   int gl = pair_blocks(p);
   switch (p.CH) {
-    case 4: k_list_build<4><<<gl, RS_BLOCK, 0, L.s>>>(p, b.cell_start, b.O64, b.nbr, b.ncnt, b.flags); break;
-    case 8: k_list_build<8><<<gl, RS_BLOCK, 0, L.s>>>(p, b.cell_start, b.O64, b.nbr, b.ncnt, b.flags); break;
-    case 16: k_list_build<16><<<gl, RS_BLOCK, 0, L.s>>>(p, b.cell_start, b.O64, b.nbr, b.ncnt, b.flags); break;
-    default: k_list_build<32><<<gl, RS_BLOCK, 0, L.s>>>(p, b.cell_start, b.O64, b.nbr, b.ncnt, b.flags); break;
+    case 4: k_list_build<4><<<gl, RS_BLOCK, 0, L.s>>>(p, b.cell_start, b.O64, b.nbr_tmp, b.nbr, b.ncnt, b.flags); break;
+    case 8: k_list_build<8><<<gl, RS_BLOCK, 0, L.s>>>(p, b.cell_start, b.O64, b.nbr_tmp, b.nbr, b.ncnt, b.flags); break;
+    case 16: k_list_build<16><<<gl, RS_BLOCK, 0, L.s>>>(p, b.cell_start, b.O64, b.nbr_tmp, b.nbr, b.ncnt, b.flags); break;
+    default: k_list_build<32><<<gl, RS_BLOCK, 0, L.s>>>(p, b.cell_start, b.O64, b.nbr_tmp, b.nbr, b.ncnt, b.flags); break;
   }
 }

@@ -166,6 +166,7 @@ SIGNATURES = [
     ("rs_timings", ctypes.c_int, [_vp, _dp, ctypes.c_int32]),
     ("rs_timing_counts", ctypes.c_int, [_vp, _lp, ctypes.c_int32]),
     ("rs_enable_timing", ctypes.c_int, [_vp, ctypes.c_int32]),
+    ("rs_copy_calibration", ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int32]),
 ]
 
 

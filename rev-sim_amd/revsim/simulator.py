@@ -160,6 +160,10 @@ class Simulator:
     def list_radius(self) -> float:
         return float(self.lib.rs_list_radius(self.h))
 
+    def copy_calibration(self, nbytes: int, reps: int = 5):
+        """Launch a plain copy kernel of known traffic (nbytes read + nbytes written per launch)."""
+        self._chk(self.lib.rs_copy_calibration(self.h, int(nbytes), int(reps)))
+
     def enable_timing(self, on: bool = True):
         self._chk(self.lib.rs_enable_timing(self.h, int(on)))
 
