@@ -98,6 +98,39 @@ def test_msd(gpu_lib, box216):
     assert rel_err(d, do) < 1e-9
 
 
+@pytest.mark.parametrize("n,box,rc,H", [(216, 1.862, 0.8, 20), (3000, 4.48, 1.0, 20)])
+def test_fp32_gradient_against_fp64_path(gpu_lib, n, box, rc, H):
+    """dL/dθ of the FP32 path against the FP64 path of the library (itself matched to the oracle at
+    1e-8) on a short trajectory, where chaos has not yet amplified rounding: this checks the FP32
+    pair arithmetic itself, which the FP32-mode oracle, sharing the FP32 membership rule, cannot."""
+    import revsim
+
+    wb = revsim.water_box(n, box)
+    g = {}
+    for prec in (FP64, FP32):
+        sim, orc = _pair(prec, n=n, box=box, rc=rc)
+        if prec == FP64:
+            tgt = revsim.perturbed_target(orc.rdf(wb.coords)[0])
+        sim.set_target_rdf(tgt)
+        g[prec] = sim.value_and_grad(wb.coords, wb.velocities, 13, H)[1]
+        sim.close()
+    assert rel_err(g[FP32], g[FP64]) < 1e-5, (g[FP32], g[FP64])
+
+
+def test_fp32_forces_3000_waters(gpu_lib, box3000):
+    """FP32 forces and adjoint terms of the 3,000-water box (cell-list path) against the oracle."""
+    sim, orc = _pair(FP32, n=3000, box=4.48, rc=1.0)
+    rng = np.random.default_rng(2)
+    x = box3000.coords + 0.01 * rng.standard_normal(box3000.coords.size)
+    assert per_atom_rel(sim.forces(x), orc.forces(x)) < 1e-5
+    c = rng.standard_normal(x.size)
+    w = rng.standard_normal(x.size)
+    F, h, v = sim.adjoint_terms(x, c, w)
+    Fo, ho, vo = orc.adjoint_terms(x, c, w)
+    assert per_atom_rel(h, ho) < 1e-5
+    assert rel_err(v, vo) < 1e-5
+
+
 def test_forward_trajectory_fp64(gpu_lib, box216):
     sim, orc = _pair(FP64)
     x, v, _ = sim.forward(box216.coords, box216.velocities, 13, 50)

@@ -69,6 +69,27 @@ def test_forces_are_minus_energy_gradient(jittered216):
         assert abs((Ep - Em) / (2 * h) + F @ d) < 1e-6 * np.linalg.norm(F)
 
 
+@pytest.mark.parametrize("n,box,rc", [(216, 1.862, 0.8), (3000, 4.48, 1.0)])
+def test_fp32_membership_rule_agrees_with_fp64(n, box, rc):
+    """The FP32 pair rule selects the same interacting atom pairs as the FP64 rule up to rounding
+    at the cutoff: forces from the two rules (both evaluated in FP64) agree except on the few atoms
+    of a pair sitting within ~1e-7 nm of r_c. A rule that mishandles periodic images (in a box
+    whose half-width is close to r_c + the molecule size) changes dozens."""
+    import oracle as O
+    import revsim
+
+    wb = revsim.water_box(n, box)
+    rng = np.random.default_rng(1)
+    x = wb.coords + 0.01 * rng.standard_normal(wb.coords.size)
+    F = {}
+    for prec in (0, 1):
+        o = O.Oracle(revsim.make_config(n, box, rc, precision=prec))
+        o.set_theta(revsim.DEFAULT_THETA)
+        F[prec] = o.forces(x)
+    d = np.linalg.norm((F[1] - F[0]).reshape(-1, 3), axis=1)
+    assert np.count_nonzero(d > 1e-9 * np.sqrt(np.mean(F[0] ** 2))) <= 4
+
+
 def test_newton_third_law(jittered216):
     F = _orc().forces(jittered216).reshape(-1, 3)
     assert np.abs(F.sum(0)).max() < 1e-9 * np.abs(F).sum()
