@@ -1,12 +1,12 @@
 // step.hip — streaming kernels of the forward Langevin step and of the reverse-time adjoint.
 //
-// Forward step n (forward_sim loop, water/water.jl:1030-1037), FP64 state, one thread per molecule:
+// Forward step n (forward_sim loop, water/water.jl:1030-1037), FP64 state, one lane per atom:
 //   v += F dt/m; x += v dt/2; v = e v + s ξ_n; x += v dt/2; wrap       e = exp(-γdt), s = √(1-e²)
 // Reverse step (reverse_sim, water/water.jl:1496-1521), fused as [part 2 of step n+1][part 1 of n]:
 //   part 2: v -= F dt/m; B = e B + dt² h; A += B   (dL/dθ is accumulated by the pair kernel)
 //   part 1: x -= v dt/2; v = (v - s ξ_n)/e; x -= v dt/2; wrap; D = e D + dt² h (1+e)/2; C += D
 // ξ_n is regenerated from the Philox counter (step n, atom) — the same bits forward and reverse.
-// Every thread also writes its atom's slot of the sorted pair inputs for the next pair kernel.
+// Every lane also writes its atom's slot of the sorted pair inputs for the next pair kernel.
 // Expressions follow the reference's broadcast order and are evaluated without contraction.
 #include "internal.h"
 #include "bonded.h"
@@ -58,25 +58,52 @@ __device__ __forceinline__ void atom_noise(const DevBufs& b, const SysParams& p,
   }
 }
 
-// Forward step n, one thread per molecule; then the bonded forces at the new positions, which
-// seed F for the next pair kernel (it adds the pair forces).
-__global__ __launch_bounds__(MOL_BLOCK) void k_step_fwd(SysParams p, int precision, DevBufs b,
-                                                        int step, uint64_t seed, double dt,
-                                                        double e, double s, double kT) {
-#pragma clang fp contract(off)
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= p.M) return;
-  double xm[9];
-  bool fin = true;
-  const int rk = b.rank[m];
+// The step kernels run one lane per atom: a 64-lane block holds ATOM_MOLS molecules in lanes
+// 3*mol + atom (lane 63 idles), so the FP64 state streams per atom, and the three lanes of a
+// molecule exchange coordinates by shuffles and evaluate its bonded terms (redundantly, in step).
+constexpr int ATOM_MOLS = 21;
+
+struct AtomLane {
+  int mol, a, atom, base;  // molecule, atom within it (0 = O), atom index, first lane of molecule
+  bool act;
+};
+
+__device__ __forceinline__ AtomLane atom_lane(const SysParams& p) {
+  AtomLane L;
+  const int l = threadIdx.x;
+  const int ml = l < 3 * ATOM_MOLS ? l / 3 : ATOM_MOLS - 1;
+  L.a = l - 3 * ml;
+  L.mol = blockIdx.x * ATOM_MOLS + ml;
+  L.act = l < 3 * ATOM_MOLS && L.mol < p.M;
+  L.atom = 3 * L.mol + (L.a < 3 ? L.a : 0);
+  L.base = 3 * ml;
+  return L;
+}
+
+// The molecule's nine values (O, H1, H2) of a per-atom 3-vector held by its three lanes.
+__device__ __forceinline__ void gather_mol(const AtomLane& L, const double v[3], double out[9]) {
 #pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    const double mass = a == 0 ? p.mO : p.mH;
+  for (int b = 0; b < 3; ++b)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) out[3 * b + k] = __shfl(v[k], L.base + b, RS_WAVE);
+}
+
+// Forward step n; then the bonded forces at the new positions, which seed F for the next pair
+// kernel (it adds the pair forces).
+__global__ __launch_bounds__(RS_WAVE) void k_step_fwd(SysParams p, int precision, DevBufs b,
+                                                      int step, uint64_t seed, double dt,
+                                                      double e, double s, double kT) {
+#pragma clang fp contract(off)
+  const AtomLane L = atom_lane(p);
+  double xa[3] = {0.0, 0.0, 0.0};
+  if (L.act) {
+    const double mass = L.a == 0 ? p.mO : p.mH;
     double xi[3];
-    atom_noise(b, p, step, seed, 3 * m + a, mass, kT, xi);
+    atom_noise(b, p, step, seed, L.atom, mass, kT, xi);
+    bool fin = true;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      const size_t o = 9 * (size_t)m + 3 * a + k;
+      const size_t o = 3 * (size_t)L.atom + k;
       double v = b.v[o] + (b.F[o] * dt) / mass;
       double x = b.x[o] + (v * dt) / 2.0;
       v = v * e + xi[k] * s;
@@ -84,46 +111,45 @@ __global__ __launch_bounds__(MOL_BLOCK) void k_step_fwd(SysParams p, int precisi
       x = wrap1(x, p.L[k]);
       b.v[o] = v;
       b.x[o] = x;
-      xm[3 * a + k] = x;
+      xa[k] = x;
       fin = fin && isfinite(x) && isfinite(v);
     }
-    write_sorted(p, precision, 3 * rk + a, xm + 3 * a, b.posq, b.pos64);
+    if (!fin) atomicOr(&b.flags[FLAG_NONFINITE], 1);
   }
-  if (!fin) atomicOr(&b.flags[FLAG_NONFINITE], 1);
-  double F[9], h[9], vj[3];
-  bonded_mol<false>(p, xm, nullptr, nullptr, F, h, vj, b.flags, b.O64[rk]);
+  double xm[9];
+  gather_mol(L, xa, xm);
+  if (L.act) {
+    const int rk = b.rank[L.mol];
+    write_sorted(p, precision, 3 * rk + L.a, xa, b.posq, b.pos64);
+    double F[9], h[9], vj[3];
+    bonded_mol<false>(p, xm, nullptr, nullptr, F, h, vj, b.flags, b.O64[rk]);
 #pragma unroll
-  for (int t = 0; t < 9; ++t) b.F[9 * (size_t)m + t] = F[t];
+    for (int k = 0; k < 3; ++k) b.F[3 * (size_t)L.atom + k] = F[3 * L.a + k];
+  }
 }
 
-// Reverse step, one thread per molecule: [part 2 of step n+1][part 1 of step n]; after part 1 the
-// bonded forces, HVP and θ-VJP at the reconstructed positions seed F, h and vjb for the adjoint
-// pair kernel of step n.
-__global__ __launch_bounds__(MOL_BLOCK) void k_step_rev(SysParams p, int precision, DevBufs b,
-                                                        int do_r2, int do_r1, int step,
-                                                        uint64_t seed, double dt, double e,
-                                                        double s, double kT, int nslots, int clip,
-                                                        int mlp) {
+// Reverse step: [part 2 of step n+1][part 1 of step n]; after part 1 the bonded forces, HVP and
+// θ-VJP at the reconstructed positions seed F, h and vjb for the adjoint pair kernel of step n.
+__global__ __launch_bounds__(RS_WAVE) void k_step_rev(SysParams p, int precision, DevBufs b,
+                                                      int do_r2, int do_r1, int step,
+                                                      uint64_t seed, double dt, double e,
+                                                      double s, double kT, int nslots, int clip,
+                                                      int mlp) {
 #pragma clang fp contract(off)
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  const AtomLane L = atom_lane(p);
   const double dt2 = dt * dt;
   // MLP scale slots alternate by step parity: this step's pair kernel reads slot (step & 1)
   if (mlp && do_r1 && blockIdx.x == 0 && threadIdx.x == 0) b.scal[6 + ((step + 1) & 1)] = 0.0;
-  double wv = 0.0;  // max |w| on this molecule's O (MLP scale)
-  if (m < p.M) {
-  const double cs = clip ? b.scal[4] : 1.0;
-  const int rk = b.rank[m];
-  double xm[9], um[9], wm[9];
-  bool fin = true;
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    const double mass = a == 0 ? p.mO : p.mH;
+  double xa[3] = {0.0, 0.0, 0.0}, ua[3] = {0.0, 0.0, 0.0}, wa[3] = {0.0, 0.0, 0.0};
+  const double mass = L.a == 0 ? p.mO : p.mH;
+  if (L.act) {
+    const double cs = clip ? b.scal[4] : 1.0;
     double xi[3] = {0.0, 0.0, 0.0};
-    if (do_r1) atom_noise(b, p, step, seed, 3 * m + a, mass, kT, xi);
+    if (do_r1) atom_noise(b, p, step, seed, L.atom, mass, kT, xi);
+    bool fin = true;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      const int t = 3 * a + k;
-      const size_t o = 9 * (size_t)m + t;
+      const size_t o = 3 * (size_t)L.atom + k;
       double x = b.x[o], v = b.v[o], h = b.h[o], A = b.A[o];
       if (clip) h = h * cs;
       if (do_r2) {
@@ -142,39 +168,45 @@ __global__ __launch_bounds__(MOL_BLOCK) void k_step_rev(SysParams p, int precisi
         double C = b.C[o] + D;
         b.D[o] = D;
         b.C[o] = C;
-        um[t] = C / mass;
-        wm[t] = A * (dt2 / (2.0 * mass));
+        ua[k] = C / mass;
+        wa[k] = A * (dt2 / (2.0 * mass));
       } else if (clip) {
         b.h[o] = h;
       }
       b.x[o] = x;
       b.v[o] = v;
-      xm[t] = x;
+      xa[k] = x;
       fin = fin && isfinite(x) && isfinite(v);
     }
+    if (!fin) atomicOr(&b.flags[FLAG_NONFINITE], 1);
   }
-  if (!fin) atomicOr(&b.flags[FLAG_NONFINITE], 1);
+  double wv = 0.0;  // max |w| on this molecule's O (MLP scale)
   if (do_r1) {
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      write_sorted(p, precision, 3 * rk + a, xm + 3 * a, b.posq, b.pos64);
+    double xm[9], um[9], wm[9];
+    gather_mol(L, xa, xm);
+    gather_mol(L, ua, um);
+    gather_mol(L, wa, wm);
+    if (L.act) {
+      const int rk = b.rank[L.mol];
+      write_sorted(p, precision, 3 * rk + L.a, xa, b.posq, b.pos64);
       if (precision == RS_PRECISION_FP32)
-        b.uf[3 * rk + a] = make_float4((float)um[3 * a], (float)um[3 * a + 1], (float)um[3 * a + 2], 0.f);
+        b.uf[3 * rk + L.a] = make_float4((float)ua[0], (float)ua[1], (float)ua[2], 0.f);
       else
-        b.ud[3 * rk + a] = make_double4(um[3 * a], um[3 * a + 1], um[3 * a + 2], 0.0);
-    }
-    wv = fmax(fabs(wm[0]), fmax(fabs(wm[1]), fabs(wm[2])));
-    double F[9], h[9], vj[3];
-    bonded_mol<true>(p, xm, um, wm, F, h, vj, b.flags, b.O64[rk]);
+        b.ud[3 * rk + L.a] = make_double4(ua[0], ua[1], ua[2], 0.0);
+      wv = fmax(fabs(wm[0]), fmax(fabs(wm[1]), fabs(wm[2])));
+      double F[9], h[9], vj[3];
+      bonded_mol<true>(p, xm, um, wm, F, h, vj, b.flags, b.O64[rk]);
 #pragma unroll
-    for (int t = 0; t < 9; ++t) {
-      b.F[9 * (size_t)m + t] = F[t];
-      b.h[9 * (size_t)m + t] = h[t];
+      for (int k = 0; k < 3; ++k) {
+        b.F[3 * (size_t)L.atom + k] = F[3 * L.a + k];
+        b.h[3 * (size_t)L.atom + k] = h[3 * L.a + k];
+      }
+      if (L.a == 0) {
+        b.vjb[3 * (size_t)L.mol] = vj[0];
+        b.vjb[3 * (size_t)L.mol + 1] = vj[1];
+        b.vjb[3 * (size_t)L.mol + 2] = vj[2];
+      }
     }
-    b.vjb[3 * (size_t)m] = vj[0];
-    b.vjb[3 * (size_t)m + 1] = vj[1];
-    b.vjb[3 * (size_t)m + 2] = vj[2];
-  }
   }
   if (mlp && do_r1) wmax_fold(wv, b.scal + 6 + (step & 1));
 }
@@ -282,8 +314,8 @@ void launch_wrap_state(const Launch& L) {
 
 void launch_step_fwd(const Launch& L, int step, uint64_t seed, double dt, double e, double s,
                      double kT) {
-  k_step_fwd<<<(L.p.M + MOL_BLOCK - 1) / MOL_BLOCK, MOL_BLOCK, 0, L.s>>>(L.p, L.precision, L.b, step,
-                                                                         seed, dt, e, s, kT);
+  k_step_fwd<<<(L.p.M + ATOM_MOLS - 1) / ATOM_MOLS, RS_WAVE, 0, L.s>>>(L.p, L.precision, L.b, step,
+                                                                       seed, dt, e, s, kT);
 }
 
 void launch_bonded(const Launch& L, bool adjoint, const double* csrc, double cO, double cH,
@@ -297,7 +329,7 @@ void launch_bonded(const Launch& L, bool adjoint, const double* csrc, double cO,
 
 void launch_step_rev(const Launch& L, bool do_r2, bool do_r1, int step, uint64_t seed, double dt,
                      double e, double s, double kT, int nslots, bool clip) {
-  k_step_rev<<<(L.p.M + MOL_BLOCK - 1) / MOL_BLOCK, MOL_BLOCK, 0, L.s>>>(
+  k_step_rev<<<(L.p.M + ATOM_MOLS - 1) / ATOM_MOLS, RS_WAVE, 0, L.s>>>(
       L.p, L.precision, L.b, do_r2, do_r1, step, seed, dt, e, s, kT, nslots, clip, L.use_mlp);
 }
 
