@@ -120,13 +120,25 @@ void set_params(rs_ctx* c) {
   c->rlist = rc + 2.0 * p.hext + g.skin;
   p.rlist2 = c->rlist * c->rlist;
   p.skin_half2 = 0.25 * g.skin * g.skin;
+  // The FP32 list screen errs by < 1e-6 relative in r^2 (quantized positions, two roundings per
+  // component), so a 1e-5 band around r_list^2 leaves every decision to the FP64 rule's outcome.
+  p.rlo2f = (float)(p.rlist2 * (1.0 - 1e-5));
+  p.rhi2f = (float)(p.rlist2 * (1.0 + 1e-5));
+  // cells of at least r_list/2 with a 5^3 stencil where the box allows it, else r_list and 3^3,
+  // else all pairs
   p.brute = 0;
   p.ncell = 1;
-  for (int k = 0; k < 3; ++k) {
-    p.nc[k] = (int)std::floor(g.box[k] / c->rlist);
-    if (p.nc[k] < 3) p.brute = 1;
+  for (p.cs = 2; p.cs >= 1; --p.cs) {
+    bool ok = true;
+    for (int k = 0; k < 3; ++k) {
+      p.nc[k] = (int)std::floor(g.box[k] * p.cs / c->rlist);
+      if (p.nc[k] < 2 * p.cs + 1) ok = false;
+    }
+    if (ok) break;
   }
-  if (p.brute) {
+  if (p.cs < 1) {
+    p.brute = 1;
+    p.cs = 1;
     p.nc[0] = p.nc[1] = p.nc[2] = 1;
   } else {
     p.ncell = p.nc[0] * p.nc[1] * p.nc[2];
@@ -176,7 +188,7 @@ int alloc_all(rs_ctx* c) {
       (r = dalloc(c, &b.tmp, M)) || (r = dalloc(c, &b.order, M)) || (r = dalloc(c, &b.rank, M)))
     return r;
   if ((r = dalloc(c, &b.posq, N)) || (r = dalloc(c, &b.pos64, N)) || (r = dalloc(c, &b.uf, N)) ||
-      (r = dalloc(c, &b.ud, N)) || (r = dalloc(c, &b.O64, M)))
+      (r = dalloc(c, &b.ud, N)) || (r = dalloc(c, &b.O64, M)) || (r = dalloc(c, &b.Oq, M)))
     return r;
   if ((r = dalloc(c, &b.ncnt, lanes)) || (r = dalloc(c, &b.flags, FLAG_COUNT)) ||
       (r = dalloc(c, &b.slots, (size_t)pair_blocks(p) * 8)) || (r = dalloc(c, &b.dth, 8)) ||
@@ -656,7 +668,7 @@ int rs_destroy(rs_ctx* c) {
   DevBufs& b = c->L.b;
   void* ptrs[] = {b.x, b.v, b.F, b.h, b.A, b.B, b.C, b.D, b.grad, b.cell, b.cell_count,
                   b.cell_start, b.cell_fill, b.tmp, b.order, b.rank, b.posq, b.pos64, b.uf,
-                  b.ud, b.O64, b.nbr, b.nbr_tmp, b.ncnt, b.flags, b.slots, b.vjb, b.dth, b.scal, b.hist, b.rdfG,
+                  b.ud, b.O64, b.Oq, b.nbr, b.nbr_tmp, b.ncnt, b.flags, b.slots, b.vjb, b.dth, b.scal, b.hist, b.rdfG,
                   b.target, b.rdf_g, b.frames, b.msd_del, b.msd_acc, b.msd_grad, b.ext_noise,
                   b.hpart, b.mlp_w, b.mlp_wb, b.mlp_tab, b.mlp_slot, b.mlp_G, b.mlp_part, b.mlp_dW};
   for (void* q : ptrs)

@@ -35,6 +35,8 @@ struct SysParams {
   // list
   double rlist2, skin_half2, hext;
   int nc[3], ncell, brute;
+  int cs;          // cell stencil half-width: cells are >= r_list / cs wide, (2 cs + 1)^3 stencil
+  float rlo2f, rhi2f;  // FP32 screen of the list rule: accept below, reject above, else FP64
   int CH, capc;
 };
 

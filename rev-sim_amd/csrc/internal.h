@@ -16,6 +16,7 @@ struct DevBufs {
   float4* uf;      // FP32 mode: accum_C / m, 3 per molecule
   double4* ud;     // FP64 mode: accum_C / m
   double4* O64;    // O position at the last list build, sorted
+  int4* Oq;        // the same, quantized (2^32 per box edge), for the list build's FP32 screen
   // molecule list, ELL by wave: nbr[((gtid/64)*capc + k)*64 + gtid%64], gtid = i*CH + c
   int *nbr, *ncnt;
   int* nbr_tmp;    // list-build scratch, same layout as nbr (per-lane discovery order)

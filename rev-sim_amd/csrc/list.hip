@@ -126,7 +126,7 @@ __global__ void k_cells_order(SysParams p, const int* start, const int* tmp, int
 // Sorted copies of the pair inputs for the current permutation.
 __global__ void k_pack(SysParams p, int precision, const int* __restrict__ order,
                        const double* __restrict__ x, const double* __restrict__ C, int4* posq,
-                       double4* pos64, float4* uf, double4* ud, double4* O64) {
+                       double4* pos64, float4* uf, double4* ud, double4* O64, int4* Oq) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;  // sorted atom slot 3*i + a
   if (t >= p.N) return;
   int i = t / 3, a = t - 3 * i;
@@ -139,7 +139,11 @@ __global__ void k_pack(SysParams p, int precision, const int* __restrict__ order
   } else {
     pos64[t] = make_double4(x0, x1, x2, 0.0);
   }
-  if (a == 0) O64[i] = make_double4(x0, x1, x2, 0.0);
+  if (a == 0) {
+    O64[i] = make_double4(x0, x1, x2, 0.0);
+    Oq[i] = make_int4((int)quant1(x0, p.qscale[0]), (int)quant1(x1, p.qscale[1]),
+                      (int)quant1(x2, p.qscale[2]), 0);
+  }
   if (C) {
     double im = 1.0 / (a == 0 ? p.mO : p.mH);
     const double* ca = C + 9 * m + 3 * a;
@@ -152,9 +156,10 @@ __global__ void k_pack(SysParams p, int precision, const int* __restrict__ order
 
 template <int CH>
 __global__ __launch_bounds__(RS_BLOCK) void k_list_build(SysParams p, const int* __restrict__ start,
-                                                         const double4* __restrict__ O64, int* tmp,
+                                                         const double4* __restrict__ O64,
+                                                         const int4* __restrict__ Oq, int* tmp,
                                                          int* nbr, int* ncnt, int* flags) {
-  // Pass 1: lane c tests every CH-th candidate of the 27-cell stencil (or of all molecules) and
+  // Pass 1: lane c tests every CH-th candidate of the cell stencil (or of all molecules) and
   // keeps the accepted ones in discovery order in its slot column of tmp.
   const int gtid = blockIdx.x * RS_BLOCK + threadIdx.x;
   const int i = gtid / CH, c = gtid % CH;
@@ -165,13 +170,23 @@ __global__ __launch_bounds__(RS_BLOCK) void k_list_build(SysParams p, const int*
   int n = 0;
   if (act) {
     const double4 oi = O64[i];
+    const int4 qi = Oq[i];
     auto test = [&](int j) {
       if (j == i) return;
-      double4 oj = O64[j];
-      double dx = mic1(oj.x - oi.x, p.L[0], p.invL[0]);
-      double dy = mic1(oj.y - oi.y, p.L[1], p.invL[1]);
-      double dz = mic1(oj.z - oi.z, p.L[2], p.invL[2]);
-      if (r2_of(dx, dy, dz) <= p.rlist2) {
+      // FP32 screen on quantized positions; the FP64 rule decides inside the uncertainty band
+      const int4 qj = Oq[j];
+      const float r2f = r2f_of(qdiff((uint32_t)qi.x, (uint32_t)qj.x, p.unitf[0]),
+                               qdiff((uint32_t)qi.y, (uint32_t)qj.y, p.unitf[1]),
+                               qdiff((uint32_t)qi.z, (uint32_t)qj.z, p.unitf[2]));
+      bool in = r2f < p.rlo2f;
+      if (!in && r2f <= p.rhi2f) {
+        const double4 oj = O64[j];
+        const double dx = mic1(oj.x - oi.x, p.L[0], p.invL[0]);
+        const double dy = mic1(oj.y - oi.y, p.L[1], p.invL[1]);
+        const double dz = mic1(oj.z - oi.z, p.L[2], p.invL[2]);
+        in = r2_of(dx, dy, dz) <= p.rlist2;
+      }
+      if (in) {
         if (n < p.capc) tout[(size_t)n * RS_WAVE] = j;
         ++n;
       }
@@ -182,9 +197,10 @@ __global__ __launch_bounds__(RS_BLOCK) void k_list_build(SysParams p, const int*
       int ci = cell_index(p, oi.x, oi.y, oi.z);
       int cx = ci % p.nc[0], cy = (ci / p.nc[0]) % p.nc[1], cz = ci / (p.nc[0] * p.nc[1]);
       int off = 0;
-      for (int dz = -1; dz <= 1; ++dz)
-        for (int dy = -1; dy <= 1; ++dy)
-          for (int dx = -1; dx <= 1; ++dx) {
+      const int S = p.cs;
+      for (int dz = -S; dz <= S; ++dz)
+        for (int dy = -S; dy <= S; ++dy)
+          for (int dx = -S; dx <= S; ++dx) {
             int ex = cx + dx, ey = cy + dy, ez = cz + dz;
             ex += (ex < 0) ? p.nc[0] : (ex >= p.nc[0] ? -p.nc[0] : 0);
             ey += (ey < 0) ? p.nc[1] : (ey >= p.nc[1] ? -p.nc[1] : 0);
@@ -239,12 +255,12 @@ void launch_list_build(const Launch& L, bool with_u) {
   k_cells_order<<<gw, RS_BLOCK, 0, L.s>>>(p, b.cell_start, b.tmp, b.order, b.rank);
   int ga = (p.N + RS_BLOCK - 1) / RS_BLOCK;
   k_pack<<<ga, RS_BLOCK, 0, L.s>>>(p, L.precision, b.order, b.x, with_u ? b.C : nullptr, b.posq,
-                                    b.pos64, b.uf, b.ud, b.O64);
+                                    b.pos64, b.uf, b.ud, b.O64, b.Oq);
   int gl = pair_blocks(p);
   switch (p.CH) {
-    case 4: k_list_build<4><<<gl, RS_BLOCK, 0, L.s>>>(p, b.cell_start, b.O64, b.nbr_tmp, b.nbr, b.ncnt, b.flags); break;
-    case 8: k_list_build<8><<<gl, RS_BLOCK, 0, L.s>>>(p, b.cell_start, b.O64, b.nbr_tmp, b.nbr, b.ncnt, b.flags); break;
-    case 16: k_list_build<16><<<gl, RS_BLOCK, 0, L.s>>>(p, b.cell_start, b.O64, b.nbr_tmp, b.nbr, b.ncnt, b.flags); break;
-    default: k_list_build<32><<<gl, RS_BLOCK, 0, L.s>>>(p, b.cell_start, b.O64, b.nbr_tmp, b.nbr, b.ncnt, b.flags); break;
+    case 4: k_list_build<4><<<gl, RS_BLOCK, 0, L.s>>>(p, b.cell_start, b.O64, b.Oq, b.nbr_tmp, b.nbr, b.ncnt, b.flags); break;
+    case 8: k_list_build<8><<<gl, RS_BLOCK, 0, L.s>>>(p, b.cell_start, b.O64, b.Oq, b.nbr_tmp, b.nbr, b.ncnt, b.flags); break;
+    case 16: k_list_build<16><<<gl, RS_BLOCK, 0, L.s>>>(p, b.cell_start, b.O64, b.Oq, b.nbr_tmp, b.nbr, b.ncnt, b.flags); break;
+    default: k_list_build<32><<<gl, RS_BLOCK, 0, L.s>>>(p, b.cell_start, b.O64, b.Oq, b.nbr_tmp, b.nbr, b.ncnt, b.flags); break;
   }
 }
