@@ -462,6 +462,7 @@ int reverse_impl(rs_ctx* c, const double* xN, const double* vN, uint64_t seed, i
     if (vN) HIPCHK(c, hipMemcpyAsync(b.v, vN, sizeof(double) * c->N3, hipMemcpyHostToDevice, c->s));
     if ((r = reset_flags(c))) return r;
     HIPCHK(c, hipMemsetAsync(b.dth, 0, sizeof(double) * 8, c->s));
+    HIPCHK(c, hipMemsetAsync(b.slots, 0, sizeof(double) * 8 * (size_t)nslots, c->s));
     HIPCHK(c, hipMemsetAsync(b.scal, 0, sizeof(double) * 16, c->s));
     rebuild(c, false);
     seed_at(c, P, n_steps, true, true, e);
@@ -478,7 +479,6 @@ int reverse_impl(rs_ctx* c, const double* xN, const double* vN, uint64_t seed, i
         tbegin(c, RS_PHASE_REV_PAIR);
         c->L.parity = n & 1;
         launch_pair(c->L, true, b.A, ws[0], ws[1]);
-        launch_vjp_reduce(c->L);
         tend(c);
         if (clip) launch_hnorm(c->L, g.clip_norm);
         pending = true;
@@ -507,6 +507,7 @@ int reverse_impl(rs_ctx* c, const double* xN, const double* vN, uint64_t seed, i
       }
     }
     if (pending) launch_step_rev(c->L, true, false, 1, seed, dt, e, s, kT, nslots, clip);
+    launch_vjp_reduce(c->L);  // dL/dθ = Σ over pair blocks of their per-trajectory slots
     int why = 0;
     if ((r = finish(c, &why))) return r;
     if (why) {
@@ -851,6 +852,7 @@ int rs_adjoint_terms(rs_ctx* c, const double* x, const double* cc, const double*
   if ((r = single_eval(c, x, true, [&] {
          c->L.parity = 0;
          hipMemsetAsync(b.scal + 6, 0, sizeof(double) * 2, c->s);
+         hipMemsetAsync(b.slots, 0, sizeof(double) * 8 * (size_t)pair_blocks(c->L.p), c->s);
          if (c->L.use_mlp)
            hipMemsetAsync(b.mlp_slot, 0, sizeof(double) * (size_t)pair_blocks(c->L.p) * MLP_NG, c->s);
          launch_bonded(c->L, true, b.C, cs[0], cs[1], b.A, 1.0, 1.0);

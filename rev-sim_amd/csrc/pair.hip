@@ -320,7 +320,7 @@ __global__ __launch_bounds__(RS_BLOCK) void k_pair(PairArgs A) {
     if (threadIdx.x < 6) {
       double s = 0.0;
       for (int q = 0; q < RS_BLOCK / CH; ++q) s += vjs[q][threadIdx.x];
-      A.slots[(size_t)blockIdx.x * 8 + threadIdx.x] = s;
+      A.slots[(size_t)blockIdx.x * 8 + threadIdx.x] += s;  // this block's slot, summed over steps
     }
     if (A.mlp) {
       // this block's table gradient of the step, accumulated over the trajectory in its own slot
