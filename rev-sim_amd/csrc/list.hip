@@ -213,6 +213,8 @@ __global__ __launch_bounds__(RS_BLOCK) void k_list_build(SysParams p, const int*
           }
     }
   }
+  // pass 2 reads entries other lanes of the wave wrote in pass 1
+  __threadfence_block();
   // Pass 2: the molecule's accepted candidates in (lane, discovery) order get consecutive global
   // ranks g = (exclusive scan of the lane counts) + k and are dealt round-robin: rank g goes to
   // sub-list g % CH, slot g / CH. Sub-lists then differ in length by at most one, so the lanes of a
@@ -224,18 +226,32 @@ __global__ __launch_bounds__(RS_BLOCK) void k_list_build(SysParams p, const int*
     if (c >= o) incl += t;
   }
   const int T = __shfl(incl, CH - 1, CH);
-  const int first = incl - n;
   const bool ovf = n > p.capc || T > CH * p.capc;
+  // Written slot row by slot row so that every store is a full coalesced row: the lane filling
+  // rank g = slot*CH + c finds the discovering lane d (the last with inclusive count <= g) among
+  // the molecule's CH prefix sums and reads its entry g - first_d from the scratch.
+  int incl_of[CH];
+#pragma unroll
+  for (int d = 0; d < CH; ++d) incl_of[d] = __shfl(incl, d, CH);
+  int mine = (T - c + CH - 1) / CH;
+  mine = mine < p.capc ? mine : p.capc;
+  if (act && !ovf) {
+    const int* src0 = tmp + wbase + (lane - c);
+    for (int slot = 0; slot < mine; ++slot) {
+      const int g = slot * CH + c;
+      int d = 0, before = 0;
+#pragma unroll
+      for (int q = 0; q < CH - 1; ++q)
+        if (incl_of[q] <= g) {
+          d = q + 1;
+          before = incl_of[q];
+        }
+      nbr[wbase + (size_t)slot * RS_WAVE + lane] = src0[(size_t)(g - before) * RS_WAVE + d];
+    }
+  }
   if (act) {
     if (ovf) atomicOr(&flags[FLAG_LIST_OVERFLOW], 1);
-    const int nn = n < p.capc ? n : p.capc;
-    for (int k = 0; k < nn; ++k) {
-      const int g = first + k;
-      const int slot = g / CH;
-      if (slot < p.capc) nbr[wbase + (size_t)slot * RS_WAVE + (lane - c + g % CH)] = tout[(size_t)k * RS_WAVE];
-    }
-    const int mine = (T - c + CH - 1) / CH;
-    ncnt[gtid] = mine < p.capc ? mine : p.capc;
+    ncnt[gtid] = mine;
   }
 }
 
