@@ -187,7 +187,7 @@ int alloc_all(rs_ctx* c) {
       (r = dalloc(c, &b.cell_start, p.ncell + 1)) || (r = dalloc(c, &b.cell_fill, p.ncell + 1)) ||
       (r = dalloc(c, &b.tmp, M)) || (r = dalloc(c, &b.order, M)) || (r = dalloc(c, &b.rank, M)))
     return r;
-  if ((r = dalloc(c, &b.posq, N)) || (r = dalloc(c, &b.pos64, N)) || (r = dalloc(c, &b.uf, N)) ||
+  if ((r = dalloc(c, &b.posq, 3 * N)) || (r = dalloc(c, &b.pos64, N)) || (r = dalloc(c, &b.uf, 3 * N)) ||
       (r = dalloc(c, &b.ud, N)) || (r = dalloc(c, &b.O64, M)) || (r = dalloc(c, &b.Oq, M)))
     return r;
   if ((r = dalloc(c, &b.ncnt, lanes)) || (r = dalloc(c, &b.flags, FLAG_COUNT)) ||

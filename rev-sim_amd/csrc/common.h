@@ -156,6 +156,11 @@ __device__ __forceinline__ void normals3(uint64_t seed, int step, int atom, doub
   n[2] = rad2 * cos(twopi * u4);
 }
 
+// Three consecutive 32-bit values (one atom of the packed FP32 pair inputs: 9 values per
+// molecule) as a 4-vector; the fourth component is unused.
+__device__ __forceinline__ int4 ld3i(const int* p) { return make_int4(p[0], p[1], p[2], 0); }
+__device__ __forceinline__ float4 ld3f(const float* p) { return make_float4(p[0], p[1], p[2], 0.f); }
+
 // Wave-level reductions over lanes that share a molecule (CH consecutive lanes).
 template <int CH>
 __device__ __forceinline__ float chunk_sum(float v) {

@@ -11,9 +11,9 @@ struct DevBufs {
   // cell sort
   int *cell, *cell_count, *cell_start, *cell_fill, *tmp, *order, *rank;
   // sorted pair inputs
-  int4* posq;      // FP32 mode: quantized positions, 3 per molecule
+  int* posq;       // FP32 mode: quantized positions, 9 ints per molecule (O, H1, H2)
   double4* pos64;  // FP64 mode: positions, 3 per molecule
-  float4* uf;      // FP32 mode: accum_C / m, 3 per molecule
+  float* uf;       // FP32 mode: accum_C / m, 9 floats per molecule
   double4* ud;     // FP64 mode: accum_C / m
   double4* O64;    // O position at the last list build, sorted
   int4* Oq;        // the same, quantized (2^32 per box edge), for the list build's FP32 screen

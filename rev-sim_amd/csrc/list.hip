@@ -125,8 +125,8 @@ __global__ void k_cells_order(SysParams p, const int* start, const int* tmp, int
 
 // Sorted copies of the pair inputs for the current permutation.
 __global__ void k_pack(SysParams p, int precision, const int* __restrict__ order,
-                       const double* __restrict__ x, const double* __restrict__ C, int4* posq,
-                       double4* pos64, float4* uf, double4* ud, double4* O64, int4* Oq) {
+                       const double* __restrict__ x, const double* __restrict__ C, int* posq,
+                       double4* pos64, float* uf, double4* ud, double4* O64, int4* Oq) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;  // sorted atom slot 3*i + a
   if (t >= p.N) return;
   int i = t / 3, a = t - 3 * i;
@@ -134,8 +134,9 @@ __global__ void k_pack(SysParams p, int precision, const int* __restrict__ order
   const double* xa = x + 9 * m + 3 * a;
   double x0 = xa[0], x1 = xa[1], x2 = xa[2];
   if (precision == RS_PRECISION_FP32) {
-    posq[t] = make_int4((int)quant1(x0, p.qscale[0]), (int)quant1(x1, p.qscale[1]),
-                        (int)quant1(x2, p.qscale[2]), 0);
+    posq[3 * t] = (int)quant1(x0, p.qscale[0]);
+    posq[3 * t + 1] = (int)quant1(x1, p.qscale[1]);
+    posq[3 * t + 2] = (int)quant1(x2, p.qscale[2]);
   } else {
     pos64[t] = make_double4(x0, x1, x2, 0.0);
   }
@@ -148,7 +149,7 @@ __global__ void k_pack(SysParams p, int precision, const int* __restrict__ order
     double im = 1.0 / (a == 0 ? p.mO : p.mH);
     const double* ca = C + 9 * m + 3 * a;
     if (precision == RS_PRECISION_FP32)
-      uf[t] = make_float4((float)(ca[0] * im), (float)(ca[1] * im), (float)(ca[2] * im), 0.f);
+      for (int k = 0; k < 3; ++k) uf[3 * t + k] = (float)(ca[k] * im);
     else
       ud[t] = make_double4(ca[0] * im, ca[1] * im, ca[2] * im, 0.0);
   }

@@ -83,7 +83,7 @@ template <int CH>
 __global__ __launch_bounds__(RS_BLOCK) void k_rdf_hist(SysParams p, int precision, RdfArgs r,
                                                         const int* __restrict__ nbr,
                                                         const int* __restrict__ ncnt,
-                                                        const int4* __restrict__ posq,
+                                                        const int* __restrict__ posq,
                                                         const double4* __restrict__ pos64,
                                                         long long* hist) {
   __shared__ unsigned long long lh[RDF_MAX_BINS];
@@ -98,7 +98,7 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rdf_hist(SysParams p, int precisio
       const int j = L[(size_t)k * RS_WAVE];
       if (j <= i) continue;
       if (precision == RS_PRECISION_FP32) {
-        const int4 a = posq[3 * i], b = posq[3 * j];
+        const int4 a = ld3i(posq + 9 * (size_t)i), b = ld3i(posq + 9 * (size_t)j);
         float dx = qdiff((uint32_t)a.x, (uint32_t)b.x, p.unitf[0]);
         float dy = qdiff((uint32_t)a.y, (uint32_t)b.y, p.unitf[1]);
         float dz = qdiff((uint32_t)a.z, (uint32_t)b.z, p.unitf[2]);
@@ -147,7 +147,7 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rdf_grad(SysParams p, int precisio
                                                         const int* __restrict__ ncnt,
                                                         const int* __restrict__ order,
                                                         const double* __restrict__ x,
-                                                        const int4* __restrict__ posq,
+                                                        const int* __restrict__ posq,
                                                         const double* __restrict__ G,
                                                         double* grad) {
   const int gtid = blockIdx.x * RS_BLOCK + threadIdx.x;
@@ -169,7 +169,7 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rdf_grad(SysParams p, int precisio
       if (rr > rmaxw + 0.01) continue;
       int lo, hi;
       if (precision == RS_PRECISION_FP32) {
-        const int4 a = posq[3 * i], b = posq[3 * j];
+        const int4 a = ld3i(posq + 9 * (size_t)i), b = ld3i(posq + 9 * (size_t)j);
         float ex = qdiff((uint32_t)a.x, (uint32_t)b.x, p.unitf[0]);
         float ey = qdiff((uint32_t)a.y, (uint32_t)b.y, p.unitf[1]);
         float ez = qdiff((uint32_t)a.z, (uint32_t)b.z, p.unitf[2]);

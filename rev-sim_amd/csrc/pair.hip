@@ -62,6 +62,9 @@ struct P32 {
   typedef int4 Pos;
   typedef float4 U;
   typedef float R;
+  // packed inputs: 9 ints / 9 floats per molecule
+  static __device__ __forceinline__ Pos ld_pos(const void* p, int slot) { return ld3i((const int*)p + 3 * slot); }
+  static __device__ __forceinline__ U ld_u(const void* p, int slot) { return ld3f((const float*)p + 3 * slot); }
   static __device__ __forceinline__ void diff(const SysParams& p, const int4& a, const int4& b,
                                               float& dx, float& dy, float& dz) {
     dx = qdiff((uint32_t)a.x, (uint32_t)b.x, p.unitf[0]);
@@ -77,6 +80,8 @@ struct P64 {
   typedef double4 Pos;
   typedef double4 U;
   typedef double R;
+  static __device__ __forceinline__ Pos ld_pos(const void* p, int slot) { return ((const double4*)p)[slot]; }
+  static __device__ __forceinline__ U ld_u(const void* p, int slot) { return ((const double4*)p)[slot]; }
   static __device__ __forceinline__ void diff(const SysParams& p, const double4& a, const double4& b,
                                               double& dx, double& dy, double& dz) {
     dx = mic1(b.x - a.x, p.L[0], p.invL[0]);
@@ -99,15 +104,13 @@ __global__ __launch_bounds__(RS_BLOCK) void k_pair(PairArgs A) {
   const int gtid = blockIdx.x * RS_BLOCK + threadIdx.x;
   const int i = gtid / CH, c = gtid % CH;
   const bool act = i < p.M;
-  const Pos* __restrict__ pos = (const Pos*)A.pos;
-  const U* __restrict__ uu = (const U*)A.u;
 
   Pos own[3];
   R uo[3][3];
   R F[3][3], hh[3][3], Flj[3], Fs[3], Fml[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    own[a] = act ? pos[3 * i + a] : pos[0];
+    own[a] = P::ld_pos(A.pos, act ? 3 * i + a : 0);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       F[a][k] = 0;
@@ -117,7 +120,7 @@ __global__ __launch_bounds__(RS_BLOCK) void k_pair(PairArgs A) {
     Fs[a] = 0;
     Fml[a] = 0;
     if (ADJ) {
-      U t = act ? uu[3 * i + a] : uu[0];
+      U t = P::ld_u(A.u, act ? 3 * i + a : 0);
       uo[a][0] = t.x;
       uo[a][1] = t.y;
       uo[a][2] = t.z;
@@ -154,9 +157,9 @@ __global__ __launch_bounds__(RS_BLOCK) void k_pair(PairArgs A) {
     R uj[3][3];
 #pragma unroll
     for (int b = 0; b < 3; ++b) {
-      pj[b] = pos[3 * j + b];
+      pj[b] = P::ld_pos(A.pos, 3 * j + b);
       if (ADJ) {
-        U t = uu[3 * j + b];
+        U t = P::ld_u(A.u, 3 * j + b);
         uj[b][0] = t.x;
         uj[b][1] = t.y;
         uj[b][2] = t.z;

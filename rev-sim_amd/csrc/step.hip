@@ -26,10 +26,9 @@ __device__ __forceinline__ void wmax_fold(double v, double* slot) {
 }
 
 __device__ __forceinline__ void write_sorted(const SysParams& p, int precision, int slot,
-                                             const double x[3], int4* posq, double4* pos64) {
+                                             const double x[3], int* posq, double4* pos64) {
   if (precision == RS_PRECISION_FP32)
-    posq[slot] = make_int4((int)quant1(x[0], p.qscale[0]), (int)quant1(x[1], p.qscale[1]),
-                           (int)quant1(x[2], p.qscale[2]), 0);
+    for (int k = 0; k < 3; ++k) posq[3 * slot + k] = (int)quant1(x[k], p.qscale[k]);
   else
     pos64[slot] = make_double4(x[0], x[1], x[2], 0.0);
 }
@@ -190,7 +189,7 @@ __global__ __launch_bounds__(RS_WAVE) void k_step_rev(SysParams p, int precision
       const int rk = b.rank[L.mol];
       write_sorted(p, precision, 3 * rk + L.a, xa, b.posq, b.pos64);
       if (precision == RS_PRECISION_FP32)
-        b.uf[3 * rk + L.a] = make_float4((float)ua[0], (float)ua[1], (float)ua[2], 0.f);
+        for (int k = 0; k < 3; ++k) b.uf[3 * (3 * rk + L.a) + k] = (float)ua[k];
       else
         b.ud[3 * rk + L.a] = make_double4(ua[0], ua[1], ua[2], 0.0);
       wv = fmax(fabs(wm[0]), fmax(fabs(wm[1]), fabs(wm[2])));
