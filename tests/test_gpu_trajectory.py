@@ -1,0 +1,151 @@
+"""GPU parity of the trajectory-level features of the reverse pass against the FP64 oracle: loss
+blocks with truncation and snapshot restore (water/water.jl:1491-1547), the MSD loss injected at
+intermediate frames (diffusion/diffusion.jl:1534-1539), gradient clipping (water/water.jl:1510-1515),
+replay of caller-supplied noise, reversibility of the FP32 path (the reference's FP32 bound,
+water/water.jl:1613), and the 3,000-water box's trajectory gradient against the oracle and against
+central finite differences of the trajectory loss (FP64 path, smooth RDF window)."""
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+FP64, FP32 = 0, 1
+
+
+def _pair(prec, n=216, box=1.862, rc=0.8, **kw):
+    import oracle as O
+    import revsim
+
+    sim = revsim.Simulator(revsim.make_config(n, box, rc, precision=prec, **kw), device=0)
+    sim.set_theta(revsim.DEFAULT_THETA)
+    orc = O.Oracle(revsim.make_config(n, box, rc, precision=prec, **kw))
+    orc.set_theta(revsim.DEFAULT_THETA)
+    return sim, orc
+
+
+def _target(orc, x):
+    import revsim
+
+    return revsim.perturbed_target(orc.rdf(x)[0])
+
+
+@pytest.mark.parametrize("prec,tol", [(FP64, 1e-8), (FP32, 1e-4)])
+def test_blocks_truncation_snapshots(gpu_lib, box216, prec, tol):
+    """Three 20-step loss blocks, gradients truncated after 10 reverse steps inside each block,
+    forward snapshots every 20 steps restored in reverse."""
+    sim, orc = _pair(prec, n_steps_loss=20, n_steps_trunc=10, n_steps_log=20)
+    tgt = _target(orc, box216.coords)
+    sim.set_target_rdf(tgt)
+    orc.set_target_rdf(tgt)
+    L, g = sim.value_and_grad(box216.coords, box216.velocities, 13, 60)
+    Lo, go = orc.value_and_grad(box216.coords, box216.velocities, 13, 60)
+    assert abs(L - Lo) <= tol * abs(Lo)
+    assert rel_err(g, go) < tol, (g, go)
+
+
+@pytest.mark.parametrize("prec,tol", [(FP64, 1e-8), (FP32, 1e-4)])
+def test_msd_loss_gradient(gpu_lib, box216, prec, tol):
+    """RDF + MSD loss: the MSD term's dL/dx at each stored frame is injected into the reverse
+    accumulators when the reverse pass reaches the frame."""
+    sim, orc = _pair(prec, msd_window_steps=20, msd_every=5, w_msd=1e-3)
+    tgt = _target(orc, box216.coords)
+    sim.set_target_rdf(tgt)
+    orc.set_target_rdf(tgt)
+    L, g = sim.value_and_grad(box216.coords, box216.velocities, 13, 40)
+    Lo, go = orc.value_and_grad(box216.coords, box216.velocities, 13, 40)
+    assert abs(L - Lo) <= tol * abs(Lo)
+    assert rel_err(g, go) < tol, (g, go)
+
+
+def test_gradient_clipping(gpu_lib, box216):
+    """A clip threshold below the HVP norm rescales h every reverse step (FP64 path)."""
+    sim, orc = _pair(FP64, clip_norm=50.0)
+    tgt = _target(orc, box216.coords)
+    sim.set_target_rdf(tgt)
+    orc.set_target_rdf(tgt)
+    L, g = sim.value_and_grad(box216.coords, box216.velocities, 13, 30)
+    Lo, go = orc.value_and_grad(box216.coords, box216.velocities, 13, 30)
+    sim2, _ = _pair(FP64)
+    sim2.set_target_rdf(tgt)
+    _, g_unclipped = sim2.value_and_grad(box216.coords, box216.velocities, 13, 30)
+    assert rel_err(g, go) < 1e-8, (g, go)
+    assert rel_err(g, g_unclipped) > 1e-3  # the threshold is active
+
+
+def test_external_noise_replay(gpu_lib, box216):
+    """Caller-supplied noise (e.g. the reference's Xoshiro draws) replaces the counter RNG in the
+    forward and the reverse pass."""
+    sim, orc = _pair(FP64)
+    n = 25
+    rng = np.random.default_rng(21)
+    m = np.repeat(box216.masses(), 3)
+    xi = rng.standard_normal((n, m.size)) * np.sqrt(0.00831446262 * 300.0 / m)
+    sim.set_external_noise(xi.reshape(-1), n)
+    orc.set_external_noise(xi.reshape(-1), n)
+    x, v, _ = sim.forward(box216.coords, box216.velocities, 13, n)
+    xo, vo, _ = orc.forward(box216.coords, box216.velocities, 13, n)
+    d = x - xo
+    d -= 1.862 * np.round(d / 1.862)
+    assert np.max(np.abs(d)) < 1e-10 and rel_err(v, vo) < 1e-10
+    tgt = _target(orc, box216.coords)
+    sim.set_target_rdf(tgt)
+    orc.set_target_rdf(tgt)
+    L, g = sim.value_and_grad(box216.coords, box216.velocities, 13, n)
+    Lo, go = orc.value_and_grad(box216.coords, box216.velocities, 13, n)
+    assert rel_err(g, go) < 1e-8
+    # the seed no longer matters
+    _, g2 = sim.value_and_grad(box216.coords, box216.velocities, 99, n)
+    assert np.array_equal(g, g2)
+
+
+def test_fp32_reversibility(gpu_lib, box3000):
+    """water/water.jl:1613: after 100 + 100 steps the FP32-force path recovers x0 within 1e-4 nm
+    (the FP64 state keeps it far below that)."""
+    import revsim
+
+    sim = revsim.Simulator(revsim.make_config(3000, 4.48, 1.0, precision=FP32), device=0)
+    sim.set_theta(revsim.DEFAULT_THETA)
+    xN, vN, lf = sim.forward(box3000.coords, box3000.velocities, 13, 100)
+    lr, g, x0, v0 = sim.reverse(xN, vN, 13, 100)
+    d = x0 - np.mod(box3000.coords, 4.48)
+    d -= 4.48 * np.round(d / 4.48)
+    assert np.abs(d).max() < 1e-4
+    assert abs(lr - lf) < 1e-7 * max(1.0, abs(lf))
+
+
+def test_trajectory_gradient_3000_waters(gpu_lib, box3000):
+    """The 3,000-water box (cell-list path): dL/dθ of the FP64 path on a 30-step trajectory equals
+    the oracle's (1e-8), and agrees with central differences of the same trajectory's loss (same
+    noise; smooth 0.1 nm RDF window) for σ, ε and q_H. The reaction-field force is nonzero at r_c
+    (as in the reference), so pairs crossing the cutoff make the loss discontinuous and bias the
+    differences by an amount that does not shrink with the step: the FD bound is 5e-3."""
+    import oracle as O
+    import revsim
+
+    cfg = revsim.make_config(3000, 4.48, 1.0, precision=FP64, rdf_window=0.1)
+    sim = revsim.Simulator(cfg, device=0)
+    sim.set_theta(revsim.DEFAULT_THETA)
+    g0 = sim.rdf(box3000.coords)[0]
+    tgt = revsim.perturbed_target(g0)
+    sim.set_target_rdf(tgt)
+    L, g = sim.value_and_grad(box3000.coords, box3000.velocities, 13, 30)
+    orc = O.Oracle(revsim.make_config(3000, 4.48, 1.0, precision=FP64, rdf_window=0.1))
+    orc.set_theta(revsim.DEFAULT_THETA)
+    orc.set_target_rdf(tgt)
+    Lo, go = orc.value_and_grad(box3000.coords, box3000.velocities, 13, 30)
+    assert abs(L - Lo) <= 1e-8 * abs(Lo)
+    assert rel_err(g, go) < 1e-8, (g, go)
+    for k in (0, 1, 2):
+        th = revsim.DEFAULT_THETA.copy()
+        hk = 1e-6 * th[k]
+        th[k] += hk
+        sim.set_theta(th)
+        lp = sim.forward(box3000.coords, box3000.velocities, 13, 30)[2]
+        th[k] -= 2 * hk
+        sim.set_theta(th)
+        lm = sim.forward(box3000.coords, box3000.velocities, 13, 30)[2]
+        sim.set_theta(revsim.DEFAULT_THETA)
+        fd = (lp - lm) / (2 * hk)
+        assert abs(g[k] - fd) < 5e-3 * abs(fd), (k, g[k], fd)
