@@ -31,6 +31,9 @@ PEAK_HBM = 8000.0  # GB/s
 # algorithmic FLOP per pair within r_c (add/sub/mul 1, FMA 2, sqrt/rcp/exp 1): base, extra for O-O LJ
 FLOP_FWD = (36, 15)
 FLOP_REV = (72, 33)
+# HBM bytes per launch of the adjoint pair kernel at this workload: FETCH_SIZE x 2.000 + WRITE_SIZE
+# from rocprofv3 counter runs, scaled by the library's copy kernel (profiles/pair_kernels_water32768.md)
+TRAFFIC_REV_PAIR = 85.9e6
 
 
 def log(*a):
@@ -219,7 +222,8 @@ def main():
         "unit": "TFLOP/s",
         "frac": round(achieved / PEAK_FP32, 4),
         "frac_of_nonpacked_peak": round(achieved / PEAK_FP32_NONPACKED, 4),
-        "traffic": None,
+        "traffic": TRAFFIC_REV_PAIR if a.workload == "water32768" and prec == revsim.RS_PRECISION_FP32 else None,
+        "traffic_unit": "bytes per launch",
         "avg_launch_ms": round(avg_ms, 5),
         "flop_per_launch": flop,
         "pairs_within_rc": n_pairs,
