@@ -74,6 +74,7 @@ struct P32 {
   static __device__ __forceinline__ float r2(float dx, float dy, float dz) { return r2f_of(dx, dy, dz); }
   static __device__ __forceinline__ float rc2(const SysParams& p) { return p.rc2f; }
   static __device__ __forceinline__ float rinv(float r2) { return __builtin_amdgcn_rsqf(r2); }
+  static __device__ __forceinline__ float fma_(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 };
 
 struct P64 {
@@ -91,6 +92,7 @@ struct P64 {
   static __device__ __forceinline__ double r2(double dx, double dy, double dz) { return r2_of(dx, dy, dz); }
   static __device__ __forceinline__ double rc2(const SysParams& p) { return p.rc2; }
   static __device__ __forceinline__ double rinv(double r2) { return 1.0 / sqrt(r2); }
+  static __device__ __forceinline__ double fma_(double a, double b, double c) { return __builtin_fma(a, b, c); }
 };
 
 // ------------------------------------------------------------------ the pair kernel
@@ -127,8 +129,12 @@ __global__ __launch_bounds__(RS_BLOCK) void k_pair(PairArgs A) {
     }
   }
   const R rc2 = P::rc2(p);
+  // per pair type (O-O, O-H, H-H): k q_a q_b, k q_a q_b 2 k_rf, -3 k q_a q_b. The forward kernel
+  // keeps the two-instruction form of f/r: the folded constants' registers would cost it a wave.
   const R krf2 = (R)(2.0 * p.krf);
   const R kq[3] = {(R)p.kqq[0], (R)p.kqq[1], (R)p.kqq[2]};
+  const R kqr[3] = {(R)(p.kqq[0] * 2.0 * p.krf), (R)(p.kqq[1] * 2.0 * p.krf), (R)(p.kqq[2] * 2.0 * p.krf)};
+  const R kq3[3] = {(R)(-3.0 * p.kqq[0]), (R)(-3.0 * p.kqq[1]), (R)(-3.0 * p.kqq[2])};
   const R sig2 = (R)(p.sig * p.sig), eps24 = (R)(24.0 * p.eps);
   const R epss = (R)(144.0 * p.eps / p.sig);
 
@@ -176,20 +182,18 @@ __global__ __launch_bounds__(RS_BLOCK) void k_pair(PairArgs A) {
         const bool in = r2 <= rc2;
         const R ri = P::rinv(r2);
         const R ri2 = ri * ri, ri3 = ri2 * ri;
-        const R q = kq[(a > 0) + (b > 0)];
-        const R fr = in ? q * (ri3 - krf2) : (R)0;
+        const int ty = (a > 0) + (b > 0);
+        const R fr = in ? (ADJ ? P::fma_(kq[ty], ri3, -kqr[ty]) : kq[ty] * (ri3 - krf2)) : (R)0;
 #pragma unroll
         for (int t = 0; t < 3; ++t) F[a][t] -= fr * d[t];
-        R du[3], dot = 0, g2 = 0;
+        R du[3], dot = 0;
         if (ADJ) {
 #pragma unroll
-          for (int t = 0; t < 3; ++t) {
-            du[t] = uo[a][t] - uj[b][t];
-            dot += d[t] * du[t];
-          }
-          g2 = in ? (R)(-3) * q * ri3 * ri2 : (R)0;
+          for (int t = 0; t < 3; ++t) du[t] = uo[a][t] - uj[b][t];
+          dot = P::fma_(d[2], du[2], P::fma_(d[1], du[1], d[0] * du[0]));
+          const R gd = in ? kq3[ty] * (ri3 * ri2) * dot : (R)0;
 #pragma unroll
-          for (int t = 0; t < 3; ++t) hh[a][t] += fr * du[t] + (g2 * dot) * d[t];
+          for (int t = 0; t < 3; ++t) hh[a][t] = P::fma_(gd, d[t], P::fma_(fr, du[t], hh[a][t]));
         }
         if (a == 0 && b == 0) {
           const R s2 = sig2 * ri2, s6 = s2 * s2 * s2, s12 = s6 * s6;
@@ -197,13 +201,13 @@ __global__ __launch_bounds__(RS_BLOCK) void k_pair(PairArgs A) {
 #pragma unroll
           for (int t = 0; t < 3; ++t) F[0][t] -= frl * d[t];
           if (ADJ) {
-            const R g2l = in ? -eps24 * ri2 * ri2 * ((R)28 * s12 - (R)8 * s6) : (R)0;
+            const R gdl = in ? -eps24 * ri2 * ri2 * ((R)28 * s12 - (R)8 * s6) * dot : (R)0;
             const R fs = in ? epss * ri2 * ((R)4 * s12 - s6) : (R)0;
 #pragma unroll
             for (int t = 0; t < 3; ++t) {
-              hh[0][t] += frl * du[t] + (g2l * dot) * d[t];
-              Flj[t] -= frl * d[t];
-              Fs[t] -= fs * d[t];
+              hh[0][t] = P::fma_(gdl, d[t], P::fma_(frl, du[t], hh[0][t]));
+              Flj[t] = P::fma_(-frl, d[t], Flj[t]);
+              Fs[t] = P::fma_(-fs, d[t], Fs[t]);
             }
           }
           if (A.mlp) {
