@@ -197,21 +197,33 @@ __global__ __launch_bounds__(RS_BLOCK) void k_list_build(SysParams p, const int*
     } else {
       int ci = cell_index(p, oi.x, oi.y, oi.z);
       int cx = ci % p.nc[0], cy = (ci / p.nc[0]) % p.nc[1], cz = ci / (p.nc[0] * p.nc[1]);
+      // The stencil's cells along x are consecutive cell indices, so each (dy, dz) row is one
+      // contiguous range of sorted molecules, or two where it wraps around the box.
       int off = 0;
       const int S = p.cs;
+      auto range = [&](int e0, int e1) {  // cells e0..e1 inclusive
+        const int s = start[e0], cnt = start[e1 + 1] - s;
+        const int t0 = ((c - off) % CH + CH) % CH;
+        for (int t = t0; t < cnt; t += CH) test(s + t);
+        off += cnt;
+      };
       for (int dz = -S; dz <= S; ++dz)
-        for (int dy = -S; dy <= S; ++dy)
-          for (int dx = -S; dx <= S; ++dx) {
-            int ex = cx + dx, ey = cy + dy, ez = cz + dz;
-            ex += (ex < 0) ? p.nc[0] : (ex >= p.nc[0] ? -p.nc[0] : 0);
-            ey += (ey < 0) ? p.nc[1] : (ey >= p.nc[1] ? -p.nc[1] : 0);
-            ez += (ez < 0) ? p.nc[2] : (ez >= p.nc[2] ? -p.nc[2] : 0);
-            int e = (ez * p.nc[1] + ey) * p.nc[0] + ex;
-            int s = start[e], cnt = start[e + 1] - s;
-            int t0 = ((c - off) % CH + CH) % CH;
-            for (int t = t0; t < cnt; t += CH) test(s + t);
-            off += cnt;
+        for (int dy = -S; dy <= S; ++dy) {
+          int ey = cy + dy, ez = cz + dz;
+          ey += (ey < 0) ? p.nc[1] : (ey >= p.nc[1] ? -p.nc[1] : 0);
+          ez += (ez < 0) ? p.nc[2] : (ez >= p.nc[2] ? -p.nc[2] : 0);
+          const int row = (ez * p.nc[1] + ey) * p.nc[0];
+          const int x0 = cx - S, x1 = cx + S;
+          if (x0 < 0) {
+            range(row + x0 + p.nc[0], row + p.nc[0] - 1);
+            range(row, row + x1);
+          } else if (x1 >= p.nc[0]) {
+            range(row + x0, row + p.nc[0] - 1);
+            range(row, row + x1 - p.nc[0]);
+          } else {
+            range(row + x0, row + x1);
           }
+        }
     }
   }
   // pass 2 reads entries other lanes of the wave wrote in pass 1
