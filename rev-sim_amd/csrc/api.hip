@@ -192,7 +192,7 @@ int alloc_all(rs_ctx* c) {
     return r;
   if ((r = dalloc(c, &b.ncnt, lanes)) || (r = dalloc(c, &b.flags, FLAG_COUNT)) ||
       (r = dalloc(c, &b.slots, (size_t)pair_blocks(p) * 8)) || (r = dalloc(c, &b.dth, 8)) ||
-      (r = dalloc(c, &b.scal, 16)) || (r = dalloc(c, &b.hpart, 256)))
+      (r = dalloc(c, &b.scal, 16)) || (r = dalloc(c, &b.wmax, 128)) || (r = dalloc(c, &b.hpart, 256)))
     return r;
   const int nb = std::max(1, c->cfg.rdf_bins);
   if ((r = dalloc(c, &b.hist, nb)) || (r = dalloc(c, &b.rdfG, nb)) || (r = dalloc(c, &b.target, nb)) ||
@@ -464,6 +464,7 @@ int reverse_impl(rs_ctx* c, const double* xN, const double* vN, uint64_t seed, i
     HIPCHK(c, hipMemsetAsync(b.dth, 0, sizeof(double) * 8, c->s));
     HIPCHK(c, hipMemsetAsync(b.slots, 0, sizeof(double) * 8 * (size_t)nslots, c->s));
     HIPCHK(c, hipMemsetAsync(b.scal, 0, sizeof(double) * 16, c->s));
+    HIPCHK(c, hipMemsetAsync(b.wmax, 0, sizeof(double) * 128, c->s));
     rebuild(c, false);
     seed_at(c, P, n_steps, true, true, e);
     int trunc_count = 0;
@@ -669,7 +670,7 @@ int rs_destroy(rs_ctx* c) {
   DevBufs& b = c->L.b;
   void* ptrs[] = {b.x, b.v, b.F, b.h, b.A, b.B, b.C, b.D, b.grad, b.cell, b.cell_count,
                   b.cell_start, b.cell_fill, b.tmp, b.order, b.rank, b.posq, b.pos64, b.uf,
-                  b.ud, b.O64, b.Oq, b.nbr, b.nbr_tmp, b.ncnt, b.flags, b.slots, b.vjb, b.dth, b.scal, b.hist, b.rdfG,
+                  b.ud, b.O64, b.Oq, b.nbr, b.nbr_tmp, b.ncnt, b.flags, b.slots, b.vjb, b.dth, b.scal, b.wmax, b.hist, b.rdfG,
                   b.target, b.rdf_g, b.frames, b.msd_del, b.msd_acc, b.msd_grad, b.ext_noise,
                   b.hpart, b.mlp_w, b.mlp_wb, b.mlp_tab, b.mlp_slot, b.mlp_G, b.mlp_part, b.mlp_dW};
   for (void* q : ptrs)
@@ -851,7 +852,7 @@ int rs_adjoint_terms(rs_ctx* c, const double* x, const double* cc, const double*
   if (c->cfg.use_mlp && !c->mlp_set) return fail(c, RS_E_STATE, "use_mlp is set but rs_set_mlp was not called");
   if ((r = single_eval(c, x, true, [&] {
          c->L.parity = 0;
-         hipMemsetAsync(b.scal + 6, 0, sizeof(double) * 2, c->s);
+         hipMemsetAsync(b.wmax, 0, sizeof(double) * 128, c->s);
          hipMemsetAsync(b.slots, 0, sizeof(double) * 8 * (size_t)pair_blocks(c->L.p), c->s);
          if (c->L.use_mlp)
            hipMemsetAsync(b.mlp_slot, 0, sizeof(double) * (size_t)pair_blocks(c->L.p) * MLP_NG, c->s);

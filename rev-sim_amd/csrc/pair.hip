@@ -45,13 +45,15 @@ struct PairArgs {
   const void* __restrict__ mtab;
   double mrlo, mh, minvh;
   double* mslot;                    // per-block table gradient (adjoint)
-  const double* __restrict__ wmax;  // max|w_O| of this step (uint64 bits of a double)
+  const double* __restrict__ wmax;  // this step's 64 max|w_O| slots (uint64 bits of doubles)
 };
 
 // Fixed-point scale of the per-block table-gradient sums: 2^(50 - e) with max|w_O| < 2^e, so one
 // block's sum (≤ 2^11 pair contributions bounded by max|w_O|) stays below 2^61.
+// Called by every lane of a wave (uniform branch): the wave reduces the 64 slots.
 __device__ __forceinline__ double mlp_scale(const double* wmax) {
-  const double m = *wmax;
+  double m = wmax[threadIdx.x & 63];
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
   if (!(m > 0.0)) return 1125899906842624.0;  // 2^50
   int e;
   frexp(m, &e);
@@ -375,7 +377,7 @@ void launch_pair(const Launch& L, bool adjoint, const double* wsrc, double wscal
   a.mh = (L.mlp_rc - MLP_RLO) / (MLP_NT - 1);
   a.minvh = 1.0 / a.mh;
   a.mslot = L.b.mlp_slot;
-  a.wmax = L.b.scal + 6 + (L.parity & 1);
+  a.wmax = L.b.wmax + 64 * (L.parity & 1);
   a.p = L.p;
   a.nbr = L.b.nbr;
   a.ncnt = L.b.ncnt;

@@ -16,13 +16,14 @@ namespace {
 // Per-molecule kernels run one wave per block so that the few waves there are spread over all CUs.
 constexpr int MOL_BLOCK = 64;
 
-// max over the block's molecules of |w| on O atoms, folded into *slot (a non-negative double kept
-// as its uint64 bit pattern, which orders like the value): the per-step fixed-point scale of the
-// MLP table gradient.
-__device__ __forceinline__ void wmax_fold(double v, double* slot) {
+// max over the block's (one wave's) molecules of |w| on O atoms, folded into one of 64 slots (a
+// non-negative double kept as its uint64 bit pattern, which orders like the value): the per-step
+// fixed-point scale of the MLP table gradient. Spreading the atomics over 64 addresses keeps
+// them from serialising on one.
+__device__ __forceinline__ void wmax_fold(double v, double* slots) {
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
   if ((threadIdx.x & 63) == 0 && v > 0.0)
-    atomicMax((unsigned long long*)slot, (unsigned long long)__double_as_longlong(v));
+    atomicMax((unsigned long long*)&slots[blockIdx.x & 63], (unsigned long long)__double_as_longlong(v));
 }
 
 __device__ __forceinline__ void write_sorted(const SysParams& p, int precision, int slot,
@@ -138,7 +139,7 @@ __global__ __launch_bounds__(RS_WAVE) void k_step_rev(SysParams p, int precision
   const AtomLane L = atom_lane(p);
   const double dt2 = dt * dt;
   // MLP scale slots alternate by step parity: this step's pair kernel reads slot (step & 1)
-  if (mlp && do_r1 && blockIdx.x == 0 && threadIdx.x == 0) b.scal[6 + ((step + 1) & 1)] = 0.0;
+  if (mlp && do_r1 && blockIdx.x == 0) b.wmax[64 * ((step + 1) & 1) + threadIdx.x] = 0.0;
   double xa[3] = {0.0, 0.0, 0.0}, ua[3] = {0.0, 0.0, 0.0}, wa[3] = {0.0, 0.0, 0.0};
   const double mass = L.a == 0 ? p.mO : p.mH;
   if (L.act) {
@@ -207,7 +208,7 @@ __global__ __launch_bounds__(RS_WAVE) void k_step_rev(SysParams p, int precision
       }
     }
   }
-  if (mlp && do_r1) wmax_fold(wv, b.scal + 6 + (step & 1));
+  if (mlp && do_r1) wmax_fold(wv, b.wmax + 64 * (step & 1));
 }
 
 // Bonded terms of the current state alone (start of a trajectory and single evaluations):
@@ -242,7 +243,7 @@ __global__ __launch_bounds__(MOL_BLOCK) void k_bonded(SysParams p, DevBufs b, co
     b.vjb[3 * (size_t)m + 2] = vj[2];
   }
   }
-  if (ADJ && mlp) wmax_fold(wv, b.scal + 6);
+  if (ADJ && mlp) wmax_fold(wv, b.wmax);
 }
 
 // Gradient seed g (b.grad) into the accumulators: reset (A = B = g(1+e), C = 0,
