@@ -28,9 +28,10 @@
  * Pair membership (which pairs interact) is defined exactly so that the GPU library and this
  * oracle agree bit for bit:
  *   FP64 rule  dx = x_b - x_a; dx -= L*rint(dx*(1/L)); r2 = fma(dz,dz,fma(dy,dy,dx*dx)); r2 <= rc*rc
- *   FP32 rule  q = (uint32)llrint(x * 2^32/L) per dimension; dxi = (int32)(q_b - q_a);
- *              dxf = (float)dxi * (float)(L/2^32); r2f = fmaf(dzf,dzf,fmaf(dyf,dyf,dxf*dxf));
- *              r2f <= (float)(rc*rc)
+ *   FP32 rule  (cubic box) q = (uint32)llrint(x * 2^32/L) per dimension; dxi = (int32)(q_b - q_a);
+ *              dxq = (float)dxi; r2q = fmaf(dzq,dzq,fmaf(dyq,dyq,dxq*dxq));
+ *              r2q <= (float)(rc*rc*qscale*qscale), qscale = 2^32/L (lengths in units of L/2^32)
+ * The RDF bins use FP64 distances in both precisions.
  * Compile with -ffp-contract=off (oracle/Makefile) so these expressions are evaluated as written.
  */
 #include "../include/revsim.h"
@@ -60,6 +61,7 @@ typedef struct or_ctx {
   double theta[NTH];
   double krf, rc2;
   float rc2f;
+  float rc2q; /* FP32 pair rule threshold in units of (L/2^32)^2 */
   double* mass3; /* per degree of freedom */
   /* atom-pair half list (a<b, different molecules), FP64 mic distance <= rlist */
   double rlist;
@@ -116,20 +118,19 @@ static inline double r2_of(const double dr[3]) {
 
 static inline uint32_t quant1(double x, double s) { return (uint32_t)(int64_t)llrint(x * s); }
 
-/* FP32 membership rule on quantized positions; also returns the FP32 r2. */
-static inline float r2f_quant(const or_ctx* o, const double* xa, const double* xb) {
+/* FP32 pair membership in quantized units (no scaling of the integer difference). */
+static inline float r2q_quant(const or_ctx* o, const double* xa, const double* xb) {
   float d[3];
   for (int k = 0; k < 3; ++k) {
     uint32_t qa = quant1(xa[k], o->qscale[k]);
     uint32_t qb = quant1(xb[k], o->qscale[k]);
-    int32_t di = (int32_t)(qb - qa);
-    d[k] = (float)di * o->unitf[k];
+    d[k] = (float)(int32_t)(qb - qa);
   }
   return fmaf(d[2], d[2], fmaf(d[1], d[1], d[0] * d[0]));
 }
 
 static inline int pair_in(const or_ctx* o, const double* xa, const double* xb, double r2) {
-  if (o->cfg.precision == RS_PRECISION_FP32) return r2f_quant(o, xa, xb) <= o->rc2f;
+  if (o->cfg.precision == RS_PRECISION_FP32) return r2q_quant(o, xa, xb) <= o->rc2q;
   return r2 <= o->rc2;
 }
 
@@ -193,6 +194,8 @@ int or_destroy(or_ctx* o) {
 int or_create(const rs_config* cfg, or_ctx** out) {
   *out = NULL;
   if (!cfg || cfg->abi_version != RS_ABI_VERSION || cfg->n_molecules < 2) return RS_E_CONFIG;
+  if (cfg->precision == RS_PRECISION_FP32 && !(cfg->box[0] == cfg->box[1] && cfg->box[1] == cfg->box[2]))
+    return RS_E_CONFIG;
   or_ctx* o = (or_ctx*)calloc(1, sizeof(or_ctx));
   o->cfg = *cfg;
   o->M = cfg->n_molecules;
@@ -212,6 +215,7 @@ int or_create(const rs_config* cfg, or_ctx** out) {
   o->krf = (1.0 / (rc * rc * rc)) * ((cfg->eps_rf - 1.0) / (2.0 * cfg->eps_rf + 1.0));
   o->rc2 = rc * rc;
   o->rc2f = (float)(rc * rc);
+  o->rc2q = (float)(rc * rc * o->qscale[0] * o->qscale[0]);
   o->mass3 = (double*)malloc(sizeof(double) * o->N3);
   for (int a = 0; a < o->N; ++a)
     for (int k = 0; k < 3; ++k) o->mass3[3 * a + k] = (a % 3 == 0) ? cfg->mass_O : cfg->mass_H;
@@ -942,21 +946,6 @@ int or_noise(or_ctx* o, uint64_t seed, int32_t step, double* xi) {
 /* exp(t) for t <= 0 by Cody-Waite reduction t = k ln2 + f (fdlibm's split of ln2) and a Taylor
    polynomial in f evaluated with fused multiply-adds; exact scaling by 2^k. The GPU library
    evaluates the same sequence so that smoothed-histogram contributions agree bit for bit. */
-static inline float rs_expf(float t) {
-  float k = rintf(t * 1.44269504088896341f);
-  float f = fmaf(-k, 0.693145751953125f, t);
-  f = fmaf(-k, 1.428606765330187045e-06f, f);
-  float p = 1.0f / 5040.0f;
-  p = fmaf(p, f, 1.0f / 720.0f);
-  p = fmaf(p, f, 1.0f / 120.0f);
-  p = fmaf(p, f, 1.0f / 24.0f);
-  p = fmaf(p, f, 1.0f / 6.0f);
-  p = fmaf(p, f, 0.5f);
-  p = fmaf(p, f, 1.0f);
-  p = fmaf(p, f, 1.0f);
-  return ldexpf(p, (int)k);
-}
-
 static inline double rs_exp(double t) {
   double k = rint(t * 1.4426950408889634);
   double f = fma(-k, 6.93147180369123816490e-01, t);
@@ -980,7 +969,6 @@ static inline double rs_exp(double t) {
   return ldexp(p, (int)k);
 }
 
-float or_expf(float t) { return rs_expf(t); }
 double or_exp(double t) { return rs_exp(t); }
 
 /* ---------------------------------------------------------------- RDF loss */
@@ -992,45 +980,28 @@ typedef struct {
 /* Smoothed-histogram contributions of one O-O distance (rdf_differentiable,
    water/water.jl:631-691, re-binned per the config: bin centres x_b = (b+1/2)Δ, Δ = rmax/bins,
    window |x_b - r| <= window, weights exp(-(x_b-r)^2/width) normalised over the window).
-   Adds llrint(φ_b · 2^32) into hist. Returns the window. */
-static win_t rdf_contrib(const or_ctx* o, double r64, float r32, int64_t* hist) {
+   Adds llrint(φ_b · 2^32) into hist. Returns the window. FP64 in both precisions: the narrow
+   smoothing makes φ_b about 160/nm sensitive to r, so FP32 distances would put ~1e-5 relative
+   noise into the histogram. */
+static win_t rdf_contrib(const or_ctx* o, double r64, int64_t* hist) {
   const int nb = o->cfg.rdf_bins;
   win_t w = {0, -1};
-  if (o->cfg.precision == RS_PRECISION_FP32) {
-    float invd = (float)(nb / o->cfg.rdf_rmax), c = (float)o->cfg.rdf_window;
-    float dx = (float)(o->cfg.rdf_rmax / nb), iw = (float)(1.0 / o->cfg.rdf_width);
-    int lo = (int)ceilf((r32 - c) * invd - 0.5f), hi = (int)floorf((r32 + c) * invd - 0.5f);
-    if (lo < 0) lo = 0;
-    if (hi > nb - 1) hi = nb - 1;
-    w.lo = lo;
-    w.hi = hi;
-    if (lo > hi) return w;
-    float e[512];
-    float den = 0.0f;
-    for (int b = lo; b <= hi; ++b) {
-      float t = ((float)b + 0.5f) * dx - r32;
-      e[b - lo] = rs_expf(-(t * t) * iw);
-      den += e[b - lo];
-    }
-    for (int b = lo; b <= hi; ++b) hist[b] += (int64_t)llrintf((e[b - lo] / den) * 4294967296.0f);
-  } else {
-    double invd = nb / o->cfg.rdf_rmax, c = o->cfg.rdf_window;
-    double dx = o->cfg.rdf_rmax / nb, iw = 1.0 / o->cfg.rdf_width;
-    int lo = (int)ceil((r64 - c) * invd - 0.5), hi = (int)floor((r64 + c) * invd - 0.5);
-    if (lo < 0) lo = 0;
-    if (hi > nb - 1) hi = nb - 1;
-    w.lo = lo;
-    w.hi = hi;
-    if (lo > hi) return w;
-    double e[512];
-    double den = 0.0;
-    for (int b = lo; b <= hi; ++b) {
-      double t = ((double)b + 0.5) * dx - r64;
-      e[b - lo] = rs_exp(-(t * t) * iw);
-      den += e[b - lo];
-    }
-    for (int b = lo; b <= hi; ++b) hist[b] += (int64_t)llrint((e[b - lo] / den) * 4294967296.0);
+  double invd = nb / o->cfg.rdf_rmax, c = o->cfg.rdf_window;
+  double dx = o->cfg.rdf_rmax / nb, iw = 1.0 / o->cfg.rdf_width;
+  int lo = (int)ceil((r64 - c) * invd - 0.5), hi = (int)floor((r64 + c) * invd - 0.5);
+  if (lo < 0) lo = 0;
+  if (hi > nb - 1) hi = nb - 1;
+  w.lo = lo;
+  w.hi = hi;
+  if (lo > hi) return w;
+  double e[512];
+  double den = 0.0;
+  for (int b = lo; b <= hi; ++b) {
+    double t = ((double)b + 0.5) * dx - r64;
+    e[b - lo] = rs_exp(-(t * t) * iw);
+    den += e[b - lo];
   }
+  for (int b = lo; b <= hi; ++b) hist[b] += (int64_t)llrint((e[b - lo] / den) * 4294967296.0);
   return w;
 }
 
@@ -1062,9 +1033,8 @@ int or_rdf(or_ctx* o, const double* x, double* g, int64_t* hist_out, double* los
       double dr[3];
       mic3(o, x + 3 * a, x + 3 * b, dr);
       double r64 = sqrt(r2_of(dr));
-      float r32 = sqrtf(r2f_quant(o, x + 3 * a, x + 3 * b));
       if (r64 > rmaxw + 0.01) continue;
-      rdf_contrib(o, r64, r32, ht);
+      rdf_contrib(o, r64, ht);
     }
   }
   int64_t* H = (int64_t*)calloc(nb, sizeof(int64_t));
@@ -1096,14 +1066,9 @@ int or_rdf(or_ctx* o, const double* x, double* g, int64_t* hist_out, double* los
       mic3(o, x + 3 * a, x + 3 * b, dr);
       double r = sqrt(r2_of(dr));
       if (r > rmaxw + 0.01) continue;
-      float r32 = sqrtf(r2f_quant(o, x + 3 * a, x + 3 * b));
       win_t w;
       /* window exactly as used for the histogram */
-      if (o->cfg.precision == RS_PRECISION_FP32) {
-        float invd = (float)(nb / o->cfg.rdf_rmax), c = (float)o->cfg.rdf_window;
-        w.lo = (int)ceilf((r32 - c) * invd - 0.5f);
-        w.hi = (int)floorf((r32 + c) * invd - 0.5f);
-      } else {
+      {
         double invd = nb / o->cfg.rdf_rmax, c = o->cfg.rdf_window;
         w.lo = (int)ceil((r - c) * invd - 0.5);
         w.hi = (int)floor((r + c) * invd - 0.5);

@@ -49,7 +49,6 @@ _SIGS = [
     ("or_adjoint_terms_mlp", ctypes.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
     ("or_mlp_grad", ctypes.c_int, [_vp, _dp]),
     ("or_num_threads", ctypes.c_int, []),
-    ("or_expf", ctypes.c_float, [ctypes.c_float]),
     ("or_exp", ctypes.c_double, [ctypes.c_double]),
     ("or_philox4x64", None, [_P(ctypes.c_uint64), _P(ctypes.c_uint64), _P(ctypes.c_uint64)]),
 ]

@@ -97,6 +97,7 @@ void set_params(rs_ctx* c) {
   const double rc = g.cutoff;
   p.rc2 = rc * rc;
   p.rc2f = (float)(rc * rc);
+  p.rc2q = (float)(rc * rc * p.qscale[0] * p.qscale[0]);
   p.krf = (1.0 / (rc * rc * rc)) * ((g.eps_rf - 1.0) / (2.0 * g.eps_rf + 1.0));
   p.krff = (float)p.krf;
   p.kc = g.coulomb_k;
@@ -616,6 +617,8 @@ int rs_create(const rs_config* cfg, int32_t device, rs_ctx** out) {
     return fail(nullptr, RS_E_CONFIG, "cutoff, dt, gamma must be > 0, nl_every >= 1, skin >= 0");
   if (g.precision != RS_PRECISION_FP32 && g.precision != RS_PRECISION_FP64)
     return fail(nullptr, RS_E_CONFIG, "precision must be RS_PRECISION_FP64 or RS_PRECISION_FP32");
+  if (g.precision == RS_PRECISION_FP32 && !(g.box[0] == g.box[1] && g.box[1] == g.box[2]))
+    return fail(nullptr, RS_E_CONFIG, "RS_PRECISION_FP32 needs a cubic box (pair arithmetic in units of L/2^32)");
   if (g.rdf_bins < 1 || g.rdf_bins > 1024 || !(g.rdf_rmax > 0.0) || !(g.rdf_width > 0.0) ||
       !(g.rdf_window > 0.0))
     return fail(nullptr, RS_E_CONFIG, "rdf_bins in [1,1024] and positive rdf_rmax/width/window required");

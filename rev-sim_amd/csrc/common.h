@@ -26,6 +26,7 @@ struct SysParams {
   float unitf[3];
   double rc2, krf, kc;
   float rc2f, krff;
+  float rc2q;      // FP32 pair rule threshold in quantized units: (float)(rc*rc*qscale*qscale)
   // θ-derived
   double sig, eps, qH, qO, kb, r0, ka, theta0;
   double kqq[3];   // k*qO*qO, k*qO*qH, k*qH*qH
@@ -45,8 +46,12 @@ struct SysParams {
 // interact" and are evaluated exactly as written (no contraction) so that the CPU oracle,
 // compiled with -ffp-contract=off, makes identical decisions.
 //   FP64: dx = x_b - x_a; dx -= L*rint(dx*(1/L)); r2 = fma(dz,dz,fma(dy,dy,dx*dx))
-//   FP32: dxi = (int32)(q_b - q_a) on uint32 positions q = llrint(x*2^32/L);
-//         dxf = (float)dxi * (float)(L/2^32); r2f = fmaf(dzf,dzf,fmaf(dyf,dyf,dxf*dxf))
+//   FP32: dxi = (int32)(q_b - q_a) on uint32 positions q = llrint(x*2^32/L) (cubic box);
+//         dxq = (float)dxi; r2q = fmaf(dzq,dzq,fmaf(dyq,dyq,dxq*dxq)) <= rc2q, in quantized units
+//         (one length unit = L/2^32), rc2q = (float)(rc*rc*qscale*qscale). The pair kernels work
+//         in these units throughout and fold L/2^32 into their constants. (float)(-dxi) is
+//         -(float)dxi, so both sides of a pair decide alike.
+//   The RDF bins and the list build's FP32 screen use physical FP32 differences, qdiff() below.
 // ----------------------------------------------------------------------------------------------
 __device__ __forceinline__ double mic1(double d, double L, double invL) {
 #pragma clang fp contract(off)
@@ -68,6 +73,8 @@ __device__ __forceinline__ float qdiff(uint32_t qa, uint32_t qb, float unit) {
   return (float)(int32_t)(qb - qa) * unit;
 }
 
+__device__ __forceinline__ float qdiffq(uint32_t qa, uint32_t qb) { return (float)(int32_t)(qb - qa); }
+
 __device__ __forceinline__ uint32_t quant1(double x, double s) {
 #pragma clang fp contract(off)
   return (uint32_t)(int64_t)rint(x * s);
@@ -83,22 +90,6 @@ __device__ __forceinline__ double wrap1(double x, double L) {
 // polynomial in f by fused multiply-adds, exact scaling by 2^k. Used for the RDF smoothing
 // weights so that histogram contributions are reproducible bit for bit on the host.
 // ----------------------------------------------------------------------------------------------
-__device__ __forceinline__ float rs_expf(float t) {
-#pragma clang fp contract(off)
-  float k = rintf(t * 1.44269504088896341f);
-  float f = fmaf(-k, 0.693145751953125f, t);
-  f = fmaf(-k, 1.428606765330187045e-06f, f);
-  float p = 1.0f / 5040.0f;
-  p = fmaf(p, f, 1.0f / 720.0f);
-  p = fmaf(p, f, 1.0f / 120.0f);
-  p = fmaf(p, f, 1.0f / 24.0f);
-  p = fmaf(p, f, 1.0f / 6.0f);
-  p = fmaf(p, f, 0.5f);
-  p = fmaf(p, f, 1.0f);
-  p = fmaf(p, f, 1.0f);
-  return ldexpf(p, (int)k);
-}
-
 __device__ __forceinline__ double rs_exp(double t) {
 #pragma clang fp contract(off)
   double k = rint(t * 1.4426950408889634);

@@ -7,6 +7,8 @@
 // pre = V/(4π N_O^2 Δ); L = Σ_b (g_b - target_b)^2.
 // Bin sums are 64-bit fixed point (2^-32 per unit weight), accumulated with LDS and global integer
 // atomics: order-independent and identical to the oracle bit for bit (same r, same exp sequence).
+// Distances are FP64 in both precisions (the FP64 state): the narrow smoothing makes φ_b about
+// 160/nm sensitive to r, and FP32 distances put ~1e-5 relative noise into the histogram.
 //
 // MSD (calculate_D / loss_fn_final, diffusion/diffusion.jl:861-911) on the O atoms of 2K frames:
 // D = (1/6K) Σ_s MSD(s, s+K)/(K Δt_frame) (Einstein relation at lag K frames, K origins) with
@@ -21,15 +23,7 @@ struct RdfArgs {
   double rmax, width, window, wgt, pre;
 };
 
-// Window and per-bin weights for one distance, FP32 rule (bit-identical with the oracle).
-__device__ __forceinline__ void window32(const RdfArgs& r, float rf, int& lo, int& hi) {
-#pragma clang fp contract(off)
-  const float invd = (float)(r.nb / r.rmax), c = (float)r.window;
-  lo = (int)ceilf((rf - c) * invd - 0.5f);
-  hi = (int)floorf((rf + c) * invd - 0.5f);
-  if (lo < 0) lo = 0;
-  if (hi > r.nb - 1) hi = r.nb - 1;
-}
+// Window of bins within ±window of one distance.
 __device__ __forceinline__ void window64(const RdfArgs& r, double rd, int& lo, int& hi) {
 #pragma clang fp contract(off)
   const double invd = r.nb / r.rmax, c = r.window;
@@ -37,25 +31,6 @@ __device__ __forceinline__ void window64(const RdfArgs& r, double rd, int& lo, i
   hi = (int)floor((rd + c) * invd - 0.5);
   if (lo < 0) lo = 0;
   if (hi > r.nb - 1) hi = r.nb - 1;
-}
-
-__device__ __forceinline__ void contrib32(const RdfArgs& r, float rf, unsigned long long* lh) {
-#pragma clang fp contract(off)
-  int lo, hi;
-  window32(r, rf, lo, hi);
-  if (lo > hi) return;
-  const float dx = (float)(r.rmax / r.nb), iw = (float)(1.0 / r.width);
-  float den = 0.0f;
-  for (int b = lo; b <= hi; ++b) {
-    float t = ((float)b + 0.5f) * dx - rf;
-    den += rs_expf(-(t * t) * iw);
-  }
-  for (int b = lo; b <= hi; ++b) {
-    float t = ((float)b + 0.5f) * dx - rf;
-    float e = rs_expf(-(t * t) * iw);
-    long long v = __float2ll_rn((e / den) * 4294967296.0f);
-    atomicAdd(&lh[b], (unsigned long long)v);
-  }
 }
 
 __device__ __forceinline__ void contrib64(const RdfArgs& r, double rd, unsigned long long* lh) {
@@ -80,11 +55,11 @@ __device__ __forceinline__ void contrib64(const RdfArgs& r, double rd, unsigned 
 #define RDF_MAX_BINS 1024
 
 template <int CH>
-__global__ __launch_bounds__(RS_BLOCK) void k_rdf_hist(SysParams p, int precision, RdfArgs r,
+__global__ __launch_bounds__(RS_BLOCK) void k_rdf_hist(SysParams p, RdfArgs r,
                                                         const int* __restrict__ nbr,
                                                         const int* __restrict__ ncnt,
-                                                        const int* __restrict__ posq,
-                                                        const double4* __restrict__ pos64,
+                                                        const int* __restrict__ order,
+                                                        const double* __restrict__ x,
                                                         long long* hist) {
   __shared__ unsigned long long lh[RDF_MAX_BINS];
   for (int b = threadIdx.x; b < r.nb; b += RS_BLOCK) lh[b] = 0ull;
@@ -94,22 +69,15 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rdf_hist(SysParams p, int precisio
   if (i < p.M) {
     const int n = ncnt[gtid];
     const int* L = nbr + (size_t)(gtid / RS_WAVE) * p.capc * RS_WAVE + (gtid & (RS_WAVE - 1));
+    const double* xi = x + 9 * (size_t)order[i];
     for (int k = 0; k < n; ++k) {
       const int j = L[(size_t)k * RS_WAVE];
       if (j <= i) continue;
-      if (precision == RS_PRECISION_FP32) {
-        const int4 a = ld3i(posq + 9 * (size_t)i), b = ld3i(posq + 9 * (size_t)j);
-        float dx = qdiff((uint32_t)a.x, (uint32_t)b.x, p.unitf[0]);
-        float dy = qdiff((uint32_t)a.y, (uint32_t)b.y, p.unitf[1]);
-        float dz = qdiff((uint32_t)a.z, (uint32_t)b.z, p.unitf[2]);
-        contrib32(r, sqrtf(r2f_of(dx, dy, dz)), lh);
-      } else {
-        const double4 a = pos64[3 * i], b = pos64[3 * j];
-        double dx = mic1(b.x - a.x, p.L[0], p.invL[0]);
-        double dy = mic1(b.y - a.y, p.L[1], p.invL[1]);
-        double dz = mic1(b.z - a.z, p.L[2], p.invL[2]);
-        contrib64(r, sqrt(r2_of(dx, dy, dz)), lh);
-      }
+      const double* xj = x + 9 * (size_t)order[j];
+      const double dx = mic1(xj[0] - xi[0], p.L[0], p.invL[0]);
+      const double dy = mic1(xj[1] - xi[1], p.L[1], p.invL[1]);
+      const double dz = mic1(xj[2] - xi[2], p.L[2], p.invL[2]);
+      contrib64(r, sqrt(r2_of(dx, dy, dz)), lh);
     }
   }
   __syncthreads();
@@ -142,12 +110,11 @@ __global__ void k_rdf_final(RdfArgs r, const long long* hist, const double* targ
 
 // dL/dx_O (times the block weight) by owner-computes over the full list, FP64.
 template <int CH>
-__global__ __launch_bounds__(RS_BLOCK) void k_rdf_grad(SysParams p, int precision, RdfArgs r,
+__global__ __launch_bounds__(RS_BLOCK) void k_rdf_grad(SysParams p, RdfArgs r,
                                                         const int* __restrict__ nbr,
                                                         const int* __restrict__ ncnt,
                                                         const int* __restrict__ order,
                                                         const double* __restrict__ x,
-                                                        const int* __restrict__ posq,
                                                         const double* __restrict__ G,
                                                         double* grad) {
   const int gtid = blockIdx.x * RS_BLOCK + threadIdx.x;
@@ -168,15 +135,7 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rdf_grad(SysParams p, int precisio
       const double rr = sqrt(r2_of(d[0], d[1], d[2]));
       if (rr > rmaxw + 0.01) continue;
       int lo, hi;
-      if (precision == RS_PRECISION_FP32) {
-        const int4 a = ld3i(posq + 9 * (size_t)i), b = ld3i(posq + 9 * (size_t)j);
-        float ex = qdiff((uint32_t)a.x, (uint32_t)b.x, p.unitf[0]);
-        float ey = qdiff((uint32_t)a.y, (uint32_t)b.y, p.unitf[1]);
-        float ez = qdiff((uint32_t)a.z, (uint32_t)b.z, p.unitf[2]);
-        window32(r, sqrtf(r2f_of(ex, ey, ez)), lo, hi);
-      } else {
-        window64(r, rr, lo, hi);
-      }
+      window64(r, rr, lo, hi);
       if (lo > hi) continue;
       double den = 0.0, dden = 0.0, s1 = 0.0, s2 = 0.0;
       for (int bb = lo; bb <= hi; ++bb) {
@@ -285,15 +244,15 @@ void launch_rdf(const Launch& L, int nb, double rmax, double width, double windo
   k_zero_ll<<<(nb + RS_BLOCK - 1) / RS_BLOCK, RS_BLOCK, 0, L.s>>>(L.b.hist, nb);
   int g = pair_blocks(p);
   if (p.CH == 8)
-    k_rdf_hist<8><<<g, RS_BLOCK, 0, L.s>>>(p, L.precision, r, L.b.nbr, L.b.ncnt, L.b.posq, L.b.pos64, L.b.hist);
+    k_rdf_hist<8><<<g, RS_BLOCK, 0, L.s>>>(p, r, L.b.nbr, L.b.ncnt, L.b.order, L.b.x, L.b.hist);
   else
-    k_rdf_hist<16><<<g, RS_BLOCK, 0, L.s>>>(p, L.precision, r, L.b.nbr, L.b.ncnt, L.b.posq, L.b.pos64, L.b.hist);
+    k_rdf_hist<16><<<g, RS_BLOCK, 0, L.s>>>(p, r, L.b.nbr, L.b.ncnt, L.b.order, L.b.x, L.b.hist);
   k_rdf_final<<<1, RS_BLOCK, 0, L.s>>>(r, L.b.hist, L.b.target, L.b.rdf_g, L.b.rdfG, L.b.scal, loss_dst);
   if (grad) {
     if (p.CH == 8)
-      k_rdf_grad<8><<<g, RS_BLOCK, 0, L.s>>>(p, L.precision, r, L.b.nbr, L.b.ncnt, L.b.order, L.b.x, L.b.posq, L.b.rdfG, L.b.grad);
+      k_rdf_grad<8><<<g, RS_BLOCK, 0, L.s>>>(p, r, L.b.nbr, L.b.ncnt, L.b.order, L.b.x, L.b.rdfG, L.b.grad);
     else
-      k_rdf_grad<16><<<g, RS_BLOCK, 0, L.s>>>(p, L.precision, r, L.b.nbr, L.b.ncnt, L.b.order, L.b.x, L.b.posq, L.b.rdfG, L.b.grad);
+      k_rdf_grad<16><<<g, RS_BLOCK, 0, L.s>>>(p, r, L.b.nbr, L.b.ncnt, L.b.order, L.b.x, L.b.rdfG, L.b.grad);
   }
 }
 

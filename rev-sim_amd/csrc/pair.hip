@@ -15,7 +15,11 @@
 // outputs in the caller's atom order.
 //
 // Pair arithmetic: FP32 on quantized positions (the int32 difference of 2^32-periodic positions is
-// the minimum image by construction) or FP64 (parity mode). Per pair of atoms a (own), b:
+// the minimum image by construction) or FP64 (parity mode). The FP32 path computes in the quantized
+// length unit lu = L/2^32: d is (float) of the int32 difference, with no scaling, and lu is folded
+// into the per-type constants of PairArgs, so F comes out in physical units and h in units of
+// 1/lu (rescaled once per molecule in FP64). The FP64 path uses the same code with lu = 1 nm.
+// Per pair of atoms a (own), b:
 //   f/r  = k q_a q_b (1/r^3 - 2 k_rf)              [+ 24ε/r^2 (2 s^12 - s^6) for O-O, s = σ/r]
 //   F_a -= (f/r) d, d = x_b - x_a
 //   h_a += K (u_a - u_b), K = f' r̂r̂ᵀ + (f/r)(I - r̂r̂ᵀ), i.e. (f/r) Δu + ((f' - f/r)/r^2)(d.Δu) d
@@ -46,6 +50,12 @@ struct PairArgs {
   double mrlo, mh, minvh;
   double* mslot;                    // per-block table gradient (adjoint)
   const double* __restrict__ wmax;  // this step's 64 max|w_O| slots (uint64 bits of doubles)
+  // constants in the kernel's length unit lu (L/2^32 in FP32, 1 nm in FP64), per pair type
+  // t = (O-O, O-H, H-H):
+  //   kA = k q q / lu^2, kB = 2 k q q k_rf lu       f/r·lu      = kA/r'^3 - kB
+  //   k3A = -3 k q q / lu^2                          ((f'-f/r)/r^2)·lu^3 = k3A / r'^5
+  //   sig2A = σ^2/lu^2, eps24A = 24ε/lu, epssA = 144ε/(σ lu), rc2 in lu^2, lu, hscale = 1/lu
+  double kA[3], kB[3], k3A[3], sig2A, eps24A, epssA, rc2, lu, hscale;
 };
 
 // Fixed-point scale of the per-block table-gradient sums: 2^(50 - e) with max|w_O| < 2^e, so one
@@ -69,12 +79,11 @@ struct P32 {
   static __device__ __forceinline__ U ld_u(const void* p, int slot) { return ld3f((const float*)p + 3 * slot); }
   static __device__ __forceinline__ void diff(const SysParams& p, const int4& a, const int4& b,
                                               float& dx, float& dy, float& dz) {
-    dx = qdiff((uint32_t)a.x, (uint32_t)b.x, p.unitf[0]);
-    dy = qdiff((uint32_t)a.y, (uint32_t)b.y, p.unitf[1]);
-    dz = qdiff((uint32_t)a.z, (uint32_t)b.z, p.unitf[2]);
+    dx = qdiffq((uint32_t)a.x, (uint32_t)b.x);
+    dy = qdiffq((uint32_t)a.y, (uint32_t)b.y);
+    dz = qdiffq((uint32_t)a.z, (uint32_t)b.z);
   }
   static __device__ __forceinline__ float r2(float dx, float dy, float dz) { return r2f_of(dx, dy, dz); }
-  static __device__ __forceinline__ float rc2(const SysParams& p) { return p.rc2f; }
   static __device__ __forceinline__ float rinv(float r2) { return __builtin_amdgcn_rsqf(r2); }
   static __device__ __forceinline__ float fma_(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 };
@@ -92,15 +101,20 @@ struct P64 {
     dz = mic1(b.z - a.z, p.L[2], p.invL[2]);
   }
   static __device__ __forceinline__ double r2(double dx, double dy, double dz) { return r2_of(dx, dy, dz); }
-  static __device__ __forceinline__ double rc2(const SysParams& p) { return p.rc2; }
   static __device__ __forceinline__ double rinv(double r2) { return 1.0 / sqrt(r2); }
   static __device__ __forceinline__ double fma_(double a, double b, double c) { return __builtin_fma(a, b, c); }
 };
 
 // ------------------------------------------------------------------ the pair kernel
 
+// The FP32 forward variant fits 64 VGPRs, i.e. 8 waves per SIMD, when asked to (unconstrained it
+// takes 66 and drops to 7); the other variants are left to the compiler.
 template <class P, int CH, bool ADJ>
-__global__ __launch_bounds__(RS_BLOCK) void k_pair(PairArgs A) {
+constexpr int pair_waves() { return (!ADJ && sizeof(typename P::R) == 4 && CH == 16) ? 8 : 1; }
+
+template <class P, int CH, bool ADJ>
+__global__ __launch_bounds__(RS_BLOCK) __attribute__((amdgpu_waves_per_eu(pair_waves<P, CH, ADJ>())))
+void k_pair(PairArgs A) {
   typedef typename P::R R;
   typedef typename P::Pos Pos;
   typedef typename P::U U;
@@ -130,15 +144,12 @@ __global__ __launch_bounds__(RS_BLOCK) void k_pair(PairArgs A) {
       uo[a][2] = t.z;
     }
   }
-  const R rc2 = P::rc2(p);
-  // per pair type (O-O, O-H, H-H): k q_a q_b, k q_a q_b 2 k_rf, -3 k q_a q_b. The forward kernel
-  // keeps the two-instruction form of f/r: the folded constants' registers would cost it a wave.
-  const R krf2 = (R)(2.0 * p.krf);
-  const R kq[3] = {(R)p.kqq[0], (R)p.kqq[1], (R)p.kqq[2]};
-  const R kqr[3] = {(R)(p.kqq[0] * 2.0 * p.krf), (R)(p.kqq[1] * 2.0 * p.krf), (R)(p.kqq[2] * 2.0 * p.krf)};
-  const R kq3[3] = {(R)(-3.0 * p.kqq[0]), (R)(-3.0 * p.kqq[1]), (R)(-3.0 * p.kqq[2])};
-  const R sig2 = (R)(p.sig * p.sig), eps24 = (R)(24.0 * p.eps);
-  const R epss = (R)(144.0 * p.eps / p.sig);
+  const R rc2 = (R)A.rc2;
+  const R kA[3] = {(R)A.kA[0], (R)A.kA[1], (R)A.kA[2]};
+  const R kB[3] = {(R)A.kB[0], (R)A.kB[1], (R)A.kB[2]};
+  const R k3A[3] = {(R)A.k3A[0], (R)A.k3A[1], (R)A.k3A[2]};
+  const R sig2 = (R)A.sig2A, eps24 = (R)A.eps24A, epss = (R)A.epssA;
+  const R lu = (R)A.lu;
 
   // MLP: own O's w (adjoint), table-gradient accumulator in LDS
   __shared__ long long lg[ADJ ? MLP_NG : 1];
@@ -180,12 +191,14 @@ __global__ __launch_bounds__(RS_BLOCK) void k_pair(PairArgs A) {
         R d[3];
         P::diff(p, own[a], pj[b], d[0], d[1], d[2]);
         const R r2 = P::r2(d[0], d[1], d[2]);
-        // r2 == 0 (coincident atoms) within the cutoff gives non-finite forces, which are flagged
+        // r2 == 0 (coincident atoms) within the cutoff gives non-finite forces, which are flagged.
+        // Outside the cutoff 1/r and the reaction-field constant are zeroed, which zeroes every
+        // term below (all carry a power of 1/r) without further selects.
         const bool in = r2 <= rc2;
-        const R ri = P::rinv(r2);
+        const R ri = in ? P::rinv(r2) : (R)0;
         const R ri2 = ri * ri, ri3 = ri2 * ri;
         const int ty = (a > 0) + (b > 0);
-        const R fr = in ? (ADJ ? P::fma_(kq[ty], ri3, -kqr[ty]) : kq[ty] * (ri3 - krf2)) : (R)0;
+        const R fr = P::fma_(kA[ty], ri3, in ? -kB[ty] : (R)0);
 #pragma unroll
         for (int t = 0; t < 3; ++t) F[a][t] -= fr * d[t];
         R du[3], dot = 0;
@@ -193,18 +206,19 @@ __global__ __launch_bounds__(RS_BLOCK) void k_pair(PairArgs A) {
 #pragma unroll
           for (int t = 0; t < 3; ++t) du[t] = uo[a][t] - uj[b][t];
           dot = P::fma_(d[2], du[2], P::fma_(d[1], du[1], d[0] * du[0]));
-          const R gd = in ? kq3[ty] * (ri3 * ri2) * dot : (R)0;
+          // (k3A r'^-3) r'^-2: the FP32 path's r'^-5 alone would be subnormal
+          const R gd = ((k3A[ty] * ri3) * ri2) * dot;
 #pragma unroll
           for (int t = 0; t < 3; ++t) hh[a][t] = P::fma_(gd, d[t], P::fma_(fr, du[t], hh[a][t]));
         }
         if (a == 0 && b == 0) {
           const R s2 = sig2 * ri2, s6 = s2 * s2 * s2, s12 = s6 * s6;
-          const R frl = in ? eps24 * ri2 * ((R)2 * s12 - s6) : (R)0;
+          const R frl = eps24 * ri2 * ((R)2 * s12 - s6);
 #pragma unroll
           for (int t = 0; t < 3; ++t) F[0][t] -= frl * d[t];
           if (ADJ) {
-            const R gdl = in ? -eps24 * ri2 * ri2 * ((R)28 * s12 - (R)8 * s6) * dot : (R)0;
-            const R fs = in ? epss * ri2 * ((R)4 * s12 - s6) : (R)0;
+            const R gdl = -eps24 * ri2 * ri2 * ((R)28 * s12 - (R)8 * s6) * dot;
+            const R fs = epss * ri2 * ((R)4 * s12 - s6);
 #pragma unroll
             for (int t = 0; t < 3; ++t) {
               hh[0][t] = P::fma_(gdl, d[t], P::fma_(frl, du[t], hh[0][t]));
@@ -214,7 +228,7 @@ __global__ __launch_bounds__(RS_BLOCK) void k_pair(PairArgs A) {
           }
           if (A.mlp) {
             // f(r) = -u'(r) by cubic Hermite between table nodes k, k+1
-            const R rr = r2 * ri;
+            const R rr = (r2 * ri) * lu;
             R tk = (rr - mrlo) * minvh;
             if (in && tk < (R)0) atomicOr(&A.flags[FLAG_MLP_RANGE], 1);
             tk = tk < (R)0 ? (R)0 : tk;
@@ -236,11 +250,11 @@ __global__ __launch_bounds__(RS_BLOCK) void k_pair(PairArgs A) {
             hermite<R>(tt, hb, hdb);
             const R f = hb[0] * fk0 + hb[1] * mh * dk0 + hb[2] * fk1 + hb[3] * mh * dk1;
             const R fp = (hdb[0] * fk0 + hdb[1] * mh * dk0 + hdb[2] * fk1 + hdb[3] * mh * dk1) * minvh;
-            const R frm = in ? f * ri : (R)0;
+            const R frm = f * ri;
 #pragma unroll
             for (int t = 0; t < 3; ++t) F[0][t] -= frm * d[t];
             if (ADJ) {
-              const R g2m = in ? (fp - f * ri) * ri2 : (R)0;
+              const R g2m = (fp * lu - f * ri) * ri2;
 #pragma unroll
               for (int t = 0; t < 3; ++t) {
                 hh[0][t] += frm * du[t] + (g2m * dot) * d[t];
@@ -307,7 +321,7 @@ __global__ __launch_bounds__(RS_BLOCK) void k_pair(PairArgs A) {
         vj[5] = A.vjb[3 * (size_t)m + 2];
         double* hm = A.h + 9 * (size_t)m;
 #pragma unroll
-        for (int t = 0; t < 9; ++t) hm[t] += hd[t];
+        for (int t = 0; t < 9; ++t) hm[t] += hd[t] * A.hscale;
       }
       // F was seeded with the molecule's bonded forces; add the pair forces
       bool fin = true;
@@ -390,7 +404,21 @@ void launch_pair(const Launch& L, bool adjoint, const double* wsrc, double wscal
   a.h = L.b.h;
   a.slots = L.b.slots;
   a.flags = L.b.flags;
-  if (L.precision == RS_PRECISION_FP32) {
+  const SysParams& p = L.p;
+  const bool f32 = L.precision == RS_PRECISION_FP32;
+  const double lu = f32 ? p.L[0] / 4294967296.0 : 1.0;
+  for (int t = 0; t < 3; ++t) {
+    a.kA[t] = p.kqq[t] / (lu * lu);
+    a.kB[t] = p.kqq[t] * 2.0 * p.krf * lu;
+    a.k3A[t] = -3.0 * p.kqq[t] / (lu * lu);
+  }
+  a.sig2A = p.sig * p.sig / (lu * lu);
+  a.eps24A = 24.0 * p.eps / lu;
+  a.epssA = 144.0 * p.eps / (p.sig * lu);
+  a.rc2 = f32 ? (double)p.rc2q : p.rc2;
+  a.lu = lu;
+  a.hscale = 1.0 / lu;
+  if (f32) {
     a.pos = L.b.posq;
     a.u = L.b.uf;
     if (adjoint) dispatch<P32, true>(L, a);

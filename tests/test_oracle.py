@@ -52,8 +52,6 @@ def test_exp_polynomial_accuracy(oracle_mod):
     t = np.linspace(-4.0, 0.0, 2001)
     e64 = np.array([L.or_exp(float(v)) for v in t])
     assert np.max(np.abs(e64 / np.exp(t) - 1.0)) < 4e-16
-    e32 = np.array([L.or_expf(float(v)) for v in t.astype(np.float32)], dtype=np.float64)
-    assert np.max(np.abs(e32 / np.exp(t.astype(np.float32).astype(np.float64)) - 1.0)) < 3e-7
 
 
 def test_forces_are_minus_energy_gradient(jittered216):
