@@ -24,6 +24,10 @@
  *   rs_rdf              rdf_differentiable + loss_rdf_part + its rrule, water/water.jl:631-736
  *   rs_msd              calculate_D + loss_fn_final + rrule, diffusion/diffusion.jl:861-911
  *   rs_molecule_pairs   Molly find_neighbors (CellListMap), called at water/water.jl:1025,1038
+ *   rs_energy           potential_energy and its θ-gradient gradient(potential_energy, sys, ...),
+ *                       water/water.jl:743,1183-1191
+ *   rs_reweight         reweighting_sim (ensemble-reweighting gradient estimator),
+ *                       water/water.jl:1107-1210
  *
  * Conventions
  *   units      nm, ps, u (g/mol), kJ/mol, e.
@@ -46,7 +50,7 @@
 extern "C" {
 #endif
 
-#define RS_ABI_VERSION 1
+#define RS_ABI_VERSION 2
 
 enum {
   RS_OK = 0,
@@ -94,6 +98,13 @@ typedef struct rs_config {
   double  clip_norm;       /* clip ||dF/dx^T lambda|| at this value (<= 0 or inf: no clip) */
   int32_t use_mlp;         /* 0 = classical force field only, 1 = + radial-basis MLP on O-O pairs */
   int32_t pair_chunks;     /* lanes cooperating on one molecule's list (0 = library default) */
+  /* Enthalpy-of-vaporization block loss (enthalpy_vaporization / loss_enth_vap,
+     water/water.jl:738-754), evaluated with the RDF loss every n_steps_loss steps:
+     dH = dhvap_u_gas - U/n_molecules + kB*temperature, L = w_dhvap * (dH - dhvap_target)^2.
+     U includes the reaction-field shift (zero at r_c). Not available with use_mlp. */
+  double  w_dhvap;         /* 0 = off */
+  double  dhvap_target;    /* kJ/mol (reference: 44.12 + 2.8) */
+  double  dhvap_u_gas;     /* mean gas-phase U per molecule, kJ/mol (reference: 3.69) */
   int32_t reserved[8];
 } rs_config;
 
@@ -157,6 +168,19 @@ int rs_rdf(rs_ctx* ctx, const double* x, double* g, int64_t* hist_fixed, double*
    dL/dframes (same layout). */
 int rs_msd(rs_ctx* ctx, const double* frames, int32_t n_frames, double* D, double* loss,
            double* dL_dframes);
+/* Potential energy U(x) (kJ/mol: reaction field with its r_c shift, Lennard-Jones, bonds, angles),
+   its gradient dU/dtheta (6, nullable) and the forces -dU/dx (3N, nullable), in FP64 in both
+   precisions. Classical force field only (RS_E_CONFIG with use_mlp). */
+int rs_energy(rs_ctx* ctx, const double* x, double* U, double* dU_dtheta6, double* F);
+/* Ensemble-reweighting estimator over one forward trajectory (reweighting_sim,
+   water/water.jl:1107-1210): at each of the B = n_steps/n_steps_loss block ends the block loss
+   L_b (RDF and enthalpy-of-vaporization terms), its direct gradient dL_b/dtheta and dU/dtheta
+   are taken, and
+     dL/dtheta = <dL_b/dtheta> - (<L_b dU_b/dtheta> - <L_b><dU_b/dtheta>) / (kB T),
+   <.> the mean over blocks; loss_out = <L_b>. x_out/v_out (nullable) receive the final state.
+   The MSD loss is not part of it (RS_E_CONFIG when configured); classical force field only. */
+int rs_reweight(rs_ctx* ctx, const double* x0, const double* v0, uint64_t seed, int32_t n_steps,
+                double* loss_out, double* dL_dtheta6, double* x_out, double* v_out);
 /* Molecule pairs (i<j) whose O-O distance is within the list radius; pairs[2*k], pairs[2*k+1]. */
 int rs_molecule_pairs(rs_ctx* ctx, const double* x, int32_t* pairs, int64_t capacity,
                       int64_t* n_pairs);

@@ -16,6 +16,10 @@
  *                       intermediate-frame gradients (diffusion/diffusion.jl:1534-1539 reset form)
  *   or_rdf              rdf_differentiable + rrule   water/water.jl:631-736 (binning per config)
  *   or_msd              calculate_D + loss_fn_final  diffusion/diffusion.jl:861-911
+ *   or_energy           potential_energy + its θ-gradient water/water.jl:743,1183-1191 (reaction
+ *                       field shifted to zero at r_c, truncated LJ, harmonic bond and angle)
+ *   dhvap_block         enthalpy_vaporization + loss_enth_vap water/water.jl:738-754
+ *   or_reweight         reweighting_sim (ensemble reweighting)  water/water.jl:1107-1210
  *   or_value_and_grad_stored
  *                       independent check: textbook (λx, λv) adjoint over the stored forward
  *                       trajectory — plays the role of the DMS gradient (water/water.jl:1052-1099)
@@ -62,6 +66,8 @@ typedef struct or_ctx {
   double krf, rc2;
   float rc2f;
   float rc2q; /* FP32 pair rule threshold in units of (L/2^32)^2 */
+  double crf; /* reaction-field energy shift (1/r_c) 3 eps_rf / (2 eps_rf + 1) */
+  int rule64; /* 1 while evaluating the energy: FP64 membership rule in both precisions */
   double* mass3; /* per degree of freedom */
   /* atom-pair half list (a<b, different molecules), FP64 mic distance <= rlist */
   double rlist;
@@ -130,7 +136,7 @@ static inline float r2q_quant(const or_ctx* o, const double* xa, const double* x
 }
 
 static inline int pair_in(const or_ctx* o, const double* xa, const double* xb, double r2) {
-  if (o->cfg.precision == RS_PRECISION_FP32) return r2q_quant(o, xa, xb) <= o->rc2q;
+  if (o->cfg.precision == RS_PRECISION_FP32 && !o->rule64) return r2q_quant(o, xa, xb) <= o->rc2q;
   return r2 <= o->rc2;
 }
 
@@ -216,6 +222,7 @@ int or_create(const rs_config* cfg, or_ctx** out) {
   o->rc2 = rc * rc;
   o->rc2f = (float)(rc * rc);
   o->rc2q = (float)(rc * rc * o->qscale[0] * o->qscale[0]);
+  o->crf = (1.0 / rc) * ((3.0 * cfg->eps_rf) / (2.0 * cfg->eps_rf + 1.0));
   o->mass3 = (double*)malloc(sizeof(double) * o->N3);
   for (int a = 0; a < o->N; ++a)
     for (int k = 0; k < 3; ++k) o->mass3[3 * a + k] = (a % 3 == 0) ? cfg->mass_O : cfg->mass_H;
@@ -653,7 +660,7 @@ int or_mlp_vjp(or_ctx* o, const double* r, const double* g1, const double* g2, i
    θ = (σ_O, ε_O, q_H, k_b, r0, k_a). Per-thread buffers followed by a reduction, as in the
    reference (water/water.jl:879-885,970-972,1424-1430). */
 static void eval_terms(or_ctx* o, const double* x, const double* c, const double* w, double* F,
-                       double* h, double* vjp, double* energy, double* dW) {
+                       double* h, double* vjp, double* energy, double* dW, double* dU) {
   const int N3 = o->N3;
   const double sig = o->theta[0], eps = o->theta[1], qH = o->theta[2];
   const double kb = o->theta[3], r0 = o->theta[4], ka = o->theta[5];
@@ -668,6 +675,7 @@ static void eval_terms(or_ctx* o, const double* x, const double* c, const double
   double* hb = adj ? (double*)calloc((size_t)N3 * nth, sizeof(double)) : NULL;
   double* vb = (double*)calloc((size_t)NTH * nth, sizeof(double));
   double* eb = (double*)calloc((size_t)nth, sizeof(double));
+  double* db = (double*)calloc((size_t)NTH * nth, sizeof(double)); /* dU/dθ per thread */
   const int mlp_on = o->use_mlp;
   double* dWb = (adj && mlp_on && dW) ? (double*)calloc((size_t)MLP_NW * nth, sizeof(double)) : NULL;
   double* u = NULL;
@@ -685,6 +693,7 @@ static void eval_terms(or_ctx* o, const double* x, const double* c, const double
     double* Ft = Fb + (size_t)N3 * t;
     double* ht = adj ? hb + (size_t)N3 * t : NULL;
     double* vt = vb + NTH * t;
+    double* dUt = db + NTH * t;
     double* dWt = dWb ? dWb + (size_t)MLP_NW * t : NULL;
     double et = 0.0;
     mlp_jet J;
@@ -706,7 +715,10 @@ static void eval_terms(or_ctx* o, const double* x, const double* c, const double
       double Frf = cp * (invr2 - 2.0 * krf * r);
       double Fv = Frf;
       double dFds = 0.0, dFde = 0.0;
-      if (energy) et += cp * (1.0 / r + krf * r2);
+      /* reaction-field pair energy, shifted to zero at r_c; -d/dr of it is Frf */
+      const double erf_ = cp * (1.0 / r + krf * r2 - o->crf);
+      if (energy) et += erf_;
+      if (dU) dUt[2] += 2.0 * erf_ / qH; /* every charge product is q_H^2 times a constant */
       if (aO && bO && eps != 0.0) {
         double six = pow(sig * sig * invr2, 3);
         double twelve = six * six;
@@ -716,6 +728,10 @@ static void eval_terms(or_ctx* o, const double* x, const double* c, const double
         dFde = Flj / eps;                                            /* times ndr */
         Fv += Flj;
         if (energy) et += 4.0 * eps * (twelve - six);
+        if (dU) {
+          dUt[0] += (24.0 * eps / sig) * (2.0 * twelve - six);
+          dUt[1] += 4.0 * (twelve - six);
+        }
       }
       if (aO && bO && mlp_on) {
         /* MLP pair energy u(r): f = -u'(r), f' = -u''(r) */
@@ -773,6 +789,10 @@ static void eval_terms(or_ctx* o, const double* x, const double* c, const double
         double Fv = -kb * disp;
         double Fp = -kb;
         if (energy) et += 0.5 * kb * disp * disp;
+        if (dU) {
+          dUt[3] += 0.5 * disp * disp;
+          dUt[4] += -kb * disp;
+        }
         for (int k = 0; k < 3; ++k) {
           Ft[3 * i + k] -= Fv * ndr[k];
           Ft[3 * j + k] += Fv * ndr[k];
@@ -803,6 +823,7 @@ static void eval_terms(or_ctx* o, const double* x, const double* c, const double
                           fi, fj, fk, hi, hj, hk, &disp, p1d, p3d, energy ? &ea : NULL);
       if (ok) {
         et += ea;
+        if (dU) dUt[5] += 0.5 * disp * disp;
         for (int k = 0; k < 3; ++k) {
           Ft[3 * (O + 1) + k] += fi[k];
           Ft[3 * O + k] += fj[k];
@@ -841,6 +862,13 @@ static void eval_terms(or_ctx* o, const double* x, const double* c, const double
     for (int t = 0; t < nth; ++t) s += eb[t];
     *energy = s;
   }
+  if (dU) {
+    for (int k = 0; k < NTH; ++k) {
+      double s = 0.0;
+      for (int t = 0; t < nth; ++t) s += db[NTH * t + k];
+      dU[k] = s;
+    }
+  }
   if (dWb) {
     for (int k = 0; k < MLP_NW; ++k) {
       double s = 0.0;
@@ -853,19 +881,35 @@ static void eval_terms(or_ctx* o, const double* x, const double* c, const double
   free(hb);
   free(vb);
   free(eb);
+  free(db);
   free(u);
 }
 
 int or_forces(or_ctx* o, const double* x, double* F, double* energy) {
   build_list(o, x);
-  eval_terms(o, x, NULL, NULL, F, NULL, NULL, energy, NULL);
+  eval_terms(o, x, NULL, NULL, F, NULL, NULL, energy, NULL, NULL);
+  return 0;
+}
+
+/* Potential energy, its θ-gradient and the forces (potential_energy, water/water.jl:743;
+   gradient(potential_energy, ...), water/water.jl:1183-1191). */
+int or_energy(or_ctx* o, const double* x, double* U, double* dU, double* F) {
+  double* Ft = F ? F : (double*)malloc(sizeof(double) * o->N3);
+  double e = 0.0, d[NTH];
+  build_list(o, x);
+  o->rule64 = 1; /* the energy is FP64 throughout, its pair set the FP64 rule's */
+  eval_terms(o, x, NULL, NULL, Ft, NULL, NULL, &e, NULL, d);
+  o->rule64 = 0;
+  if (U) *U = e;
+  if (dU) memcpy(dU, d, sizeof(d));
+  if (!F) free(Ft);
   return 0;
 }
 
 int or_adjoint_terms(or_ctx* o, const double* x, const double* c, const double* w, double* F,
                      double* h, double* vjp6) {
   build_list(o, x);
-  eval_terms(o, x, c, w, F, h, vjp6, NULL, NULL);
+  eval_terms(o, x, c, w, F, h, vjp6, NULL, NULL, NULL);
   return 0;
 }
 
@@ -874,7 +918,7 @@ int or_adjoint_terms_mlp(or_ctx* o, const double* x, const double* c, const doub
                          double* h, double* vjp6, double* dW) {
   build_list(o, x);
   memset(dW, 0, sizeof(double) * MLP_NW);
-  eval_terms(o, x, c, w, F, h, vjp6, NULL, dW);
+  eval_terms(o, x, c, w, F, h, vjp6, NULL, dW, NULL);
   return 0;
 }
 
@@ -1207,10 +1251,27 @@ static snap_t* find_snap(or_ctx* o, int step) {
   return NULL;
 }
 
+/* Enthalpy-of-vaporization block loss at x (enthalpy_vaporization / loss_enth_vap,
+   water/water.jl:738-754): dH = u_gas - U/M + kB T, l = w (dH - target)^2. Returns l; *cp_out =
+   c' = 2 w (dH - target)/M, the factor of dl/dx = c' F and dl/dθ = -c' dU/dθ; dU_out (nullable)
+   receives dU/dθ. */
+static double dhvap_block(or_ctx* o, const double* x, double* F, double* dU_out, double* cp_out) {
+  double U, dU[NTH];
+  or_energy(o, x, &U, dU, F);
+  const double w = o->cfg.w_dhvap;
+  const double dH = o->cfg.dhvap_u_gas - U / o->M + o->cfg.kB * o->cfg.temperature;
+  const double d = dH - o->cfg.dhvap_target;
+  if (cp_out) *cp_out = 2.0 * w * d / o->M;
+  if (dU_out) memcpy(dU_out, dU, sizeof(dU));
+  return w * d * d;
+}
+
 /* forward_sim loop (water/water.jl:1029-1045), run_grads=false, with the loss of each block and
-   the MSD frames. Optionally stores every state (for the stored-trajectory adjoint). */
+   the MSD frames. Optionally stores every state (for the stored-trajectory adjoint). With rw
+   (19 accumulators) it is the forward loop of reweighting_sim (water/water.jl:1157-1192): at each
+   block end rw[0] += L_b, rw[1+k] += dL_b/dθ_k, rw[7+k] += dU/dθ_k, rw[13+k] += L_b dU/dθ_k. */
 static int forward_impl(or_ctx* o, const double* x0, const double* v0, uint64_t seed, int n_steps,
-                        double* x_out, double* v_out, double* loss_out, double* traj_x) {
+                        double* x_out, double* v_out, double* loss_out, double* traj_x, double* rw) {
   plan_t p;
   int rc = make_plan(o, n_steps, &p);
   if (rc) return rc;
@@ -1232,9 +1293,11 @@ static int forward_impl(or_ctx* o, const double* x0, const double* v0, uint64_t 
   add_snap(o, 0, x, v);
   if (traj_x) memcpy(traj_x, x, sizeof(double) * N3);
   build_list(o, x);
-  double lrdf = 0.0;
+  double lrdf = 0.0, ldh = 0.0;
+  double* Fe = (double*)malloc(sizeof(double) * N3);
+  if (rw) memset(rw, 0, sizeof(double) * 19);
   for (int n = 1; n <= n_steps; ++n) {
-    eval_terms(o, x, NULL, NULL, F, NULL, NULL, NULL, NULL);
+    eval_terms(o, x, NULL, NULL, F, NULL, NULL, NULL, NULL, NULL);
     or_noise(o, seed, n, xi);
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < N3; ++i) {
@@ -1249,17 +1312,38 @@ static int forward_impl(or_ctx* o, const double* x0, const double* v0, uint64_t 
     if (n % o->cfg.nl_every == 0) build_list(o, x);
     if (traj_x) memcpy(traj_x + (size_t)n * N3, x, sizeof(double) * N3);
     if ((p.n_log > 0 && n % p.n_log == 0) || n % p.n_loss == 0) add_snap(o, n, x, v);
+    double lb = 0.0;
     if (n % p.n_loss == 0 && o->cfg.w_rdf != 0.0) {
       double l;
       or_rdf(o, x, NULL, NULL, &l, NULL);
       lrdf += l;
+      lb += o->cfg.w_rdf * l;
+    }
+    if (n % p.n_loss == 0 && (o->cfg.w_dhvap != 0.0 || rw)) {
+      double dU[NTH], cp = 0.0, l = 0.0;
+      if (o->cfg.w_dhvap != 0.0) {
+        l = dhvap_block(o, x, Fe, dU, &cp);
+        ldh += l;
+        lb += l;
+      } else {
+        or_energy(o, x, NULL, dU, Fe);
+      }
+      if (rw) {
+        rw[0] += lb;
+        for (int k = 0; k < NTH; ++k) {
+          rw[1 + k] += -cp * dU[k];
+          rw[7 + k] += dU[k];
+          rw[13 + k] += lb * dU[k];
+        }
+      }
     }
     int f = frame_index(&p, n, o->cfg.msd_every);
     if (f >= 0)
       for (int i = 0; i < M; ++i)
         for (int k = 0; k < 3; ++k) o->frames[(size_t)f * 3 * M + 3 * i + k] = x[9 * i + k];
   }
-  double loss = o->cfg.w_rdf * lrdf / p.n_blocks;
+  double loss = o->cfg.w_rdf * lrdf / p.n_blocks + ldh / p.n_blocks;
+  free(Fe);
   if (p.msd_on) {
     double lm;
     or_msd(o, o->frames, p.nf, NULL, &lm, NULL);
@@ -1279,12 +1363,13 @@ static int forward_impl(or_ctx* o, const double* x0, const double* v0, uint64_t 
 
 int or_forward(or_ctx* o, const double* x0, const double* v0, uint64_t seed, int32_t n_steps,
                double* x_out, double* v_out, double* loss_out) {
-  return forward_impl(o, x0, v0, seed, n_steps, x_out, v_out, loss_out, NULL);
+  return forward_impl(o, x0, v0, seed, n_steps, x_out, v_out, loss_out, NULL, NULL);
 }
 
-/* Gradient seed at state t: RDF block loss (weight w_rdf/n_blocks) and the MSD frame gradient. */
+/* Gradient seed at state t: RDF and enthalpy-of-vaporization block losses (weights w/n_blocks;
+   the latter also adds its direct term -c' dU/dθ into dth) and the MSD frame gradient. */
 static double seed_grad(or_ctx* o, const plan_t* p, int t, const double* x, const double* dLdf,
-                        double* g, int with_rdf) {
+                        double* g, int with_rdf, double* dth) {
   const int N3 = o->N3, M = o->M;
   memset(g, 0, sizeof(double) * N3);
   double l = 0.0;
@@ -1295,6 +1380,15 @@ static double seed_grad(or_ctx* o, const plan_t* p, int t, const double* x, cons
     for (int i = 0; i < N3; ++i) g[i] += wgt * gr[i];
     l *= wgt;
     free(gr);
+  }
+  if (with_rdf && o->cfg.w_dhvap != 0.0) {
+    double* Fe = (double*)malloc(sizeof(double) * N3);
+    double dU[NTH], cp;
+    const double wgt = 1.0 / p->n_blocks;
+    l += wgt * dhvap_block(o, x, Fe, dU, &cp);
+    for (int i = 0; i < N3; ++i) g[i] += wgt * cp * Fe[i];
+    for (int k = 0; k < NTH; ++k) dth[k] -= wgt * cp * dU[k];
+    free(Fe);
   }
   int f = frame_index(p, t, o->cfg.msd_every);
   if (f >= 0 && dLdf)
@@ -1330,13 +1424,13 @@ int or_reverse(or_ctx* o, const double* xN, const double* vN, uint64_t seed, int
     or_msd(o, o->frames, p.nf, NULL, &lm, dLdf);
     loss += lm;
   }
-  loss += seed_grad(o, &p, n_steps, x, dLdf, g, 1);
+  double th[NTH] = {0};
+  loss += seed_grad(o, &p, n_steps, x, dLdf, g, 1, th);
   for (int i = 0; i < N3; ++i) {
     A[i] = g[i] * (1.0 + e);
     B[i] = g[i] * (1.0 + e);
     Dd[i] = g[i] * (1.0 + 2.0 * e + e * e) / (2.0 * e);
   }
-  double th[NTH] = {0};
   if (o->use_mlp) {
     if (!o->dW) o->dW = (double*)malloc(sizeof(double) * MLP_NW);
     memset(o->dW, 0, sizeof(double) * MLP_NW);
@@ -1363,7 +1457,7 @@ int or_reverse(or_ctx* o, const double* xN, const double* vN, uint64_t seed, int
         since_build = 0;
       }
       double vj[NTH];
-      eval_terms(o, x, C, w, F, h, vj, NULL, o->dW);
+      eval_terms(o, x, C, w, F, h, vj, NULL, o->dW, NULL);
       if (o->cfg.clip_norm > 0.0 && isfinite(o->cfg.clip_norm)) {
         double nn = 0.0;
         for (int i = 0; i < N3; ++i) nn += h[i] * h[i];
@@ -1397,7 +1491,7 @@ int or_reverse(or_ctx* o, const double* xN, const double* vN, uint64_t seed, int
         memcpy(v, sn->v, sizeof(double) * N3);
       }
       trunc_count = 0;
-      loss += seed_grad(o, &p, t, x, dLdf, g, 1);
+      loss += seed_grad(o, &p, t, x, dLdf, g, 1, th);
       build_list(o, x);
       since_build = 0;
       for (int i = 0; i < N3; ++i) {
@@ -1408,7 +1502,7 @@ int or_reverse(or_ctx* o, const double* xN, const double* vN, uint64_t seed, int
         h[i] = 0.0;
       }
     } else if (frame_index(&p, t, o->cfg.msd_every) >= 0) {
-      seed_grad(o, &p, t, x, dLdf, g, 0);
+      seed_grad(o, &p, t, x, dLdf, g, 0, th);
       for (int i = 0; i < N3; ++i) {
         A[i] += g[i] * (1.0 + e);
         B[i] += g[i] * (1.0 + e);
@@ -1460,7 +1554,7 @@ int or_value_and_grad_stored(or_ctx* o, const double* x0, const double* v0, uint
   const double dt = o->cfg.dt, e = exp(-o->cfg.gamma * dt);
   double* traj = (double*)malloc(sizeof(double) * (size_t)(n_steps + 1) * N3);
   double loss;
-  rc = forward_impl(o, x0, v0, seed, n_steps, NULL, NULL, &loss, traj);
+  rc = forward_impl(o, x0, v0, seed, n_steps, NULL, NULL, &loss, traj, NULL);
   if (rc) {
     free(traj);
     return rc;
@@ -1473,8 +1567,8 @@ int or_value_and_grad_stored(or_ctx* o, const double* x0, const double* v0, uint
     dLdf = (double*)malloc(sizeof(double) * (size_t)p.nf * 3 * M);
     or_msd(o, o->frames, p.nf, NULL, NULL, dLdf);
   }
-  seed_grad(o, &p, n_steps, traj + (size_t)n_steps * N3, dLdf, lx, 1);
   double th[NTH] = {0};
+  seed_grad(o, &p, n_steps, traj + (size_t)n_steps * N3, dLdf, lx, 1, th);
   if (o->use_mlp) {
     if (!o->dW) o->dW = (double*)malloc(sizeof(double) * MLP_NW);
     memset(o->dW, 0, sizeof(double) * MLP_NW);
@@ -1490,12 +1584,12 @@ int or_value_and_grad_stored(or_ctx* o, const double* x0, const double* v0, uint
     }
     double vj[NTH];
     build_list(o, xp);
-    eval_terms(o, xp, cc, hh, F, g, vj, NULL, o->dW);
+    eval_terms(o, xp, cc, hh, F, g, vj, NULL, o->dW, NULL);
     for (int i = 0; i < N3; ++i) lx[i] += g[i];
     for (int k = 0; k < NTH; ++k) th[k] += vj[k];
     int t = n - 1;
     if (t > 0 && frame_index(&p, t, o->cfg.msd_every) >= 0) {
-      seed_grad(o, &p, t, traj + (size_t)t * N3, dLdf, g, 0);
+      seed_grad(o, &p, t, traj + (size_t)t * N3, dLdf, g, 0, th);
       for (int i = 0; i < N3; ++i) lx[i] += g[i];
     }
   }
@@ -1509,6 +1603,26 @@ int or_value_and_grad_stored(or_ctx* o, const double* x0, const double* v0, uint
   free(hh);
   free(cc);
   free(dLdf);
+  return 0;
+}
+
+/* reweighting_sim (water/water.jl:1107-1210): forward loop with per-block loss, direct loss
+   gradient and dU/dθ; dL/dθ = <dL_b/dθ> - β(<L_b dU_b/dθ> - <L_b><dU_b/dθ>) (:1201-1207). */
+int or_reweight(or_ctx* o, const double* x0, const double* v0, uint64_t seed, int32_t n_steps,
+                double* loss_out, double* dLdth, double* x_out, double* v_out) {
+  plan_t p;
+  int rc = make_plan(o, n_steps, &p);
+  if (rc) return rc;
+  if (p.msd_on) return RS_E_CONFIG;
+  double rw[19];
+  rc = forward_impl(o, x0, v0, seed, n_steps, x_out, v_out, NULL, NULL, rw);
+  if (rc) return rc;
+  o->rec_steps = 0; /* no records for a reverse pass */
+  const double B = p.n_blocks, beta = 1.0 / (o->cfg.kB * o->cfg.temperature);
+  const double loss = rw[0] / B;
+  if (loss_out) *loss_out = loss;
+  if (dLdth)
+    for (int k = 0; k < NTH; ++k) dLdth[k] = rw[1 + k] / B - beta * (rw[13 + k] / B - loss * rw[7 + k] / B);
   return 0;
 }
 

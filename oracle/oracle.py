@@ -49,6 +49,8 @@ _SIGS = [
     ("or_adjoint_terms_mlp", ctypes.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
     ("or_mlp_grad", ctypes.c_int, [_vp, _dp]),
     ("or_num_threads", ctypes.c_int, []),
+    ("or_energy", ctypes.c_int, [_vp, _dp, _dp, _dp, _dp]),
+    ("or_reweight", ctypes.c_int, [_vp, _dp, _dp, _u64, ctypes.c_int32, _dp, _dp, _dp, _dp]),
     ("or_exp", ctypes.c_double, [ctypes.c_double]),
     ("or_philox4x64", None, [_P(ctypes.c_uint64), _P(ctypes.c_uint64), _P(ctypes.c_uint64)]),
 ]
@@ -191,6 +193,21 @@ class Oracle:
         self._chk(self.L.or_value_and_grad_stored(self.h, _d(x0), _d(v0), seed, n_steps, _d(loss),
                                                   _d(g)))
         return float(loss[0]), g
+
+    def energy(self, x):
+        """(U, dU/dθ, F) at x."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        U, dU, F = np.zeros(1), np.zeros(6), np.zeros(self.N3)
+        self._chk(self.L.or_energy(self.h, _d(x), _d(U), _d(dU), _d(F)))
+        return float(U[0]), dU, F
+
+    def reweight(self, x0, v0, seed, n_steps):
+        x0, v0 = (np.ascontiguousarray(a, dtype=np.float64) for a in (x0, v0))
+        loss, g = np.zeros(1), np.zeros(6)
+        x, v = np.zeros(self.N3), np.zeros(self.N3)
+        self._chk(self.L.or_reweight(self.h, _d(x0), _d(v0), seed, n_steps, _d(loss), _d(g), _d(x),
+                                     _d(v)))
+        return float(loss[0]), g, x, v
 
     def list_radius(self):
         return self.L.or_list_radius(self.h)

@@ -100,6 +100,7 @@ void set_params(rs_ctx* c) {
   p.rc2q = (float)(rc * rc * p.qscale[0] * p.qscale[0]);
   p.krf = (1.0 / (rc * rc * rc)) * ((g.eps_rf - 1.0) / (2.0 * g.eps_rf + 1.0));
   p.krff = (float)p.krf;
+  p.crf = (1.0 / rc) * ((3.0 * g.eps_rf) / (2.0 * g.eps_rf + 1.0));
   p.kc = g.coulomb_k;
   p.sig = c->theta[0];
   p.eps = c->theta[1];
@@ -200,6 +201,9 @@ int alloc_all(rs_ctx* c) {
       (r = dalloc(c, &b.rdf_g, nb)))
     return r;
   if ((r = dalloc(c, &b.msd_acc, 64))) return r;
+  if ((r = dalloc(c, &b.Fe, N3)) || (r = dalloc(c, &b.eslots, (size_t)pair_blocks(p) * 8)) ||
+      (r = dalloc(c, &b.eout, 8)) || (r = dalloc(c, &b.rwacc, 32)))
+    return r;
   if (c->cfg.use_mlp) {
     if ((r = dalloc(c, &b.mlp_w, MLP_NW)) || (r = dalloc(c, &b.mlp_wb, MLP_NW)) ||
         (r = dalloc(c, (double2**)&b.mlp_tab, MLP_NT)) ||
@@ -334,8 +338,12 @@ int grow_or_widen(rs_ctx* c, int why, int attempt) {
 
 double kT_of(const rs_ctx* c) { return c->cfg.kB * c->cfg.temperature; }
 
+// Forward trajectory (forward_sim, water/water.jl:1029-1045). With `reweight` it is the forward
+// loop of reweighting_sim (water/water.jl:1157-1192) instead: no records for a reverse pass, and at
+// every block end the block loss, its direct θ-gradient and dU/dθ go into the reweighting
+// accumulators.
 int forward_impl(rs_ctx* c, const double* x0, const double* v0, uint64_t seed, int n_steps,
-                 double* x_out, double* v_out, double* loss_out) {
+                 double* x_out, double* v_out, double* loss_out, bool reweight = false) {
   Plan P;
   int r = make_plan(c, n_steps, &P);
   if (r) return r;
@@ -346,8 +354,14 @@ int forward_impl(rs_ctx* c, const double* x0, const double* v0, uint64_t seed, i
   const rs_config& g = c->cfg;
   const double dt = g.dt, e = std::exp(-g.gamma * dt), s = std::sqrt(1.0 - e * e), kT = kT_of(c);
   free_records(c);
+  if (reweight && P.msd_on)
+    return fail(c, RS_E_CONFIG, "the reweighting estimator takes block losses only: set w_msd = 0");
+  if (reweight && c->cfg.use_mlp)
+    return fail(c, RS_E_CONFIG, "the reweighting estimator needs the classical force field (use_mlp = 0)");
   if (P.msd_on && (r = ensure_frames(c, P.nf))) return r;
-  for (int t = 0; t <= n_steps; ++t)
+  const double RT = kT;
+  const bool dh = g.w_dhvap != 0.0;
+  for (int t = 0; t <= n_steps && !reweight; ++t)
     if (P.snap_at(t)) {
       double *sx, *sv;
       if ((r = dalloc(c, &sx, c->N3)) || (r = dalloc(c, &sv, c->N3))) return r;
@@ -360,6 +374,7 @@ int forward_impl(rs_ctx* c, const double* x0, const double* v0, uint64_t seed, i
     HIPCHK(c, hipMemcpyAsync(b.v, v0, sizeof(double) * c->N3, hipMemcpyHostToDevice, c->s));
     if ((r = reset_flags(c))) return r;
     HIPCHK(c, hipMemsetAsync(b.scal, 0, sizeof(double) * 16, c->s));
+    if (reweight) HIPCHK(c, hipMemsetAsync(b.rwacc, 0, sizeof(double) * 32, c->s));
     launch_wrap_state(c->L);
     rebuild(c, false);
     launch_bonded(c->L, false, nullptr, 0, 0, nullptr, 0, 0);
@@ -387,6 +402,15 @@ int forward_impl(rs_ctx* c, const double* x0, const double* v0, uint64_t seed, i
                    false, b.scal + 0);
         tend(c);
       }
+      if (n % P.n_loss == 0 && (dh || reweight)) {
+        // enthalpy of vaporization of the block (the state after the step, as the RDF)
+        tbegin(c, RS_PHASE_LOSS);
+        launch_energy(c->L);
+        launch_dhvap(c->L, g.w_dhvap, g.dhvap_target, g.dhvap_u_gas, RT, dh ? 1.0 / P.n_blocks : 0.0,
+                     false, false);
+        if (reweight) launch_reweight_acc(c->L, g.w_rdf, dh);
+        tend(c);
+      }
       int f = P.frame_of(n);
       if (f >= 0) launch_frame_copy(c->L, f);
     }
@@ -409,7 +433,7 @@ int forward_impl(rs_ctx* c, const double* x0, const double* v0, uint64_t seed, i
   if (x_out) HIPCHK(c, hipMemcpy(x_out, b.x, sizeof(double) * c->N3, hipMemcpyDeviceToHost));
   if (v_out) HIPCHK(c, hipMemcpy(v_out, b.v, sizeof(double) * c->N3, hipMemcpyDeviceToHost));
   if (loss_out) *loss_out = sc[0];
-  c->rec_valid = true;
+  c->rec_valid = !reweight;
   c->rec_steps = n_steps;
   c->rec_seed = seed;
   return 0;
@@ -425,6 +449,11 @@ void seed_at(rs_ctx* c, const Plan& P, int t, bool with_rdf, bool reset, double 
   if (with_rdf && g.w_rdf != 0.0)
     launch_rdf(c->L, g.rdf_bins, g.rdf_rmax, g.rdf_width, g.rdf_window, g.w_rdf / P.n_blocks, true,
                b.scal + 0);
+  if (with_rdf && g.w_dhvap != 0.0) {
+    // enthalpy-of-vaporization block loss: dl/dx = c' F into the seed, -c' dU/dθ into dL/dθ
+    launch_energy(c->L);
+    launch_dhvap(c->L, g.w_dhvap, g.dhvap_target, g.dhvap_u_gas, kT_of(c), 1.0 / P.n_blocks, true, true);
+  }
   int f = P.frame_of(t);
   if (f >= 0) launch_msd_inject(c->L, f);
   launch_seed(c->L, reset, e);
@@ -599,6 +628,9 @@ void rs_default_config(rs_config* g) {
   g->w_rdf = 1.0;
   g->msd_target = 2.3e-9;
   g->clip_norm = INFINITY;
+  g->w_dhvap = 0.0;
+  g->dhvap_target = 44.12 + 2.8;
+  g->dhvap_u_gas = 3.69;
 }
 
 const char* rs_last_error(const rs_ctx* c) { return c ? c->err.c_str() : g_last_error.c_str(); }
@@ -623,6 +655,10 @@ int rs_create(const rs_config* cfg, int32_t device, rs_ctx** out) {
       !(g.rdf_window > 0.0))
     return fail(nullptr, RS_E_CONFIG, "rdf_bins in [1,1024] and positive rdf_rmax/width/window required");
   if (g.use_mlp != 0 && g.use_mlp != 1) return fail(nullptr, RS_E_CONFIG, "use_mlp must be 0 or 1");
+  if (g.use_mlp && g.w_dhvap != 0.0)
+    return fail(nullptr, RS_E_CONFIG, "the enthalpy-of-vaporization loss needs the classical force field (use_mlp = 0)");
+  if (!std::isfinite(g.w_dhvap) || !std::isfinite(g.dhvap_target) || !std::isfinite(g.dhvap_u_gas))
+    return fail(nullptr, RS_E_CONFIG, "w_dhvap, dhvap_target and dhvap_u_gas must be finite");
   if (g.use_mlp && (g.cutoff <= MLP_RLO + 0.1))
     return fail(nullptr, RS_E_CONFIG, "the MLP term needs a cutoff above 0.25 nm");
   int ndev = 0;
@@ -675,7 +711,8 @@ int rs_destroy(rs_ctx* c) {
                   b.cell_start, b.cell_fill, b.tmp, b.order, b.rank, b.posq, b.pos64, b.uf,
                   b.ud, b.O64, b.Oq, b.nbr, b.nbr_tmp, b.ncnt, b.flags, b.slots, b.vjb, b.dth, b.scal, b.wmax, b.hist, b.rdfG,
                   b.target, b.rdf_g, b.frames, b.msd_del, b.msd_acc, b.msd_grad, b.ext_noise,
-                  b.hpart, b.mlp_w, b.mlp_wb, b.mlp_tab, b.mlp_slot, b.mlp_G, b.mlp_part, b.mlp_dW};
+                  b.hpart, b.mlp_w, b.mlp_wb, b.mlp_tab, b.mlp_slot, b.mlp_G, b.mlp_part, b.mlp_dW,
+                  b.Fe, b.eslots, b.eout, b.rwacc};
   for (void* q : ptrs)
     if (q) hipFree(q);
   for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
@@ -756,6 +793,40 @@ int rs_value_and_grad(rs_ctx* c, const double* x0, const double* v0, uint64_t se
     if ((r = rs_mlp_grad(c, d.data(), MLP_NW))) return r;
     for (int k = 0; k < MLP_NW; ++k) dL_dmlp[k] = (float)d[k];
   }
+  return 0;
+}
+
+int rs_reweight(rs_ctx* c, const double* x0, const double* v0, uint64_t seed, int32_t n_steps,
+                double* loss_out, double* dth_out, double* x_out, double* v_out) {
+  int r = check_ctx(c);
+  if (r) return r;
+  if (!x0 || !v0) return fail(c, RS_E_CONFIG, "null argument");
+  if ((r = forward_impl(c, x0, v0, seed, n_steps, x_out, v_out, nullptr, true))) return r;
+  Plan P;
+  if ((r = make_plan(c, n_steps, &P))) return r;
+  double acc[19];
+  HIPCHK(c, hipMemcpy(acc, c->L.b.rwacc, sizeof(acc), hipMemcpyDeviceToHost));
+  // water/water.jl:1201-1207: dl_dp/B - β (l_dE_dp/B - loss dE_dp/B), loss = Σ L_b / B
+  const double B = P.n_blocks, beta = 1.0 / kT_of(c);
+  const double loss = acc[0] / B;
+  if (loss_out) *loss_out = loss;
+  if (dth_out)
+    for (int k = 0; k < 6; ++k)
+      dth_out[k] = acc[1 + k] / B - beta * (acc[13 + k] / B - loss * acc[7 + k] / B);
+  return 0;
+}
+
+int rs_energy(rs_ctx* c, const double* x, double* U, double* dU, double* F) {
+  int r = check_ctx(c);
+  if (r) return r;
+  if (!x) return fail(c, RS_E_CONFIG, "null argument");
+  if (c->cfg.use_mlp) return fail(c, RS_E_CONFIG, "rs_energy covers the classical force field only (use_mlp = 0)");
+  if ((r = single_eval(c, x, false, [&] { launch_energy(c->L); }))) return r;
+  double e[8];
+  HIPCHK(c, hipMemcpy(e, c->L.b.eout, sizeof(e), hipMemcpyDeviceToHost));
+  if (U) *U = e[0];
+  if (dU) std::memcpy(dU, e + 1, sizeof(double) * 6);
+  if (F) HIPCHK(c, hipMemcpy(F, c->L.b.Fe, sizeof(double) * c->N3, hipMemcpyDeviceToHost));
   return 0;
 }
 

@@ -25,6 +25,7 @@ struct SysParams {
   double L[3], invL[3], qscale[3];
   float unitf[3];
   double rc2, krf, kc;
+  double crf;      // reaction-field energy shift (1/r_c) 3ε_rf/(2ε_rf + 1): pair energy zero at r_c
   float rc2f, krff;
   float rc2q;      // FP32 pair rule threshold in quantized units: (float)(rc*rc*qscale*qscale)
   // θ-derived

@@ -40,6 +40,11 @@ struct DevBufs {
   void* mlp_tab;            // (f, f') per node: float2 (FP32 path) or double2
   double* mlp_slot;         // per pair-block table gradient [blocks * MLP_NG]
   double *mlp_G, *mlp_part, *mlp_dW;
+  // potential energy (energy.hip)
+  double* Fe;      // -dU/dx of the last energy evaluation, caller order [3N]
+  double* eslots;  // per pair-block partial sums of (U, dU/dθ) [blocks * 8]
+  double* eout;    // (U, dU/dθ[6]) [8]
+  double* rwacc;   // reweighting accumulators [19]: ΣL, ΣdL/dθ, ΣdU/dθ, ΣL dU/dθ
 };
 
 struct Launch {
@@ -77,4 +82,9 @@ void launch_msd_inject(const Launch& L, int f);
 void launch_zero(const Launch& L, double* a, size_t n);
 int pair_blocks(const SysParams& p);
 void launch_mlp_tabulate(const Launch& L, double rc);
+// energy
+void launch_energy(const Launch& L);
+void launch_dhvap(const Launch& L, double w, double target, double ugas, double RT, double wgt,
+                  bool direct, bool seed);
+void launch_reweight_acc(const Launch& L, double w_rdf, bool dh);
 void launch_mlp_grad(const Launch& L, double rc);

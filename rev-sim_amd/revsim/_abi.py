@@ -11,7 +11,7 @@ import os
 
 import numpy as np
 
-RS_ABI_VERSION = 1
+RS_ABI_VERSION = 2
 RS_PRECISION_FP64 = 0
 RS_PRECISION_FP32 = 1
 
@@ -65,6 +65,9 @@ class RsConfig(ctypes.Structure):
         ("clip_norm", _f64),
         ("use_mlp", _i32),
         ("pair_chunks", _i32),
+        ("w_dhvap", _f64),
+        ("dhvap_target", _f64),
+        ("dhvap_u_gas", _f64),
         ("reserved", _i32 * 8),
     ]
 
@@ -105,6 +108,9 @@ def make_config(n_molecules: int, box, cutoff: float, **kw) -> RsConfig:
     c.clip_norm = math.inf
     c.use_mlp = 0
     c.pair_chunks = 0
+    c.w_dhvap = 0.0
+    c.dhvap_target = 44.12 + 2.8  # water/water.jl:751
+    c.dhvap_u_gas = 3.69          # water/water.jl:741
     for k, v in kw.items():
         if not hasattr(c, k):
             raise KeyError(f"unknown rs_config field {k!r}")
@@ -157,6 +163,9 @@ SIGNATURES = [
     ("rs_mlp_grad", ctypes.c_int, [_vp, _dp, ctypes.c_int64]),
     ("rs_mlp_table", ctypes.c_int, [_vp, _dp, _dp, ctypes.c_int32, _ip, _dp]),
     ("rs_forces", ctypes.c_int, [_vp, _dp, _dp]),
+    ("rs_energy", ctypes.c_int, [_vp, _dp, _dp, _dp, _dp]),
+    ("rs_reweight", ctypes.c_int,
+     [_vp, _dp, _dp, ctypes.c_uint64, ctypes.c_int32, _dp, _dp, _dp, _dp]),
     ("rs_adjoint_terms", ctypes.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp]),
     ("rs_noise", ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.c_int32, _dp]),
     ("rs_rdf", ctypes.c_int, [_vp, _dp, _dp, _lp, _dp, _dp]),

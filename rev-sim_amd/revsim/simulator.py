@@ -115,7 +115,25 @@ class Simulator:
                                              dptr(g), fptr(gm)))
         return (float(loss[0]), g) if gm is None else (float(loss[0]), g, gm)
 
+    def reweight(self, x0, v0, seed: int, n_steps: int):
+        """Ensemble-reweighting estimate of dL/dθ over one forward trajectory (reweighting_sim,
+        water/water.jl:1107-1210): returns (mean block loss, dL/dθ, x_N, v_N)."""
+        x0, v0 = self._vec(x0, self.N3), self._vec(v0, self.N3)
+        loss, g = np.zeros(1), np.zeros(6)
+        x, v = np.empty(self.N3), np.empty(self.N3)
+        self._chk(self.lib.rs_reweight(self.h, dptr(x0), dptr(v0), seed, n_steps, dptr(loss), dptr(g),
+                                       dptr(x), dptr(v)))
+        return float(loss[0]), g, x, v
+
     # -- single evaluations -----------------------------------------------------------------
+    def energy(self, x):
+        """Potential energy U(x), dU/dθ (6) and F = -dU/dx, FP64 (potential_energy,
+        water/water.jl:743,1183)."""
+        x = self._vec(x, self.N3)
+        U, dU, F = np.zeros(1), np.zeros(6), np.empty(self.N3)
+        self._chk(self.lib.rs_energy(self.h, dptr(x), dptr(U), dptr(dU), dptr(F)))
+        return float(U[0]), dU, F
+
     def forces(self, x):
         x = self._vec(x, self.N3)
         F = np.empty(self.N3)

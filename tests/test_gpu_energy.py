@@ -1,0 +1,101 @@
+"""GPU parity of the potential-energy kernel, the enthalpy-of-vaporization block loss and the
+ensemble-reweighting estimator (water/water.jl:738-754,1107-1210) against the FP64 oracle. The
+energy is FP64 in both precisions; trajectories in the FP32 path differ from the oracle's by the
+pair arithmetic's rounding, hence the looser FP32 bounds."""
+import numpy as np
+import pytest
+
+from conftest import per_atom_rel, rel_err
+
+pytestmark = pytest.mark.gpu
+
+FP64, FP32 = 0, 1
+
+
+def _pair(prec, n=216, box=1.862, rc=0.8, **kw):
+    import oracle as O
+    import revsim
+
+    sim = revsim.Simulator(revsim.make_config(n, box, rc, precision=prec, **kw), device=0)
+    sim.set_theta(revsim.DEFAULT_THETA)
+    orc = O.Oracle(revsim.make_config(n, box, rc, precision=prec, **kw))
+    orc.set_theta(revsim.DEFAULT_THETA)
+    return sim, orc
+
+
+@pytest.mark.parametrize("prec", [FP64, FP32])
+@pytest.mark.parametrize("n,box,rc", [(216, 1.862, 0.8), (3000, 4.48, 1.0)])
+def test_energy_matches_oracle(gpu_lib, prec, n, box, rc):
+    import revsim
+
+    wb = revsim.water_box(n, box)
+    rng = np.random.default_rng(7)
+    x = np.ascontiguousarray(wb.coords + 0.01 * rng.standard_normal(wb.coords.size))
+    sim, orc = _pair(prec, n=n, box=box, rc=rc)
+    U, dU, F = sim.energy(x)
+    Uo, dUo, Fo = orc.energy(x)
+    assert U == pytest.approx(Uo, rel=1e-11)
+    assert rel_err(dU, dUo) < 1e-10
+    assert per_atom_rel(F, Fo) < 1e-10
+    # the same forces as the force kernel's FP64 path
+    if prec == FP64:
+        assert per_atom_rel(sim.forces(x), F) < 1e-10
+    # deterministic: fixed-order reductions
+    U2, dU2, F2 = sim.energy(x)
+    assert U2 == U and np.array_equal(dU2, dU) and np.array_equal(F2, F)
+
+
+def test_energy_needs_the_classical_force_field(gpu_lib, box216):
+    import revsim
+
+    sim = revsim.Simulator(revsim.make_config(216, 1.862, 0.8, use_mlp=1), device=0)
+    sim.set_mlp(revsim.system.mlp_init())
+    with pytest.raises(revsim.RevSimError) as e:
+        sim.energy(box216.coords)
+    assert e.value.code == -1
+
+
+@pytest.mark.parametrize("prec,tol", [(FP64, 1e-9), (FP32, 1e-5)])
+def test_dhvap_trajectory_gradient(gpu_lib, box216, prec, tol):
+    """RDF + enthalpy-of-vaporization loss over two 10-step blocks: the enthalpy term seeds
+    dL/dx = c' F and adds its direct -c' dU/dθ at each block end."""
+    sim, orc = _pair(prec, w_dhvap=1.0, n_steps_loss=10)
+    import revsim
+
+    tgt = revsim.perturbed_target(orc.rdf(box216.coords)[0])
+    sim.set_target_rdf(tgt)
+    orc.set_target_rdf(tgt)
+    xN, vN, Lf = sim.forward(box216.coords, box216.velocities, 13, 20)
+    xo, vo, Lfo = orc.forward(box216.coords, box216.velocities, 13, 20)
+    assert abs(Lf - Lfo) <= tol * abs(Lfo)
+    L, g = sim.value_and_grad(box216.coords, box216.velocities, 13, 20)
+    Lo, go = orc.value_and_grad(box216.coords, box216.velocities, 13, 20)
+    assert abs(L - Lo) <= tol * abs(Lo)
+    assert rel_err(g, go) < tol, (g, go)
+
+
+@pytest.mark.parametrize("prec,tol", [(FP64, 1e-9), (FP32, 1e-5)])
+def test_reweight_matches_oracle(gpu_lib, box216, prec, tol):
+    """Three 6-step blocks, RDF + enthalpy losses."""
+    import revsim
+
+    sim, orc = _pair(prec, w_dhvap=0.5, n_steps_loss=6)
+    tgt = revsim.perturbed_target(orc.rdf(box216.coords)[0])
+    sim.set_target_rdf(tgt)
+    orc.set_target_rdf(tgt)
+    L, g, xN, _ = sim.reweight(box216.coords, box216.velocities, 13, 18)
+    Lo, go, xo, _ = orc.reweight(box216.coords, box216.velocities, 13, 18)
+    assert abs(L - Lo) <= tol * abs(Lo)
+    assert rel_err(g, go) < tol, (g, go)
+    d = xN - xo
+    d -= 1.862 * np.round(d / 1.862)
+    assert np.abs(d).max() < (1e-10 if prec == FP64 else 1e-5)
+
+
+def test_reweight_rejects_msd_loss(gpu_lib, box216):
+    import revsim
+
+    sim = revsim.Simulator(revsim.make_config(216, 1.862, 0.8, msd_window_steps=10, msd_every=5,
+                                              w_msd=1.0), device=0)
+    with pytest.raises(revsim.RevSimError):
+        sim.reweight(box216.coords, box216.velocities, 13, 10)
