@@ -105,6 +105,14 @@ typedef struct rs_config {
   double  w_dhvap;         /* 0 = off */
   double  dhvap_target;    /* kJ/mol (reference: 44.12 + 2.8) */
   double  dhvap_u_gas;     /* mean gas-phase U per molecule, kJ/mol (reference: 3.69) */
+  /* Monte Carlo barostat in the forward pass (MonteCarloBarostatNoUnits / apply_coupling!,
+     water/water.jl:789-865; the reference equilibrates with it): every barostat_every steps one
+     isotropic volume move dV = s (2u - 1), molecule centres of mass scaled, accepted with
+     probability min(1, exp(-(dU + P dV - M kT ln(V'/V))/kT)); s starts at 0.01 V and is
+     rescaled by 1.1 to keep acceptance in [0.25, 0.75] (capped at 0.3 V). Forward-only: a
+     trajectory that moved the box has no reverse pass. Uniforms from Philox keyed by the seed. */
+  int32_t barostat_every;  /* 0 = constant volume */
+  double  barostat_pressure; /* bar */
   int32_t reserved[8];
 } rs_config;
 
@@ -181,6 +189,11 @@ int rs_energy(rs_ctx* ctx, const double* x, double* U, double* dU_dtheta6, doubl
    The MSD loss is not part of it (RS_E_CONFIG when configured); classical force field only. */
 int rs_reweight(rs_ctx* ctx, const double* x0, const double* v0, uint64_t seed, int32_t n_steps,
                 double* loss_out, double* dL_dtheta6, double* x_out, double* v_out);
+/* Current box edges (nm): the configured box, or where the barostat left it. */
+int rs_get_box(const rs_ctx* ctx, double* box3);
+/* Replace the box for the following calls (e.g. the equilibrated box of a barostat run); the
+   state of the last forward trajectory is discarded. Same constraints as rs_create. */
+int rs_set_box(rs_ctx* ctx, const double* box3);
 /* Molecule pairs (i<j) whose O-O distance is within the list radius; pairs[2*k], pairs[2*k+1]. */
 int rs_molecule_pairs(rs_ctx* ctx, const double* x, int32_t* pairs, int64_t capacity,
                       int64_t* n_pairs);

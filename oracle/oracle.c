@@ -1251,6 +1251,92 @@ static snap_t* find_snap(or_ctx* o, int step) {
   return NULL;
 }
 
+/* New box edges (the per-box constants follow). */
+static void set_box(or_ctx* o, const double* box) {
+  for (int k = 0; k < 3; ++k) {
+    o->cfg.box[k] = box[k];
+    o->L[k] = box[k];
+    o->invL[k] = 1.0 / box[k];
+    o->qscale[k] = 4294967296.0 / box[k];
+    o->unitf[k] = (float)(box[k] / 4294967296.0);
+  }
+  o->rc2q = (float)(o->rc2 * o->qscale[0] * o->qscale[0]);
+}
+
+/* Uniforms of the barostat move at step n: Philox4x64-10, counter (n, "BARS", 0, 0), key
+   (seed, "barostat"), 53-bit mantissas. */
+static void barostat_uniforms(uint64_t seed, int n, double u[2]) {
+  uint64_t ctr[4] = {(uint64_t)(uint32_t)n, 0x42415253ULL, 0, 0}, key[2] = {seed, 0x6261726f73746174ULL}, out[4];
+  or_philox4x64(ctr, key, out);
+  u[0] = (double)(out[0] >> 11) * (1.0 / 9007199254740992.0);
+  u[1] = (double)(out[1] >> 11) * (1.0 / 9007199254740992.0);
+}
+
+typedef struct {
+  double volume_scale;
+  int n_attempted, n_accepted;
+} baro_t;
+
+/* One Monte Carlo barostat move (apply_coupling!, water/water.jl:811-865) with Molly's
+   scale_coords! for molecules: each water's centre of mass (H at the image nearest its O) scaled
+   by (V'/V)^(1/3), the molecule moved rigidly, atoms wrapped into the new box; Metropolis on
+   dW = dU + P dV - M kT ln(V'/V), P in bar. */
+static int barostat_move(or_ctx* o, baro_t* bs, uint64_t seed, int n, double* x) {
+  const int N3 = o->N3, M = o->M;
+  double u[2], E0, E1;
+  barostat_uniforms(seed, n, u);
+  or_energy(o, x, &E0, NULL, NULL);
+  const double V = o->L[0] * o->L[1] * o->L[2];
+  const double dV = bs->volume_scale * (2.0 * u[0] - 1.0);
+  const double vs = (V + dV) / V, ls = cbrt(vs);
+  double old_box[3] = {o->L[0], o->L[1], o->L[2]}, new_box[3];
+  for (int k = 0; k < 3; ++k) {
+    new_box[k] = old_box[k] * ls;
+    if (o->cfg.cutoff > 0.5 * new_box[k]) return RS_E_CONFIG;
+  }
+  double* xb = (double*)malloc(sizeof(double) * N3);
+  memcpy(xb, x, sizeof(double) * N3);
+  const double mO = o->cfg.mass_O, mH = o->cfg.mass_H, mt = mO + 2.0 * mH;
+  for (int m = 0; m < M; ++m) {
+    double* xm = x + 9 * m;
+    double a[9];
+    for (int k = 0; k < 3; ++k) {
+      a[k] = xm[k];
+      a[3 + k] = xm[k] + mic1(xm[3 + k] - xm[k], o->L[k], o->invL[k]);
+      a[6 + k] = xm[k] + mic1(xm[6 + k] - xm[k], o->L[k], o->invL[k]);
+    }
+    for (int k = 0; k < 3; ++k) {
+      const double com = (mO * a[k] + mH * a[3 + k] + mH * a[6 + k]) / mt;
+      const double sh = (ls - 1.0) * com;
+      for (int t = 0; t < 3; ++t) xm[3 * t + k] = wrap1(a[3 * t + k] + sh, new_box[k]);
+    }
+  }
+  set_box(o, new_box);
+  or_energy(o, x, &E1, NULL, NULL);
+  const double kT = o->cfg.kB * o->cfg.temperature;
+  const double P = o->cfg.barostat_pressure * 0.0602214076; /* bar -> kJ mol^-1 nm^-3 */
+  const double dW = (E1 - E0) + P * dV - M * kT * log(vs);
+  if (dW <= 0.0 || u[1] < exp(-dW / kT)) {
+    bs->n_accepted++;
+  } else {
+    memcpy(x, xb, sizeof(double) * N3);
+    set_box(o, old_box);
+  }
+  free(xb);
+  bs->n_attempted++;
+  if (bs->n_attempted >= 10) {
+    if (bs->n_accepted < 0.25 * bs->n_attempted) {
+      bs->volume_scale /= 1.1;
+      bs->n_attempted = bs->n_accepted = 0;
+    } else if (bs->n_accepted > 0.75 * bs->n_attempted) {
+      bs->volume_scale = fmin(bs->volume_scale * 1.1, V * 0.3);
+      bs->n_attempted = bs->n_accepted = 0;
+    }
+  }
+  build_list(o, x);
+  return 0;
+}
+
 /* Enthalpy-of-vaporization block loss at x (enthalpy_vaporization / loss_enth_vap,
    water/water.jl:738-754): dH = u_gas - U/M + kB T, l = w (dH - target)^2. Returns l; *cp_out =
    c' = 2 w (dH - target)/M, the factor of dl/dx = c' F and dl/dθ = -c' dU/dθ; dU_out (nullable)
@@ -1293,6 +1379,15 @@ static int forward_impl(or_ctx* o, const double* x0, const double* v0, uint64_t 
   add_snap(o, 0, x, v);
   if (traj_x) memcpy(traj_x, x, sizeof(double) * N3);
   build_list(o, x);
+  const int baro = o->cfg.barostat_every > 0;
+  if (baro && p.msd_on) {
+    free(x);
+    free(v);
+    free(F);
+    free(xi);
+    return RS_E_CONFIG;
+  }
+  baro_t bs = {0.01 * o->L[0] * o->L[1] * o->L[2], 0, 0};
   double lrdf = 0.0, ldh = 0.0;
   double* Fe = (double*)malloc(sizeof(double) * N3);
   if (rw) memset(rw, 0, sizeof(double) * 19);
@@ -1341,6 +1436,17 @@ static int forward_impl(or_ctx* o, const double* x0, const double* v0, uint64_t 
     if (f >= 0)
       for (int i = 0; i < M; ++i)
         for (int k = 0; k < 3; ++k) o->frames[(size_t)f * 3 * M + 3 * i + k] = x[9 * i + k];
+    if (baro && n % o->cfg.barostat_every == 0) {
+      int brc = barostat_move(o, &bs, seed, n, x);
+      if (brc) {
+        free(x);
+        free(v);
+        free(F);
+        free(xi);
+        free(Fe);
+        return brc;
+      }
+    }
   }
   double loss = o->cfg.w_rdf * lrdf / p.n_blocks + ldh / p.n_blocks;
   free(Fe);
@@ -1349,7 +1455,7 @@ static int forward_impl(or_ctx* o, const double* x0, const double* v0, uint64_t 
     or_msd(o, o->frames, p.nf, NULL, &lm, NULL);
     loss += lm;
   }
-  o->rec_steps = n_steps;
+  o->rec_steps = baro ? 0 : n_steps; /* a trajectory that moved the box has no reverse pass */
   o->rec_seed = seed;
   if (x_out) memcpy(x_out, x, sizeof(double) * N3);
   if (v_out) memcpy(v_out, v, sizeof(double) * N3);
@@ -1627,6 +1733,11 @@ int or_reweight(or_ctx* o, const double* x0, const double* v0, uint64_t seed, in
 }
 
 double or_list_radius(const or_ctx* o) { return o->rlist; }
+
+int or_get_box(const or_ctx* o, double* box) {
+  for (int k = 0; k < 3; ++k) box[k] = o->L[k];
+  return 0;
+}
 
 /* dL/dW of the last reverse pass (or_reverse / or_value_and_grad / or_value_and_grad_stored). */
 int or_mlp_grad(const or_ctx* o, double* dW) {

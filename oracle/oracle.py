@@ -50,6 +50,7 @@ _SIGS = [
     ("or_mlp_grad", ctypes.c_int, [_vp, _dp]),
     ("or_num_threads", ctypes.c_int, []),
     ("or_energy", ctypes.c_int, [_vp, _dp, _dp, _dp, _dp]),
+    ("or_get_box", ctypes.c_int, [_vp, _dp]),
     ("or_reweight", ctypes.c_int, [_vp, _dp, _dp, _u64, ctypes.c_int32, _dp, _dp, _dp, _dp]),
     ("or_exp", ctypes.c_double, [ctypes.c_double]),
     ("or_philox4x64", None, [_P(ctypes.c_uint64), _P(ctypes.c_uint64), _P(ctypes.c_uint64)]),
@@ -208,6 +209,11 @@ class Oracle:
         self._chk(self.L.or_reweight(self.h, _d(x0), _d(v0), seed, n_steps, _d(loss), _d(g), _d(x),
                                      _d(v)))
         return float(loss[0]), g, x, v
+
+    def get_box(self):
+        b = np.zeros(3)
+        self._chk(self.L.or_get_box(self.h, _d(b)))
+        return b
 
     def list_radius(self):
         return self.L.or_list_radius(self.h)

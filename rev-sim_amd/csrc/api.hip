@@ -54,6 +54,8 @@ struct rs_ctx {
   int ext_steps = 0;
   int since_build = 0;
   double hext_extra = 0.0;  // added to the O-H extent after an extent violation
+  int cell_cap = 0;         // allocated length of the per-cell arrays (>= ncell + 1)
+  int64_t baro_attempted = 0, baro_accepted = 0;  // barostat moves of the last forward trajectory
   bool mlp_set = false;     // weights uploaded and tabulated
   std::string err;
   // timing: events recorded around launches on the context's stream
@@ -185,6 +187,7 @@ int alloc_all(rs_ctx* c) {
   if ((r = dalloc(c, &b.vjb, 3 * M))) return r;
   for (auto q : st)
     if ((r = dalloc(c, q, N3))) return r;
+  c->cell_cap = p.ncell + 1;
   if ((r = dalloc(c, &b.cell, M)) || (r = dalloc(c, &b.cell_count, p.ncell + 1)) ||
       (r = dalloc(c, &b.cell_start, p.ncell + 1)) || (r = dalloc(c, &b.cell_fill, p.ncell + 1)) ||
       (r = dalloc(c, &b.tmp, M)) || (r = dalloc(c, &b.order, M)) || (r = dalloc(c, &b.rank, M)))
@@ -204,6 +207,7 @@ int alloc_all(rs_ctx* c) {
   if ((r = dalloc(c, &b.Fe, N3)) || (r = dalloc(c, &b.eslots, (size_t)pair_blocks(p) * 8)) ||
       (r = dalloc(c, &b.eout, 8)) || (r = dalloc(c, &b.rwacc, 32)))
     return r;
+  if (c->cfg.barostat_every > 0 && (r = dalloc(c, &b.xbak, N3))) return r;
   if (c->cfg.use_mlp) {
     if ((r = dalloc(c, &b.mlp_w, MLP_NW)) || (r = dalloc(c, &b.mlp_wb, MLP_NW)) ||
         (r = dalloc(c, (double2**)&b.mlp_tab, MLP_NT)) ||
@@ -212,6 +216,131 @@ int alloc_all(rs_ctx* c) {
       return r;
   }
   return alloc_list(c);
+}
+
+// The per-cell arrays follow the cell grid, which a box change can enlarge.
+int ensure_cells(rs_ctx* c) {
+  const int need = c->L.p.ncell + 1;
+  if (need <= c->cell_cap) return 0;
+  DevBufs& b = c->L.b;
+  hipFree(b.cell_count);
+  hipFree(b.cell_start);
+  hipFree(b.cell_fill);
+  b.cell_count = b.cell_start = b.cell_fill = nullptr;
+  int r;
+  if ((r = dalloc(c, &b.cell_count, need)) || (r = dalloc(c, &b.cell_start, need)) ||
+      (r = dalloc(c, &b.cell_fill, need)))
+    return r;
+  c->cell_cap = need;
+  return 0;
+}
+
+int check_box(rs_ctx* c, const double* box) {
+  for (int k = 0; k < 3; ++k)
+    if (!(box[k] > 0.0) || !std::isfinite(box[k]) || c->cfg.cutoff > 0.5 * box[k])
+      return fail(c, RS_E_CONFIG, "cutoff must be <= half of every box edge");
+  if (c->cfg.precision == RS_PRECISION_FP32 && !(box[0] == box[1] && box[1] == box[2]))
+    return fail(c, RS_E_CONFIG, "RS_PRECISION_FP32 needs a cubic box");
+  return 0;
+}
+
+// New box edges: parameters, cell grid and list radius follow (the caller rebuilds the list).
+int change_box(rs_ctx* c, const double* box) {
+  int r = check_box(c, box);
+  if (r) return r;
+  for (int k = 0; k < 3; ++k) c->cfg.box[k] = box[k];
+  set_params(c);
+  return ensure_cells(c);
+}
+
+// Philox4x64-10 on the host (the stream the device uses for the noise, another key): the
+// barostat's uniforms of step n are a pure function of (seed, n).
+void philox_host(uint64_t ctr[4], uint64_t k0, uint64_t k1) {
+  const uint64_t M0 = 0xD2E7470EE14C6C93ULL, M1 = 0xCA5A826395121157ULL;
+  const uint64_t W0 = 0x9E3779B97F4A7C15ULL, W1 = 0xBB67AE8584CAA73BULL;
+  for (int r = 0; r < 10; ++r) {
+    const unsigned __int128 p0 = (unsigned __int128)M0 * ctr[0], p1 = (unsigned __int128)M1 * ctr[2];
+    const uint64_t hi0 = (uint64_t)(p0 >> 64), lo0 = (uint64_t)p0;
+    const uint64_t hi1 = (uint64_t)(p1 >> 64), lo1 = (uint64_t)p1;
+    const uint64_t n0 = hi1 ^ ctr[1] ^ k0, n2 = hi0 ^ ctr[3] ^ k1;
+    ctr[0] = n0;
+    ctr[1] = lo1;
+    ctr[2] = n2;
+    ctr[3] = lo0;
+    k0 += W0;
+    k1 += W1;
+  }
+}
+
+void barostat_uniforms(uint64_t seed, int step, double u[2]) {
+  uint64_t ctr[4] = {(uint64_t)(uint32_t)step, 0x42415253ULL, 0, 0};
+  philox_host(ctr, seed, 0x6261726f73746174ULL);
+  u[0] = (double)(ctr[0] >> 11) * (1.0 / 9007199254740992.0);
+  u[1] = (double)(ctr[1] >> 11) * (1.0 / 9007199254740992.0);
+}
+
+// Adaptive state of one trajectory's barostat (MonteCarloBarostatNoUnits, water/water.jl:789-807:
+// scale_factor 0.01, scale_increment 1.1, max_volume_frac 0.3, one iteration per call).
+struct BarostatState {
+  double volume_scale;
+  int n_attempted = 0, n_accepted = 0;
+  int64_t total_attempted = 0, total_accepted = 0;
+};
+
+double box_volume(const rs_ctx* c) { return c->cfg.box[0] * c->cfg.box[1] * c->cfg.box[2]; }
+double kT_of(const rs_ctx* c);
+void rebuild(rs_ctx* c, bool with_u);
+
+// One barostat move at step n (apply_coupling!, water/water.jl:811-865): U, a trial volume
+// V + dV with dV = s (2u - 1), molecule centres scaled by (V'/V)^(1/3), the list rebuilt at the
+// trial state (the reference's trial_find_neighbors = true form), U', and Metropolis on
+// dW = dU + P dV - M kT ln(V'/V).
+int barostat_move(rs_ctx* c, BarostatState& bs, uint64_t seed, int n) {
+  DevBufs& b = c->L.b;
+  const rs_config& g = c->cfg;
+  double u[2], e0[8], e1[8];
+  barostat_uniforms(seed, n, u);
+  launch_energy(c->L);
+  HIPCHK(c, hipMemcpyAsync(e0, b.eout, sizeof(e0), hipMemcpyDeviceToHost, c->s));
+  HIPCHK(c, hipMemcpyAsync(b.xbak, b.x, sizeof(double) * c->N3, hipMemcpyDeviceToDevice, c->s));
+  HIPCHK(c, hipStreamSynchronize(c->s));
+  const double V = box_volume(c);
+  const double dV = bs.volume_scale * (2.0 * u[0] - 1.0);
+  const double vs = (V + dV) / V, ls = std::cbrt(vs);
+  const double old_box[3] = {g.box[0], g.box[1], g.box[2]};
+  const double new_box[3] = {old_box[0] * ls, old_box[1] * ls, old_box[2] * ls};
+  int r = check_box(c, new_box);
+  if (r) return fail(c, RS_E_CONFIG, "the barostat shrank the box below twice the cutoff");
+  launch_scale_molecules(c->L, ls, new_box);
+  if ((r = change_box(c, new_box))) return r;
+  rebuild(c, false);
+  launch_energy(c->L);
+  HIPCHK(c, hipMemcpyAsync(e1, b.eout, sizeof(e1), hipMemcpyDeviceToHost, c->s));
+  HIPCHK(c, hipStreamSynchronize(c->s));
+  const double kT = kT_of(c);
+  const double P = g.barostat_pressure * 0.0602214076;  // bar -> kJ mol^-1 nm^-3
+  const double dW = (e1[0] - e0[0]) + P * dV - c->M * kT * std::log(vs);
+  const bool accept = dW <= 0.0 || u[1] < std::exp(-dW / kT);
+  if (accept) {
+    ++bs.n_accepted;
+    ++bs.total_accepted;
+  } else {
+    HIPCHK(c, hipMemcpyAsync(b.x, b.xbak, sizeof(double) * c->N3, hipMemcpyDeviceToDevice, c->s));
+    if ((r = change_box(c, old_box))) return r;
+    rebuild(c, false);
+  }
+  ++bs.n_attempted;
+  ++bs.total_attempted;
+  if (bs.n_attempted >= 10) {
+    if (bs.n_accepted < 0.25 * bs.n_attempted) {
+      bs.volume_scale /= 1.1;
+      bs.n_attempted = bs.n_accepted = 0;
+    } else if (bs.n_accepted > 0.75 * bs.n_attempted) {
+      bs.volume_scale = std::min(bs.volume_scale * 1.1, V * 0.3);
+      bs.n_attempted = bs.n_accepted = 0;
+    }
+  }
+  return 0;
 }
 
 void free_records(rs_ctx* c) {
@@ -358,10 +487,15 @@ int forward_impl(rs_ctx* c, const double* x0, const double* v0, uint64_t seed, i
     return fail(c, RS_E_CONFIG, "the reweighting estimator takes block losses only: set w_msd = 0");
   if (reweight && c->cfg.use_mlp)
     return fail(c, RS_E_CONFIG, "the reweighting estimator needs the classical force field (use_mlp = 0)");
+  const bool baro = g.barostat_every > 0;
+  if (baro && P.msd_on) return fail(c, RS_E_CONFIG, "the MSD loss needs a constant box (barostat_every = 0)");
+  if (baro && c->cfg.use_mlp) return fail(c, RS_E_CONFIG, "the barostat needs the classical force field (use_mlp = 0)");
+  const double box0[3] = {g.box[0], g.box[1], g.box[2]};
+  BarostatState bs;
   if (P.msd_on && (r = ensure_frames(c, P.nf))) return r;
   const double RT = kT;
   const bool dh = g.w_dhvap != 0.0;
-  for (int t = 0; t <= n_steps && !reweight; ++t)
+  for (int t = 0; t <= n_steps && !reweight && !baro; ++t)
     if (P.snap_at(t)) {
       double *sx, *sv;
       if ((r = dalloc(c, &sx, c->N3)) || (r = dalloc(c, &sv, c->N3))) return r;
@@ -370,6 +504,12 @@ int forward_impl(rs_ctx* c, const double* x0, const double* v0, uint64_t seed, i
       c->snap_v.push_back(sv);
     }
   for (int attempt = 0;; ++attempt) {
+    if (baro) {
+      // every attempt starts from the call's box with a fresh barostat (water/water.jl:1133-1139)
+      if ((r = change_box(c, box0))) return r;
+      bs = BarostatState();
+      bs.volume_scale = 0.01 * box_volume(c);
+    }
     HIPCHK(c, hipMemcpyAsync(b.x, x0, sizeof(double) * c->N3, hipMemcpyHostToDevice, c->s));
     HIPCHK(c, hipMemcpyAsync(b.v, v0, sizeof(double) * c->N3, hipMemcpyHostToDevice, c->s));
     if ((r = reset_flags(c))) return r;
@@ -413,6 +553,7 @@ int forward_impl(rs_ctx* c, const double* x0, const double* v0, uint64_t seed, i
       }
       int f = P.frame_of(n);
       if (f >= 0) launch_frame_copy(c->L, f);
+      if (baro && n % g.barostat_every == 0 && (r = barostat_move(c, bs, seed, n))) return r;
     }
     if (P.msd_on) {
       tbegin(c, RS_PHASE_LOSS);
@@ -433,7 +574,9 @@ int forward_impl(rs_ctx* c, const double* x0, const double* v0, uint64_t seed, i
   if (x_out) HIPCHK(c, hipMemcpy(x_out, b.x, sizeof(double) * c->N3, hipMemcpyDeviceToHost));
   if (v_out) HIPCHK(c, hipMemcpy(v_out, b.v, sizeof(double) * c->N3, hipMemcpyDeviceToHost));
   if (loss_out) *loss_out = sc[0];
-  c->rec_valid = !reweight;
+  c->rec_valid = !reweight && !baro;
+  c->baro_attempted = baro ? bs.total_attempted : 0;
+  c->baro_accepted = baro ? bs.total_accepted : 0;
   c->rec_steps = n_steps;
   c->rec_seed = seed;
   return 0;
@@ -655,6 +798,8 @@ int rs_create(const rs_config* cfg, int32_t device, rs_ctx** out) {
       !(g.rdf_window > 0.0))
     return fail(nullptr, RS_E_CONFIG, "rdf_bins in [1,1024] and positive rdf_rmax/width/window required");
   if (g.use_mlp != 0 && g.use_mlp != 1) return fail(nullptr, RS_E_CONFIG, "use_mlp must be 0 or 1");
+  if (g.barostat_every < 0 || !std::isfinite(g.barostat_pressure))
+    return fail(nullptr, RS_E_CONFIG, "barostat_every must be >= 0 and barostat_pressure finite");
   if (g.use_mlp && g.w_dhvap != 0.0)
     return fail(nullptr, RS_E_CONFIG, "the enthalpy-of-vaporization loss needs the classical force field (use_mlp = 0)");
   if (!std::isfinite(g.w_dhvap) || !std::isfinite(g.dhvap_target) || !std::isfinite(g.dhvap_u_gas))
@@ -712,7 +857,7 @@ int rs_destroy(rs_ctx* c) {
                   b.ud, b.O64, b.Oq, b.nbr, b.nbr_tmp, b.ncnt, b.flags, b.slots, b.vjb, b.dth, b.scal, b.wmax, b.hist, b.rdfG,
                   b.target, b.rdf_g, b.frames, b.msd_del, b.msd_acc, b.msd_grad, b.ext_noise,
                   b.hpart, b.mlp_w, b.mlp_wb, b.mlp_tab, b.mlp_slot, b.mlp_G, b.mlp_part, b.mlp_dW,
-                  b.Fe, b.eslots, b.eout, b.rwacc};
+                  b.Fe, b.eslots, b.eout, b.rwacc, b.xbak};
   for (void* q : ptrs)
     if (q) hipFree(q);
   for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
@@ -1033,6 +1178,23 @@ int rs_molecule_pairs(rs_ctx* c, const double* x, int32_t* pairs, int64_t capaci
 }
 
 double rs_list_radius(const rs_ctx* c) { return c ? c->rlist : 0.0; }
+
+int rs_get_box(const rs_ctx* c, double* box) {
+  if (!c || !box) return fail(nullptr, RS_E_CONFIG, "null argument");
+  for (int k = 0; k < 3; ++k) box[k] = c->cfg.box[k];
+  return 0;
+}
+
+int rs_set_box(rs_ctx* c, const double* box) {
+  int r = check_ctx(c);
+  if (r) return r;
+  if (!box) return fail(c, RS_E_CONFIG, "null argument");
+  if ((r = change_box(c, box))) return r;
+  c->L.p.capc = std::max(c->L.p.capc, default_capc(c));
+  if ((r = alloc_list(c))) return r;
+  c->rec_valid = false;
+  return 0;
+}
 
 int rs_timings(rs_ctx* c, double* ms, int32_t n) {
   int r = check_ctx(c);

@@ -199,3 +199,37 @@ void launch_dhvap(const Launch& L, double w, double target, double ugas, double 
 void launch_reweight_acc(const Launch& L, double w_rdf, bool dh) {
   k_reweight_acc<<<1, 64, 0, L.s>>>(L.b.scal, L.b.eout, w_rdf, dh ? 1 : 0, L.b.rwacc);
 }
+
+// ------------------------------------------------------------------ Monte Carlo barostat move
+
+namespace {
+
+// Molly's scale_coords! for a system with molecules (called by the reference's barostat,
+// water/water.jl:831): each water's centre of mass, with its H atoms taken at the image nearest
+// their O in the old box, is scaled by l; the molecule moves rigidly with it and its atoms are
+// wrapped into the new box (edges Ln).
+__global__ void k_scale_molecules(SysParams p, double* x, double l, double Ln0, double Ln1, double Ln2) {
+#pragma clang fp contract(off)
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= p.M) return;
+  double* xm = x + 9 * (size_t)m;
+  const double Ln[3] = {Ln0, Ln1, Ln2};
+  double a[9];
+  for (int k = 0; k < 3; ++k) {
+    a[k] = xm[k];
+    a[3 + k] = xm[k] + mic1(xm[3 + k] - xm[k], p.L[k], p.invL[k]);
+    a[6 + k] = xm[k] + mic1(xm[6 + k] - xm[k], p.L[k], p.invL[k]);
+  }
+  const double mt = p.mO + 2.0 * p.mH;
+  for (int k = 0; k < 3; ++k) {
+    const double com = (p.mO * a[k] + p.mH * a[3 + k] + p.mH * a[6 + k]) / mt;
+    const double sh = (l - 1.0) * com;
+    for (int t = 0; t < 3; ++t) xm[3 * t + k] = wrap1(a[3 * t + k] + sh, Ln[k]);
+  }
+}
+
+}  // namespace
+
+void launch_scale_molecules(const Launch& L, double l, const double* Lnew) {
+  k_scale_molecules<<<(L.p.M + RS_BLOCK - 1) / RS_BLOCK, RS_BLOCK, 0, L.s>>>(L.p, L.b.x, l, Lnew[0], Lnew[1], Lnew[2]);
+}

@@ -45,6 +45,7 @@ struct DevBufs {
   double* eslots;  // per pair-block partial sums of (U, dU/dθ) [blocks * 8]
   double* eout;    // (U, dU/dθ[6]) [8]
   double* rwacc;   // reweighting accumulators [19]: ΣL, ΣdL/dθ, ΣdU/dθ, ΣL dU/dθ
+  double* xbak;    // positions before a barostat move [3N]
 };
 
 struct Launch {
@@ -87,4 +88,5 @@ void launch_energy(const Launch& L);
 void launch_dhvap(const Launch& L, double w, double target, double ugas, double RT, double wgt,
                   bool direct, bool seed);
 void launch_reweight_acc(const Launch& L, double w_rdf, bool dh);
+void launch_scale_molecules(const Launch& L, double l, const double* Lnew);
 void launch_mlp_grad(const Launch& L, double rc);

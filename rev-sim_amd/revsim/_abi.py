@@ -68,6 +68,8 @@ class RsConfig(ctypes.Structure):
         ("w_dhvap", _f64),
         ("dhvap_target", _f64),
         ("dhvap_u_gas", _f64),
+        ("barostat_every", _i32),
+        ("barostat_pressure", _f64),
         ("reserved", _i32 * 8),
     ]
 
@@ -111,6 +113,8 @@ def make_config(n_molecules: int, box, cutoff: float, **kw) -> RsConfig:
     c.w_dhvap = 0.0
     c.dhvap_target = 44.12 + 2.8  # water/water.jl:751
     c.dhvap_u_gas = 3.69          # water/water.jl:741
+    c.barostat_every = 0
+    c.barostat_pressure = 1.0     # bar (water/water.jl:1136)
     for k, v in kw.items():
         if not hasattr(c, k):
             raise KeyError(f"unknown rs_config field {k!r}")
@@ -172,6 +176,8 @@ SIGNATURES = [
     ("rs_msd", ctypes.c_int, [_vp, _dp, ctypes.c_int32, _dp, _dp, _dp]),
     ("rs_molecule_pairs", ctypes.c_int, [_vp, _dp, _ip, ctypes.c_int64, _lp]),
     ("rs_list_radius", ctypes.c_double, [_vp]),
+    ("rs_get_box", ctypes.c_int, [_vp, _dp]),
+    ("rs_set_box", ctypes.c_int, [_vp, _dp]),
     ("rs_timings", ctypes.c_int, [_vp, _dp, ctypes.c_int32]),
     ("rs_timing_counts", ctypes.c_int, [_vp, _lp, ctypes.c_int32]),
     ("rs_enable_timing", ctypes.c_int, [_vp, ctypes.c_int32]),

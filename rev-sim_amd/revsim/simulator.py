@@ -175,6 +175,16 @@ class Simulator:
                                              ctypes.byref(n)))
         return pairs[: 2 * n.value].reshape(-1, 2)
 
+    def get_box(self) -> np.ndarray:
+        """Current box edges (nm): where the barostat left it after a forward trajectory."""
+        b = np.zeros(3)
+        self._chk(self.lib.rs_get_box(self.h, dptr(b)))
+        return b
+
+    def set_box(self, box) -> None:
+        b = np.ascontiguousarray(np.broadcast_to(np.asarray(box, dtype=np.float64), (3,)))
+        self._chk(self.lib.rs_set_box(self.h, dptr(b)))
+
     def list_radius(self) -> float:
         return float(self.lib.rs_list_radius(self.h))
 

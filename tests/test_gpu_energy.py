@@ -99,3 +99,46 @@ def test_reweight_rejects_msd_loss(gpu_lib, box216):
                                               w_msd=1.0), device=0)
     with pytest.raises(revsim.RevSimError):
         sim.reweight(box216.coords, box216.velocities, 13, 10)
+
+
+def test_barostat_matches_oracle(gpu_lib, box216):
+    """FP64 forward trajectory with a Monte Carlo barostat move every 10 steps (water/water.jl:
+    811-865): the same accepted volumes as the oracle (same Philox uniforms, FP64 energies) and
+    the same final state; the FP32 path ends in the same box within the pair arithmetic's noise."""
+    import revsim
+
+    boxes = {}
+    for prec in (FP64, FP32):
+        sim, orc = _pair(prec, barostat_every=10, barostat_pressure=1.0, w_rdf=0.0)
+        assert np.array_equal(sim.get_box(), [1.862] * 3)
+        x, v, _ = sim.forward(box216.coords, box216.velocities, 13, 60)
+        xo, vo, _ = orc.forward(box216.coords, box216.velocities, 13, 60)
+        boxes[prec] = sim.get_box()
+        if prec == FP64:
+            assert np.allclose(boxes[prec], orc.get_box(), rtol=1e-13, atol=0)
+            d = (x - xo).reshape(-1, 3)
+            d -= boxes[prec] * np.round(d / boxes[prec])
+            assert np.abs(d).max() < 1e-9
+            assert rel_err(v, vo) < 1e-9
+        # a trajectory that moved the box has no reverse pass
+        with pytest.raises(revsim.RevSimError) as e:
+            sim.reverse(x, v, 13, 60)
+        assert e.value.code == -6
+    assert not np.allclose(boxes[FP64], 1.862)
+    assert np.allclose(boxes[FP32], boxes[FP64], rtol=1e-3)
+
+
+def test_set_box(gpu_lib, jittered216):
+    """rs_set_box: the forces at the new box equal the oracle's at that box."""
+    import oracle as O
+    import revsim
+
+    sim, _ = _pair(FP64)
+    sim.set_box([1.9] * 3)
+    assert np.array_equal(sim.get_box(), [1.9] * 3)
+    x = np.mod(jittered216 * (1.9 / 1.862), 1.9)
+    orc = O.Oracle(revsim.make_config(216, 1.9, 0.8, precision=FP64))
+    orc.set_theta(revsim.DEFAULT_THETA)
+    assert per_atom_rel(sim.forces(x), orc.forces(x)) < 1e-10
+    with pytest.raises(revsim.RevSimError):
+        sim.set_box([1.5] * 3)  # below twice the cutoff
