@@ -25,11 +25,6 @@ __device__ __forceinline__ int cell_index(const SysParams& p, double x, double y
   return (c[2] * p.nc[1] + c[1]) * p.nc[0] + c[0];
 }
 
-__global__ void k_zero_int(int* a, int n) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n) a[t] = 0;
-}
-
 // Cell of each molecule's O and the per-cell counts. Counts go through a block-private LDS
 // histogram (molecules of one block share few cells) and one global atomic per nonzero bin.
 constexpr int LDS_CELLS = 8192;
@@ -54,8 +49,9 @@ __global__ __launch_bounds__(RS_BLOCK) void k_cells_assign(SysParams p, const do
       if (hist[c]) atomicAdd(&count[c], hist[c]);
 }
 
-// Exclusive scan of cell counts (one block); also clears the fill counters.
-__global__ void k_cells_scan(int ncell, const int* count, int* start, int* fill) {
+// Exclusive scan of cell counts (one block); clears the fill counters, and the counts for the
+// next build (they start at zero when allocated).
+__global__ void k_cells_scan(int ncell, int* count, int* start, int* fill) {
   __shared__ int part[RS_BLOCK];
   int t = threadIdx.x;
   int per = (ncell + RS_BLOCK - 1) / RS_BLOCK;
@@ -79,6 +75,7 @@ __global__ void k_cells_scan(int ncell, const int* count, int* start, int* fill)
     start[c] = acc;
     fill[c] = 0;
     acc += count[c];
+    count[c] = 0;
   }
 }
 
@@ -268,6 +265,130 @@ __global__ __launch_bounds__(RS_BLOCK) void k_list_build(SysParams p, const int*
   }
 }
 
+// Cell-centric build (the cell-stencil path): one block per home cell. The stencil's quantized O
+// positions are staged in LDS in chunks, in stencil order (rows (dy, dz), then cells along x, then
+// sorted index); each wave takes the cell's molecules in turn and tests 64 staged candidates per
+// iteration. A wave ballot ranks the accepted ones in stencil order, so candidate g of molecule i
+// goes straight to its dealt place, sub-list g % CH, slot g / CH, of the pair kernels' ELL-by-wave
+// layout: no scratch, deterministic, sub-lists balanced to within one entry.
+constexpr int LC_CHUNK = 1024;  // staged candidates per pass
+constexpr int LC_MAXH = 512;    // molecules per home cell
+constexpr int LC_MAXR = 64;     // stencil ranges: (2 cs + 1)^2 rows, at most two each
+
+template <int CH>
+__global__ __launch_bounds__(RS_BLOCK) void k_list_cells(SysParams p, const int* __restrict__ start,
+                                                         const double4* __restrict__ O64,
+                                                         const int4* __restrict__ Oq, int* nbr,
+                                                         int* ncnt, int* flags) {
+  __shared__ int4 sq[LC_CHUNK];
+  __shared__ int sj[LC_CHUNK];
+  __shared__ int rbeg[LC_MAXR], roff[LC_MAXR + 1];
+  __shared__ int Ts[LC_MAXH];
+  __shared__ int nr;
+  const int h = blockIdx.x;
+  const int h0 = start[h], nh = start[h + 1] - h0;
+  if (nh == 0) return;  // uniform for the block
+  if (nh > LC_MAXH) {
+    if (threadIdx.x == 0) atomicOr(&flags[FLAG_LIST_OVERFLOW], 2);
+    return;
+  }
+  if (threadIdx.x == 0) {
+    const int cx = h % p.nc[0], cy = (h / p.nc[0]) % p.nc[1], cz = h / (p.nc[0] * p.nc[1]);
+    const int S = p.cs;
+    int k = 0, off = 0;
+    auto add = [&](int e0, int e1) {  // cells e0..e1 inclusive of one row: one molecule range
+      rbeg[k] = start[e0];
+      roff[k] = off;
+      off += start[e1 + 1] - start[e0];
+      ++k;
+    };
+    for (int dz = -S; dz <= S; ++dz)
+      for (int dy = -S; dy <= S; ++dy) {
+        int ey = cy + dy, ez = cz + dz;
+        ey += (ey < 0) ? p.nc[1] : (ey >= p.nc[1] ? -p.nc[1] : 0);
+        ez += (ez < 0) ? p.nc[2] : (ez >= p.nc[2] ? -p.nc[2] : 0);
+        const int row = (ez * p.nc[1] + ey) * p.nc[0];
+        const int x0 = cx - S, x1 = cx + S;
+        if (x0 < 0) {
+          add(row + x0 + p.nc[0], row + p.nc[0] - 1);
+          add(row, row + x1);
+        } else if (x1 >= p.nc[0]) {
+          add(row + x0, row + p.nc[0] - 1);
+          add(row, row + x1 - p.nc[0]);
+        } else {
+          add(row + x0, row + x1);
+        }
+      }
+    roff[k] = off;
+    nr = k;
+  }
+  for (int m = threadIdx.x; m < nh; m += RS_BLOCK) Ts[m] = 0;
+  __syncthreads();
+  const int total = roff[nr];
+  const int wave = threadIdx.x / RS_WAVE, lane = threadIdx.x & (RS_WAVE - 1);
+  const unsigned long long below = (1ull << lane) - 1ull;
+  for (int base = 0; base < total; base += LC_CHUNK) {
+    const int nchunk = min(LC_CHUNK, total - base);
+    for (int t = threadIdx.x; t < nchunk; t += RS_BLOCK) {
+      const int gi = base + t;
+      int r = 0;
+      while (roff[r + 1] <= gi) ++r;
+      const int j = rbeg[r] + (gi - roff[r]);
+      sj[t] = j;
+      sq[t] = Oq[j];
+    }
+    __syncthreads();
+    for (int m = wave; m < nh; m += RS_BLOCK / RS_WAVE) {
+      const int i = h0 + m;
+      const int4 qi = Oq[i];
+      const double4 oi = O64[i];
+      const size_t g0 = (size_t)i * CH;
+      int* out = nbr + (g0 / RS_WAVE) * p.capc * RS_WAVE + (g0 & (RS_WAVE - 1));
+      int T = Ts[m];
+      for (int c0 = 0; c0 < nchunk; c0 += RS_WAVE) {
+        const int t = c0 + lane;
+        bool in = false;
+        int j = -1;
+        if (t < nchunk) {
+          j = sj[t];
+          if (j != i) {
+            // FP32 screen on quantized positions; the FP64 rule decides inside the uncertainty band
+            const int4 qj = sq[t];
+            const float r2f = r2f_of(qdiff((uint32_t)qi.x, (uint32_t)qj.x, p.unitf[0]),
+                                     qdiff((uint32_t)qi.y, (uint32_t)qj.y, p.unitf[1]),
+                                     qdiff((uint32_t)qi.z, (uint32_t)qj.z, p.unitf[2]));
+            in = r2f < p.rlo2f;
+            if (!in && r2f <= p.rhi2f) {
+              const double4 oj = O64[j];
+              const double dx = mic1(oj.x - oi.x, p.L[0], p.invL[0]);
+              const double dy = mic1(oj.y - oi.y, p.L[1], p.invL[1]);
+              const double dz = mic1(oj.z - oi.z, p.L[2], p.invL[2]);
+              in = r2_of(dx, dy, dz) <= p.rlist2;
+            }
+          }
+        }
+        const unsigned long long bal = __ballot(in);
+        if (in) {
+          const int g = T + __popcll(bal & below);
+          const int slot = g / CH;
+          if (slot < p.capc) out[(size_t)slot * RS_WAVE + (g % CH)] = j;
+        }
+        T += __popcll(bal);
+      }
+      if (lane == 0) Ts[m] = T;
+    }
+    __syncthreads();
+  }
+  for (int m = wave; m < nh; m += RS_BLOCK / RS_WAVE) {
+    const int i = h0 + m, T = Ts[m];
+    if (lane < CH) {
+      int mine = (T - lane + CH - 1) / CH;
+      ncnt[(size_t)i * CH + lane] = mine < p.capc ? mine : p.capc;
+    }
+    if (lane == 0 && T > CH * p.capc) atomicOr(&flags[FLAG_LIST_OVERFLOW], 1);
+  }
+}
+
 }  // namespace
 
 int pair_blocks(const SysParams& p) { return (p.M * p.CH + RS_BLOCK - 1) / RS_BLOCK; }
@@ -276,7 +397,6 @@ void launch_list_build(const Launch& L, bool with_u) {
   const SysParams& p = L.p;
   const DevBufs& b = L.b;
   int gm = (p.M + RS_BLOCK - 1) / RS_BLOCK;
-  k_zero_int<<<(p.ncell + 1 + RS_BLOCK - 1) / RS_BLOCK, RS_BLOCK, 0, L.s>>>(b.cell_count, p.ncell + 1);
   k_cells_assign<<<gm, RS_BLOCK, 0, L.s>>>(p, b.x, b.cell, b.cell_count);
   k_cells_scan<<<1, RS_BLOCK, 0, L.s>>>(p.ncell, b.cell_count, b.cell_start, b.cell_fill);
   k_cells_scatter<<<gm, RS_BLOCK, 0, L.s>>>(p, b.cell, b.cell_start, b.cell_fill, b.tmp);
@@ -285,6 +405,15 @@ void launch_list_build(const Launch& L, bool with_u) {
   int ga = (p.N + RS_BLOCK - 1) / RS_BLOCK;
   k_pack<<<ga, RS_BLOCK, 0, L.s>>>(p, L.precision, b.order, b.x, with_u ? b.C : nullptr, b.posq,
                                     b.pos64, b.uf, b.ud, b.O64, b.Oq);
+  if (!p.brute) {
+    switch (p.CH) {
+      case 4: k_list_cells<4><<<p.ncell, RS_BLOCK, 0, L.s>>>(p, b.cell_start, b.O64, b.Oq, b.nbr, b.ncnt, b.flags); break;
+      case 8: k_list_cells<8><<<p.ncell, RS_BLOCK, 0, L.s>>>(p, b.cell_start, b.O64, b.Oq, b.nbr, b.ncnt, b.flags); break;
+      case 16: k_list_cells<16><<<p.ncell, RS_BLOCK, 0, L.s>>>(p, b.cell_start, b.O64, b.Oq, b.nbr, b.ncnt, b.flags); break;
+      default: k_list_cells<32><<<p.ncell, RS_BLOCK, 0, L.s>>>(p, b.cell_start, b.O64, b.Oq, b.nbr, b.ncnt, b.flags); break;
+    }
+    return;
+  }
   int gl = pair_blocks(p);
   switch (p.CH) {
     case 4: k_list_build<4><<<gl, RS_BLOCK, 0, L.s>>>(p, b.cell_start, b.O64, b.Oq, b.nbr_tmp, b.nbr, b.ncnt, b.flags); break;
