@@ -128,6 +128,11 @@ void set_params(rs_ctx* c) {
   // component), so a 1e-5 band around r_list^2 leaves every decision to the FP64 rule's outcome.
   p.rlo2f = (float)(p.rlist2 * (1.0 - 1e-5));
   p.rhi2f = (float)(p.rlist2 * (1.0 + 1e-5));
+  // quantized units share one length scale only in a cubic box; otherwise every candidate goes
+  // to the FP64 rule
+  const bool cubic = g.box[0] == g.box[1] && g.box[1] == g.box[2];
+  p.rlo2q = cubic ? (float)(p.rlist2 * p.qscale[0] * p.qscale[0] * (1.0 - 1e-5)) : -1.0f;
+  p.rhi2q = cubic ? (float)(p.rlist2 * p.qscale[0] * p.qscale[0] * (1.0 + 1e-5)) : INFINITY;
   // cells of at least r_list/2 with a 5^3 stencil where the box allows it, else r_list and 3^3,
   // else all pairs
   p.brute = 0;

@@ -39,6 +39,7 @@ struct SysParams {
   int nc[3], ncell, brute;
   int cs;          // cell stencil half-width: cells are >= r_list / cs wide, (2 cs + 1)^3 stencil
   float rlo2f, rhi2f;  // FP32 screen of the list rule: accept below, reject above, else FP64
+  float rlo2q, rhi2q;  // the same band in quantized units (cubic boxes)
   int CH, capc;
 };
 

@@ -271,9 +271,9 @@ __global__ __launch_bounds__(RS_BLOCK) void k_list_build(SysParams p, const int*
 // iteration. A wave ballot ranks the accepted ones in stencil order, so candidate g of molecule i
 // goes straight to its dealt place, sub-list g % CH, slot g / CH, of the pair kernels' ELL-by-wave
 // layout: no scratch, deterministic, sub-lists balanced to within one entry.
-constexpr int LC_CHUNK = 1024;  // staged candidates per pass
+constexpr int LC_CHUNK = 1536;  // staged candidates per pass (a 98k-atom stencil holds ~1,200)
 constexpr int LC_MAXH = 512;    // molecules per home cell
-constexpr int LC_MAXR = 64;     // stencil ranges: (2 cs + 1)^2 rows, at most two each
+constexpr int LC_MAXR = 2 * RS_WAVE;  // stencil ranges: (2 cs + 1)^2 <= 25 rows, two slots each
 
 template <int CH>
 __global__ __launch_bounds__(RS_BLOCK) void k_list_cells(SysParams p, const int* __restrict__ start,
@@ -292,35 +292,57 @@ __global__ __launch_bounds__(RS_BLOCK) void k_list_cells(SysParams p, const int*
     if (threadIdx.x == 0) atomicOr(&flags[FLAG_LIST_OVERFLOW], 2);
     return;
   }
-  if (threadIdx.x == 0) {
-    const int cx = h % p.nc[0], cy = (h / p.nc[0]) % p.nc[1], cz = h / (p.nc[0] * p.nc[1]);
-    const int S = p.cs;
-    int k = 0, off = 0;
-    auto add = [&](int e0, int e1) {  // cells e0..e1 inclusive of one row: one molecule range
-      rbeg[k] = start[e0];
-      roff[k] = off;
-      off += start[e1 + 1] - start[e0];
-      ++k;
-    };
-    for (int dz = -S; dz <= S; ++dz)
-      for (int dy = -S; dy <= S; ++dy) {
-        int ey = cy + dy, ez = cz + dz;
-        ey += (ey < 0) ? p.nc[1] : (ey >= p.nc[1] ? -p.nc[1] : 0);
-        ez += (ez < 0) ? p.nc[2] : (ez >= p.nc[2] ? -p.nc[2] : 0);
-        const int row = (ez * p.nc[1] + ey) * p.nc[0];
-        const int x0 = cx - S, x1 = cx + S;
-        if (x0 < 0) {
-          add(row + x0 + p.nc[0], row + p.nc[0] - 1);
-          add(row, row + x1);
-        } else if (x1 >= p.nc[0]) {
-          add(row + x0, row + p.nc[0] - 1);
-          add(row, row + x1 - p.nc[0]);
-        } else {
-          add(row + x0, row + x1);
-        }
+  if (threadIdx.x < RS_WAVE) {
+    // lane r < (2S+1)^2 takes stencil row r = (dy, dz): one contiguous molecule range, or two
+    // where the row wraps around the box in x; offsets by a prefix sum over the wave
+    const int S = p.cs, W = 2 * S + 1, lane = threadIdx.x;
+    int b0 = 0, n0 = 0, b1 = 0, n1 = 0;
+    if (lane < W * W) {
+      const int cx = h % p.nc[0], cy = (h / p.nc[0]) % p.nc[1], cz = h / (p.nc[0] * p.nc[1]);
+      const int dy = lane % W - S, dz = lane / W - S;
+      int ey = cy + dy, ez = cz + dz;
+      ey += (ey < 0) ? p.nc[1] : (ey >= p.nc[1] ? -p.nc[1] : 0);
+      ez += (ez < 0) ? p.nc[2] : (ez >= p.nc[2] ? -p.nc[2] : 0);
+      const int row = (ez * p.nc[1] + ey) * p.nc[0];
+      const int x0 = cx - S, x1 = cx + S;
+      int a0, a1, c0 = 0, c1 = -1;  // cells a0..a1 and c0..c1 of the row
+      if (x0 < 0) {
+        a0 = row + x0 + p.nc[0];
+        a1 = row + p.nc[0] - 1;
+        c0 = row;
+        c1 = row + x1;
+      } else if (x1 >= p.nc[0]) {
+        a0 = row + x0;
+        a1 = row + p.nc[0] - 1;
+        c0 = row;
+        c1 = row + x1 - p.nc[0];
+      } else {
+        a0 = row + x0;
+        a1 = row + x1;
       }
-    roff[k] = off;
-    nr = k;
+      b0 = start[a0];
+      n0 = start[a1 + 1] - b0;
+      if (c1 >= c0) {
+        b1 = start[c0];
+        n1 = start[c1 + 1] - b1;
+      }
+    }
+    int incl = n0 + n1;
+#pragma unroll
+    for (int o = 1; o < RS_WAVE; o <<= 1) {
+      const int t = __shfl_up(incl, o, RS_WAVE);
+      if (lane >= o) incl += t;
+    }
+    const int excl = incl - n0 - n1;
+    // ranges in row order, the wrapped row's two pieces in x order: stencil order
+    rbeg[2 * lane] = b0;
+    roff[2 * lane] = excl;
+    rbeg[2 * lane + 1] = b1;
+    roff[2 * lane + 1] = excl + n0;
+    if (lane == RS_WAVE - 1) {
+      roff[2 * RS_WAVE] = incl;
+      nr = 2 * W * W;
+    }
   }
   for (int m = threadIdx.x; m < nh; m += RS_BLOCK) Ts[m] = 0;
   __syncthreads();
@@ -329,13 +351,16 @@ __global__ __launch_bounds__(RS_BLOCK) void k_list_cells(SysParams p, const int*
   const unsigned long long below = (1ull << lane) - 1ull;
   for (int base = 0; base < total; base += LC_CHUNK) {
     const int nchunk = min(LC_CHUNK, total - base);
-    for (int t = threadIdx.x; t < nchunk; t += RS_BLOCK) {
-      const int gi = base + t;
-      int r = 0;
-      while (roff[r + 1] <= gi) ++r;
-      const int j = rbeg[r] + (gi - roff[r]);
-      sj[t] = j;
-      sq[t] = Oq[j];
+    // one range per wave at a time, lanes along it (no search for an element's range)
+    for (int r = wave; r < nr; r += RS_BLOCK / RS_WAVE) {
+      const int b = rbeg[r], o = roff[r] - base, len = roff[r + 1] - roff[r];
+      for (int t = lane; t < len; t += RS_WAVE) {
+        const int gi = o + t;
+        if (gi >= 0 && gi < nchunk) {
+          sj[gi] = b + t;
+          sq[gi] = Oq[b + t];
+        }
+      }
     }
     __syncthreads();
     for (int m = wave; m < nh; m += RS_BLOCK / RS_WAVE) {
@@ -345,27 +370,26 @@ __global__ __launch_bounds__(RS_BLOCK) void k_list_cells(SysParams p, const int*
       const size_t g0 = (size_t)i * CH;
       int* out = nbr + (g0 / RS_WAVE) * p.capc * RS_WAVE + (g0 & (RS_WAVE - 1));
       int T = Ts[m];
+#pragma unroll 2
       for (int c0 = 0; c0 < nchunk; c0 += RS_WAVE) {
         const int t = c0 + lane;
-        bool in = false;
-        int j = -1;
-        if (t < nchunk) {
-          j = sj[t];
-          if (j != i) {
-            // FP32 screen on quantized positions; the FP64 rule decides inside the uncertainty band
-            const int4 qj = sq[t];
-            const float r2f = r2f_of(qdiff((uint32_t)qi.x, (uint32_t)qj.x, p.unitf[0]),
-                                     qdiff((uint32_t)qi.y, (uint32_t)qj.y, p.unitf[1]),
-                                     qdiff((uint32_t)qi.z, (uint32_t)qj.z, p.unitf[2]));
-            in = r2f < p.rlo2f;
-            if (!in && r2f <= p.rhi2f) {
-              const double4 oj = O64[j];
-              const double dx = mic1(oj.x - oi.x, p.L[0], p.invL[0]);
-              const double dy = mic1(oj.y - oi.y, p.L[1], p.invL[1]);
-              const double dz = mic1(oj.z - oi.z, p.L[2], p.invL[2]);
-              in = r2_of(dx, dy, dz) <= p.rlist2;
-            }
-          }
+        // read at a clamped index and mask, so the LDS reads do not wait on a branch
+        const int tc = t < nchunk ? t : nchunk - 1;
+        const int j = sj[tc];
+        const int4 qj = sq[tc];
+        const bool valid = t < nchunk && j != i;
+        // FP32 screen in quantized units (cubic box: one unit for all axes); the FP64 rule
+        // decides inside the uncertainty band
+        const float r2q = r2f_of(qdiffq((uint32_t)qi.x, (uint32_t)qj.x),
+                                 qdiffq((uint32_t)qi.y, (uint32_t)qj.y),
+                                 qdiffq((uint32_t)qi.z, (uint32_t)qj.z));
+        bool in = valid && r2q < p.rlo2q;
+        if (valid && !in && r2q <= p.rhi2q) {
+          const double4 oj = O64[j];
+          const double dx = mic1(oj.x - oi.x, p.L[0], p.invL[0]);
+          const double dy = mic1(oj.y - oi.y, p.L[1], p.invL[1]);
+          const double dz = mic1(oj.z - oi.z, p.L[2], p.invL[2]);
+          in = r2_of(dx, dy, dz) <= p.rlist2;
         }
         const unsigned long long bal = __ballot(in);
         if (in) {
