@@ -142,41 +142,85 @@ __global__ __launch_bounds__(RS_WAVE) void k_step_rev(SysParams p, int precision
   if (mlp && do_r1 && blockIdx.x == 0) b.wmax[64 * ((step + 1) & 1) + threadIdx.x] = 0.0;
   double xa[3] = {0.0, 0.0, 0.0}, ua[3] = {0.0, 0.0, 0.0}, wa[3] = {0.0, 0.0, 0.0};
   const double mass = L.a == 0 ? p.mO : p.mH;
+  // every load is issued before any store (one memory round trip per wave; the state arrays may
+  // alias as far as the compiler knows)
+  double X[3] = {0, 0, 0}, V[3] = {0, 0, 0}, H[3] = {0, 0, 0}, Aa[3] = {0, 0, 0};
+  double Fo[3] = {0, 0, 0}, Bo[3] = {0, 0, 0}, Do[3] = {0, 0, 0}, Co[3] = {0, 0, 0};
+  int rk = 0;
+  double4 ob = make_double4(0.0, 0.0, 0.0, 0.0);
+  double cs = 1.0;
   if (L.act) {
-    const double cs = clip ? b.scal[4] : 1.0;
-    double xi[3] = {0.0, 0.0, 0.0};
-    if (do_r1) atom_noise(b, p, step, seed, L.atom, mass, kT, xi);
-    bool fin = true;
+    if (do_r1) rk = b.rank[L.mol];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       const size_t o = 3 * (size_t)L.atom + k;
-      double x = b.x[o], v = b.v[o], h = b.h[o], A = b.A[o];
-      if (clip) h = h * cs;
+      X[k] = b.x[o];
+      V[k] = b.v[o];
+      H[k] = b.h[o];
+      Aa[k] = b.A[o];
       if (do_r2) {
-        v = v - (b.F[o] * dt) / mass;
-        double B = e * b.B[o] + dt2 * h;
-        b.B[o] = B;
-        A = A + B;
-        b.A[o] = A;
+        Fo[k] = b.F[o];
+        Bo[k] = b.B[o];
       }
+      if (do_r1) {
+        Do[k] = b.D[o];
+        Co[k] = b.C[o];
+      }
+    }
+    if (clip) cs = b.scal[4];
+    if (do_r1) ob = b.O64[rk];
+  }
+  if (L.act) {
+    double xi[3] = {0.0, 0.0, 0.0};
+    if (do_r1) atom_noise(b, p, step, seed, L.atom, mass, kT, xi);
+    bool fin = true;
+    double vn[3], Bn[3], An[3], Dn[3], Cn[3], hn[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      double x = X[k], v = V[k], h = H[k], A = Aa[k];
+      if (clip) h = h * cs;
+      hn[k] = h;
+      Bn[k] = 0.0;
+      if (do_r2) {
+        v = v - (Fo[k] * dt) / mass;
+        double B = e * Bo[k] + dt2 * h;
+        Bn[k] = B;
+        A = A + B;
+      }
+      An[k] = A;
+      Dn[k] = 0.0;
+      Cn[k] = 0.0;
       if (do_r1) {
         x = x - (v * dt) / 2.0;
         v = (v - xi[k] * s) / e;
         x = x - (v * dt) / 2.0;
         x = wrap1(x, p.L[k]);
-        double D = e * b.D[o] + ((dt2 * h) * (1.0 + e)) / 2.0;
-        double C = b.C[o] + D;
-        b.D[o] = D;
-        b.C[o] = C;
+        double D = e * Do[k] + ((dt2 * h) * (1.0 + e)) / 2.0;
+        double C = Co[k] + D;
+        Dn[k] = D;
+        Cn[k] = C;
         ua[k] = C / mass;
         wa[k] = A * (dt2 / (2.0 * mass));
-      } else if (clip) {
-        b.h[o] = h;
       }
-      b.x[o] = x;
-      b.v[o] = v;
+      vn[k] = v;
       xa[k] = x;
       fin = fin && isfinite(x) && isfinite(v);
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const size_t o = 3 * (size_t)L.atom + k;
+      if (do_r2) {
+        b.B[o] = Bn[k];
+        b.A[o] = An[k];
+      }
+      if (do_r1) {
+        b.D[o] = Dn[k];
+        b.C[o] = Cn[k];
+      } else if (clip) {
+        b.h[o] = hn[k];
+      }
+      b.x[o] = xa[k];
+      b.v[o] = vn[k];
     }
     if (!fin) atomicOr(&b.flags[FLAG_NONFINITE], 1);
   }
@@ -187,7 +231,6 @@ __global__ __launch_bounds__(RS_WAVE) void k_step_rev(SysParams p, int precision
     gather_mol(L, ua, um);
     gather_mol(L, wa, wm);
     if (L.act) {
-      const int rk = b.rank[L.mol];
       write_sorted(p, precision, 3 * rk + L.a, xa, b.posq, b.pos64);
       if (precision == RS_PRECISION_FP32)
         for (int k = 0; k < 3; ++k) b.uf[3 * (3 * rk + L.a) + k] = (float)ua[k];
@@ -195,7 +238,7 @@ __global__ __launch_bounds__(RS_WAVE) void k_step_rev(SysParams p, int precision
         b.ud[3 * rk + L.a] = make_double4(ua[0], ua[1], ua[2], 0.0);
       wv = fmax(fabs(wm[0]), fmax(fabs(wm[1]), fabs(wm[2])));
       double F[9], h[9], vj[3];
-      bonded_mol<true>(p, xm, um, wm, F, h, vj, b.flags, b.O64[rk]);
+      bonded_mol<true>(p, xm, um, wm, F, h, vj, b.flags, ob);
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         b.F[3 * (size_t)L.atom + k] = F[3 * L.a + k];

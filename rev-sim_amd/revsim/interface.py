@@ -1,8 +1,8 @@
 """The reference's Julia call surface, restated over librevsim.
 
-forward_sim / reverse_sim / train mirror water/water.jl:977-979 (forward_sim), :1435-1436
-(reverse_sim) and :1623-1698 (train): same argument order and meaning, same return tuples, same
-argument errors. Differences, all at the boundary:
+forward_sim / reverse_sim / reweighting_sim / train mirror water/water.jl:977-979 (forward_sim),
+:1435-1436 (reverse_sim), :1107-1108 (reweighting_sim) and :1623-1698 (train): same argument order
+and meaning, same return tuples, same argument errors. Differences, all at the boundary:
 
 * `params` is the reference's LJ vector (σ, ε, q_O, k_b, r0, k_a, θ0) (water/water.jl:1572); the
   library's θ is (σ, ε, q_H, k_b, r0, k_a) with q_H = -q_O/2 and θ0 a fixed configuration value, so
@@ -14,6 +14,11 @@ argument errors. Differences, all at the boundary:
 * `n_threads` is accepted and ignored (one GPU per context).
 * `fwd_loggers` is the handle returned in the forward System: the library keeps the forward
   snapshots and MSD frames on the device.
+* `loss_type` :rdf, :enth_vap and :both as in loss_fn (water/water.jl:756-769), with the RDF term
+  weighted 1 (the reference's rdf_loss_weighting folds into w_rdf), plus :msd and :rdf_msd for the
+  diffusion loss (diffusion/diffusion.jl:897-911).
+* `use_barostat` runs the Monte Carlo barostat every 30 steps at 1 bar (MonteCarloBarostatNoUnits
+  defaults, water/water.jl:803-807,1136); the returned System carries the new box.
 """
 from __future__ import annotations
 
@@ -27,7 +32,9 @@ from .simulator import Simulator
 from .system import DEFAULT_THETA, THETA0
 
 INTER_TYPES = ("lj",)
-LOSS_TYPES = ("rdf", "msd", "rdf_msd")
+LOSS_TYPES = ("rdf", "enth_vap", "both", "msd", "rdf_msd")
+BAROSTAT_EVERY = 30      # MonteCarloBarostatNoUnits n_steps (water/water.jl:803)
+BAROSTAT_PRESSURE = 1.0  # bar (water/water.jl:1136)
 
 
 @dataclasses.dataclass
@@ -86,7 +93,11 @@ class Context:
         self.sim = None
         self.key = None
 
-    def simulator(self, **overrides) -> Simulator:
+    def simulator(self, boundary=None, **overrides) -> Simulator:
+        """The context's Simulator for this configuration, at `boundary` (nm) when given (the
+        reference passes the box with each call; a barostat run moves it)."""
+        if boundary is not None:
+            self.box = _box(boundary)
         kw = dict(self.base)
         kw.update(overrides)
         key = tuple(sorted(kw.items()))
@@ -97,6 +108,8 @@ class Context:
             cfg = _abi.make_config(n, list(self.box), rc, **kw)
             self.sim = Simulator(cfg, device=self.device)
             self.key = key
+        elif not np.array_equal(self.sim.get_box(), self.box):
+            self.sim.set_box(self.box)
         return self.sim
 
 
@@ -104,7 +117,8 @@ def _loss_cfg(loss_type, n_steps, n_steps_loss, n_steps_trunc=None, clip=math.in
               msd_every=50):
     lt = str(loss_type).lstrip(":")
     kw = dict(n_steps_loss=int(n_steps_loss) if n_steps_loss and n_steps_loss != n_steps else 0,
-              w_rdf=1.0 if lt in ("rdf", "rdf_msd") else 0.0,
+              w_rdf=1.0 if lt in ("rdf", "both", "rdf_msd") else 0.0,
+              w_dhvap=1.0 if lt in ("enth_vap", "both") else 0.0,
               clip_norm=float(clip))
     if n_steps_trunc is not None:
         kw["n_steps_trunc"] = 0 if n_steps_trunc == (n_steps_loss or n_steps) else int(n_steps_trunc)
@@ -124,21 +138,48 @@ def forward_sim(coords, velocities, boundary, noise_seeds, gamma, n_steps, n_ste
     _check_types(inter_type, loss_type)
     if run_grads:
         raise ValueError("run_grads (the direct-differentiation gradient) is not part of this library")
-    if use_barostat:
-        raise ValueError("the Monte Carlo barostat is not part of this library (NVT only)")
     if n_steps_loss and n_steps % n_steps_loss:
         raise ValueError("n_steps_loss must divide n_steps")
     kw = _loss_cfg(loss_type, n_steps, n_steps_loss)
     if not calc_loss:
-        kw.update(w_rdf=0.0, w_msd=0.0, msd_window_steps=0, msd_every=0)
-    sim = context.simulator(gamma=float(gamma), dt=float(dt), **kw)
+        kw.update(w_rdf=0.0, w_dhvap=0.0, w_msd=0.0, msd_window_steps=0, msd_every=0)
+    if use_barostat:
+        kw.update(barostat_every=BAROSTAT_EVERY, barostat_pressure=BAROSTAT_PRESSURE)
+    sim = context.simulator(boundary, gamma=float(gamma), dt=float(dt), **kw)
     sim.set_theta(params_to_theta(params))
     if target_rdf is not None:
         sim.set_target_rdf(target_rdf)
     x, v, loss = sim.forward(np.asarray(coords, dtype=np.float64).reshape(-1),
                              np.asarray(velocities, dtype=np.float64).reshape(-1), _seed(noise_seeds), n_steps)
-    sys = System(x.reshape(-1, 3), v.reshape(-1, 3), context.box.copy(), loggers=(sim, _seed(noise_seeds), n_steps))
+    box = sim.get_box()
+    context.box = box.copy()
+    sys = System(x.reshape(-1, 3), v.reshape(-1, 3), box, loggers=(sim, _seed(noise_seeds), n_steps))
     return sys, loss, None
+
+
+def reweighting_sim(coords, velocities, boundary, noise_seeds, gamma, n_steps, n_steps_loss, params, dt,
+                    n_threads=None, inter_type="lj", loss_type="rdf", use_barostat=False, *,
+                    context: Context, target_rdf=None):
+    """water/water.jl:1107-1210: one forward trajectory with the ensemble-reweighting gradient
+    estimator over its loss blocks; returns (System, loss, dl_dp, l_rdf, l_ev) with dl_dp in the
+    reference's 7-parameter layout (l_rdf, l_ev are not separated: None)."""
+    _check_types(inter_type, loss_type)
+    if str(loss_type).lstrip(":") in ("msd", "rdf_msd"):
+        raise ValueError("loss_type not recognised")  # reweighting_sim has the water losses only
+    if n_steps_loss and n_steps % n_steps_loss:
+        raise ValueError("n_steps_loss must divide n_steps")
+    kw = _loss_cfg(loss_type, n_steps, n_steps_loss)
+    if use_barostat:
+        kw.update(barostat_every=BAROSTAT_EVERY, barostat_pressure=BAROSTAT_PRESSURE)
+    sim = context.simulator(boundary, gamma=float(gamma), dt=float(dt), **kw)
+    sim.set_theta(params_to_theta(params))
+    if target_rdf is not None:
+        sim.set_target_rdf(target_rdf)
+    loss, g, x, v = sim.reweight(np.asarray(coords, dtype=np.float64).reshape(-1),
+                                 np.asarray(velocities, dtype=np.float64).reshape(-1), _seed(noise_seeds), n_steps)
+    box = sim.get_box()
+    context.box = box.copy()
+    return System(x.reshape(-1, 3), v.reshape(-1, 3), box), loss, theta_grad_to_params(g), None, None
 
 
 def reverse_sim(coords, velocities, boundary, noise_seeds, gamma, n_steps, n_steps_loss, n_steps_trunc, params,
@@ -185,7 +226,8 @@ class Adam:
 
 def train(sim: Simulator, x0, v0, params_start, n_epochs: int, n_steps: int, n_steps_equil: int = 0,
           learning_rate: float = 2e-3, grad_clamp_val: float = 1e3, scale_params: bool = True,
-          seed: int = 1, log=None, mlp_weights=None, eval_fn=None):
+          seed: int = 1, log=None, mlp_weights=None, eval_fn=None, reweighting: bool = False,
+          equil_sim: Simulator | None = None, velocity_temperature: float | None = None):
     """The epoch loop of water/water.jl:1623-1698 on one context: fresh noise seed per epoch,
     optional equilibration, forward + reverse, a log line per epoch, Adam on parameters scaled by
     their starting values with the gradient clamped to ±grad_clamp_val (water/water.jl:1682-1696).
@@ -195,9 +237,17 @@ def train(sim: Simulator, x0, v0, params_start, n_epochs: int, n_steps: int, n_s
     master copy and uploaded (rounded to bfloat16 by the library) before every epoch. The context
     must have been created with use_mlp = 1.
 
+    As in the reference's loop: with `reweighting` the gradient is the ensemble-reweighting
+    estimate (reweighting_sim) instead of the reversible one; `equil_sim` (e.g. a context with the
+    barostat on) equilibrates each epoch from (x0, v0) and the epoch's trajectory then runs at the
+    box it reached (boundary_start, water/water.jl:1651-1669); `velocity_temperature` draws fresh
+    Maxwell-Boltzmann velocities each epoch (random_velocity, water/water.jl:1648).
+
     Returns (params, history) — or (params, mlp_weights, history) when training the MLP — with
     history the per-epoch (loss, dL/dparams[, dL/dW]) and, if `eval_fn` is given, its value on the
     epoch's parameters appended."""
+    if reweighting and mlp_weights is not None:
+        raise ValueError("the reweighting estimator covers the classical parameters only")
     params = np.asarray(params_start, dtype=np.float64).copy()
     p0 = params.copy()
     joint = mlp_weights is not None
@@ -213,15 +263,30 @@ def train(sim: Simulator, x0, v0, params_start, n_epochs: int, n_steps: int, n_s
     rng = np.random.Generator(np.random.Philox(seed))
     history = []
     x, v = np.asarray(x0, dtype=np.float64).reshape(-1), np.asarray(v0, dtype=np.float64).reshape(-1)
+    x0_box = equil_sim.get_box() if equil_sim is not None else None
     for epoch in range(1, n_epochs + 1):
         s_eq, s_run = (int(k) for k in rng.integers(0, 2 ** 63, size=2))
         sim.set_theta(params_to_theta(params))
         if joint:
             sim.set_mlp(W.astype(np.float32))
         xs, vs = x, v
+        if velocity_temperature is not None:
+            from .io import maxwell_boltzmann
+
+            vs = maxwell_boltzmann(x.size // 9, velocity_temperature, s_eq ^ 0x5A5A)
         if n_steps_equil:
-            xs, vs, _ = sim.forward(x, v, s_eq, n_steps_equil)
-        if joint:
+            if equil_sim is not None:
+                equil_sim.set_theta(params_to_theta(params))
+                if joint:
+                    equil_sim.set_mlp(W.astype(np.float32))
+                equil_sim.set_box(x0_box)
+                xs, vs, _ = equil_sim.forward(xs, vs, s_eq, n_steps_equil)
+                sim.set_box(equil_sim.get_box())
+            else:
+                xs, vs, _ = sim.forward(xs, vs, s_eq, n_steps_equil)
+        if reweighting:
+            loss, g, _, _ = sim.reweight(xs, vs, s_run, n_steps)
+        elif joint:
             loss, g, gW = sim.value_and_grad(xs, vs, s_run, n_steps, mlp_grad_size=W.size)
             gW = gW.astype(np.float64)
         else:
@@ -248,5 +313,5 @@ def train(sim: Simulator, x0, v0, params_start, n_epochs: int, n_steps: int, n_s
     return (params, W, history) if joint else (params, history)
 
 
-__all__ = ["System", "Context", "forward_sim", "reverse_sim", "train", "Adam", "params_to_theta",
+__all__ = ["System", "Context", "forward_sim", "reverse_sim", "reweighting_sim", "train", "Adam", "params_to_theta",
            "theta_grad_to_params", "DEFAULT_THETA", "THETA0"]

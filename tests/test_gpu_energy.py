@@ -142,3 +142,31 @@ def test_set_box(gpu_lib, jittered216):
     assert per_atom_rel(sim.forces(x), orc.forces(x)) < 1e-10
     with pytest.raises(revsim.RevSimError):
         sim.set_box([1.5] * 3)  # below twice the cutoff
+
+
+def test_interface_barostat_then_reweighting(gpu_lib, box216):
+    """The reference's epoch shape (water/water.jl:1646-1669): an equilibration forward_sim with the
+    barostat, then a trajectory at the box it reached (here reweighting_sim with the :both loss)."""
+    import revsim
+    from revsim import interface as I
+
+    ctx = I.Context(216, 1.862, cutoff=0.8, precision=FP64)
+    p = np.array([0.315, 0.636, -0.834, 462750.4, 0.09572, 836.8, 1.82421813418])
+    sys_eq, _, _ = I.forward_sim(box216.coords, box216.velocities, None, 5, 1.0, 60, 60, p, 0.001,
+                                 use_barostat=True, calc_loss=False, context=ctx)
+    assert not np.allclose(sys_eq.boundary, 1.862)
+    tgt = revsim.perturbed_target(ctx.simulator().rdf(sys_eq.coords.reshape(-1))[0])
+    sys_r, L, g, _, _ = I.reweighting_sim(sys_eq.coords, sys_eq.velocities, sys_eq.boundary, 7, 1.0, 20,
+                                          5, p, 0.001, loss_type="both", context=ctx, target_rdf=tgt)
+    assert np.array_equal(sys_r.boundary, sys_eq.boundary)
+    assert np.isfinite(L) and np.all(np.isfinite(g)) and g[6] == 0.0
+    # the same trajectory in the oracle at that box
+    import oracle as O
+
+    orc = O.Oracle(revsim.make_config(216, list(sys_eq.boundary), 0.8, precision=FP64, n_steps_loss=5,
+                                      w_dhvap=1.0))
+    orc.set_theta(I.params_to_theta(p))
+    orc.set_target_rdf(tgt)
+    Lo, go, _, _ = orc.reweight(sys_eq.coords.reshape(-1), sys_eq.velocities.reshape(-1), 7, 20)
+    assert L == pytest.approx(Lo, rel=1e-9)
+    assert rel_err(g, I.theta_grad_to_params(go)) < 1e-8

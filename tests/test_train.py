@@ -30,6 +30,9 @@ class OracleSim:
     def forward(self, x, v, seed, n):
         return self.o.forward(x, v, seed, n)
 
+    def reweight(self, x, v, seed, n):
+        return self.o.reweight(x, v, seed, n)
+
     def value_and_grad(self, x, v, seed, n, mlp_grad_size=0):
         L, g = self.o.value_and_grad(x, v, seed, n)
         if mlp_grad_size:
@@ -118,3 +121,35 @@ def test_theta_param_mapping_round_trip():
     gp = I.theta_grad_to_params(g)
     # L(q_O) = L(q_H = -q_O/2): dL/dq_O = -dL/dq_H / 2; θ0 held fixed
     assert gp[2] == -1.5 and gp[6] == 0.0 and np.array_equal(gp[[0, 1, 3, 4, 5]], g[[0, 1, 3, 4, 5]])
+
+
+def test_train_with_reweighting_uses_the_estimator():
+    """reweighting = true (water/water.jl:1625,1656-1659): the epoch's gradient is the reweighting
+    estimate over its loss blocks."""
+    import revsim
+    from revsim import interface as I
+
+    wb = revsim.water_box(216, 1.862)
+    cfg = revsim.make_config(216, 1.862, 0.8, n_steps_loss=5, w_dhvap=1.0)
+    sim = OracleSim(cfg)
+    sim.set_theta(revsim.DEFAULT_THETA)
+    sim.set_target_rdf(revsim.perturbed_target(sim.o.rdf(wb.coords)[0]))
+    p0 = np.array([0.315, 0.636, -0.834, 462750.0, 0.09572, 836.8, 1.82421813418])
+    _, hist = I.train(sim, wb.coords, wb.velocities, p0, 1, 10, reweighting=True)
+    rng = np.random.Generator(np.random.Philox(1))
+    _, s_run = (int(k) for k in rng.integers(0, 2 ** 63, size=2))
+    sim.set_theta(I.params_to_theta(p0))
+    L, g, _, _ = sim.reweight(wb.coords, wb.velocities, s_run, 10)
+    assert hist[0][0] == L and np.array_equal(hist[0][1], I.theta_grad_to_params(g))
+
+
+def test_loss_types_map_to_weights():
+    """loss_fn's :rdf, :enth_vap, :both (water/water.jl:756-769) and the diffusion losses."""
+    from revsim import interface as I
+
+    for lt, (wr, wd, wm) in {"rdf": (1, 0, 0), ":enth_vap": (0, 1, 0), "both": (1, 1, 0),
+                              "msd": (0, 0, 1), "rdf_msd": (1, 0, 1)}.items():
+        kw = I._loss_cfg(lt, 1000, 100)
+        assert (kw["w_rdf"], kw["w_dhvap"], kw.get("w_msd", 0.0)) == (wr, wd, wm), lt
+    with pytest.raises(ValueError):
+        I._check_types("lj", "energy")
