@@ -124,6 +124,13 @@ void set_params(rs_ctx* c) {
   c->rlist = rc + 2.0 * p.hext + g.skin;
   p.rlist2 = c->rlist * c->rlist;
   p.skin_half2 = 0.25 * g.skin * g.skin;
+  // shells: an O-O pair can come within r_c only if its distance at the build was <= r_c + skin
+  // (each O moves less than skin/2), an O-H pair only if <= r_c + h_ext + skin
+  const double shr[2] = {rc + g.skin, rc + p.hext + g.skin};
+  for (int k = 0; k < 2; ++k) {
+    p.sh2[k] = shr[k] * shr[k] * (1.0 + 1e-5);
+    p.sh2q[k] = (float)(p.sh2[k] * p.qscale[0] * p.qscale[0]);
+  }
   // The FP32 list screen errs by < 1e-6 relative in r^2 (quantized positions, two roundings per
   // component), so a 1e-5 band around r_list^2 leaves every decision to the FP64 rule's outcome.
   p.rlo2f = (float)(p.rlist2 * (1.0 - 1e-5));
@@ -177,7 +184,7 @@ int alloc_list(rs_ctx* c) {
   if (b.nbr_tmp) hipFree(b.nbr_tmp);
   b.nbr = b.nbr_tmp = nullptr;
   const size_t lanes = (size_t)pair_blocks(c->L.p) * RS_BLOCK;
-  int r = dalloc(c, &b.nbr, lanes * c->L.p.capc);
+  int r = dalloc(c, &b.nbr, lanes * LIST_SHELLS * c->L.p.capc);
   return r ? r : dalloc(c, &b.nbr_tmp, lanes * c->L.p.capc);
 }
 
@@ -1157,19 +1164,22 @@ int rs_molecule_pairs(rs_ctx* c, const double* x, int32_t* pairs, int64_t capaci
   if ((r = single_eval(c, x, false, [] {}))) return r;
   const SysParams& p = c->L.p;
   const size_t lanes = (size_t)pair_blocks(p) * RS_BLOCK;
-  std::vector<int> nbr(lanes * p.capc), cnt(lanes), order(c->M);
+  std::vector<int> nbr(lanes * LIST_SHELLS * p.capc), order(c->M);
+  std::vector<int4> cnt(lanes);
   HIPCHK(c, hipMemcpy(nbr.data(), c->L.b.nbr, sizeof(int) * nbr.size(), hipMemcpyDeviceToHost));
-  HIPCHK(c, hipMemcpy(cnt.data(), c->L.b.ncnt, sizeof(int) * cnt.size(), hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(cnt.data(), c->L.b.ncnt, sizeof(int4) * cnt.size(), hipMemcpyDeviceToHost));
   HIPCHK(c, hipMemcpy(order.data(), c->L.b.order, sizeof(int) * c->M, hipMemcpyDeviceToHost));
   std::vector<std::pair<int, int>> out;
   for (int i = 0; i < c->M; ++i)
     for (int ch = 0; ch < p.CH; ++ch) {
       const size_t gt = (size_t)i * p.CH + ch;
-      for (int k = 0; k < cnt[gt]; ++k) {
-        int j = nbr[((gt / RS_WAVE) * p.capc + k) * RS_WAVE + (gt % RS_WAVE)];
-        int a = order[i], bb = order[j];
-        if (a < bb) out.emplace_back(a, bb);
-      }
+      const int n[LIST_SHELLS] = {cnt[gt].x, cnt[gt].y, cnt[gt].z};
+      for (int s = 0; s < LIST_SHELLS; ++s)
+        for (int k = 0; k < n[s]; ++k) {
+          int j = nbr[list_base(p, gt, s) + (size_t)k * RS_WAVE];
+          int a = order[i], bb = order[j];
+          if (a < bb) out.emplace_back(a, bb);
+        }
     }
   std::sort(out.begin(), out.end());
   *n_pairs = (int64_t)out.size();

@@ -36,6 +36,11 @@ struct SysParams {
   double mO, mH;
   // list
   double rlist2, skin_half2, hext;
+  // distance shells of the list (O-O distance at the build): shell 0 up to r_c + skin may hold
+  // all nine atom pairs, shell 1 up to r_c + h_ext + skin no O-O pair, shell 2 (up to r_list) only
+  // H-H pairs. Upper bounds of r^2 with a 1e-5 relative margin, FP64 and in quantized units.
+  double sh2[2];
+  float sh2q[2];
   int nc[3], ncell, brute;
   int cs;          // cell stencil half-width: cells are >= r_list / cs wide, (2 cs + 1)^3 stencil
   float rlo2f, rhi2f;  // FP32 screen of the list rule: accept below, reject above, else FP64
@@ -153,6 +158,15 @@ __device__ __forceinline__ void normals3(uint64_t seed, int step, int atom, doub
 // molecule) as a 4-vector; the fourth component is unused.
 __device__ __forceinline__ int4 ld3i(const int* p) { return make_int4(p[0], p[1], p[2], 0); }
 __device__ __forceinline__ float4 ld3f(const float* p) { return make_float4(p[0], p[1], p[2], 0.f); }
+
+// The molecule list, ELL by wave and shell: lane gtid = i*CH + c holds sub-list c of molecule i in
+// each of the three distance shells; entry k of shell s is at
+//   nbr[list_base(p, gtid, s) + k*64]
+// and ncnt[gtid] holds the three sub-list lengths (x, y, z; w unused).
+constexpr int LIST_SHELLS = 3;
+__device__ __host__ __forceinline__ size_t list_base(const SysParams& p, size_t gtid, int s) {
+  return (((gtid / RS_WAVE) * LIST_SHELLS + s) * (size_t)p.capc) * RS_WAVE + (gtid & (RS_WAVE - 1));
+}
 
 // Wave-level reductions over lanes that share a molecule (CH consecutive lanes).
 template <int CH>

@@ -24,7 +24,7 @@ constexpr int NE = 7;  // U, dU/dθ[6]
 
 template <int CH>
 __global__ __launch_bounds__(RS_BLOCK) void k_energy(SysParams p, const int* __restrict__ nbr,
-                                                      const int* __restrict__ ncnt,
+                                                      const int4* __restrict__ ncnt,
                                                       const int* __restrict__ order,
                                                       const double* __restrict__ x, double* Fe,
                                                       double* slots, int* flags) {
@@ -38,32 +38,35 @@ __global__ __launch_bounds__(RS_BLOCK) void k_energy(SysParams p, const int* __r
 #pragma unroll
   for (int t = 0; t < 9; ++t) f[t] = 0.0;
   if (act) {
-    const int n = ncnt[gtid];
-    const int* L = nbr + (size_t)(gtid / RS_WAVE) * p.capc * RS_WAVE + (gtid & (RS_WAVE - 1));
-    const double sig2 = p.sig * p.sig;
-    for (int k = 0; k < n; ++k) {
-      const double* xj = x + 9 * (size_t)order[L[(size_t)k * RS_WAVE]];
+    const int4 nc4 = ncnt[gtid];
+    for (int s = 0; s < LIST_SHELLS; ++s) {
+      const int n = s == 0 ? nc4.x : (s == 1 ? nc4.y : nc4.z);
+      const int* L = nbr + list_base(p, gtid, s);
+      const double sig2 = p.sig * p.sig;
+      for (int k = 0; k < n; ++k) {
+        const double* xj = x + 9 * (size_t)order[L[(size_t)k * RS_WAVE]];
 #pragma unroll
-      for (int a = 0; a < 3; ++a) {
+        for (int a = 0; a < 3; ++a) {
 #pragma unroll
-        for (int b = 0; b < 3; ++b) {
-          double d[3];
+          for (int b = 0; b < 3; ++b) {
+            double d[3];
 #pragma unroll
-          for (int t = 0; t < 3; ++t) d[t] = mic1(xj[3 * b + t] - xi[3 * a + t], p.L[t], p.invL[t]);
-          const double r2 = r2_of(d[0], d[1], d[2]);
-          if (!(r2 <= p.rc2)) continue;
-          const double r = sqrt(r2), ir2 = 1.0 / r2;
-          const double cp = p.kqq[(a > 0) + (b > 0)];
-          urf += cp * (1.0 / r + p.krf * r2 - p.crf);
-          double fr = cp * (ir2 / r - 2.0 * p.krf);  // f/r
-          if (a == 0 && b == 0 && p.eps != 0.0) {
-            const double s2 = sig2 * ir2, s6 = s2 * s2 * s2, s12 = s6 * s6;
-            ulj += 4.0 * p.eps * (s12 - s6);
-            dsig += (24.0 * p.eps / p.sig) * (2.0 * s12 - s6);
-            fr += 24.0 * p.eps * ir2 * (2.0 * s12 - s6);
+            for (int t = 0; t < 3; ++t) d[t] = mic1(xj[3 * b + t] - xi[3 * a + t], p.L[t], p.invL[t]);
+            const double r2 = r2_of(d[0], d[1], d[2]);
+            if (!(r2 <= p.rc2)) continue;
+            const double r = sqrt(r2), ir2 = 1.0 / r2;
+            const double cp = p.kqq[(a > 0) + (b > 0)];
+            urf += cp * (1.0 / r + p.krf * r2 - p.crf);
+            double fr = cp * (ir2 / r - 2.0 * p.krf);  // f/r
+            if (a == 0 && b == 0 && p.eps != 0.0) {
+              const double s2 = sig2 * ir2, s6 = s2 * s2 * s2, s12 = s6 * s6;
+              ulj += 4.0 * p.eps * (s12 - s6);
+              dsig += (24.0 * p.eps / p.sig) * (2.0 * s12 - s6);
+              fr += 24.0 * p.eps * ir2 * (2.0 * s12 - s6);
+            }
+#pragma unroll
+            for (int t = 0; t < 3; ++t) f[3 * a + t] -= fr * d[t];
           }
-#pragma unroll
-          for (int t = 0; t < 3; ++t) f[3 * a + t] -= fr * d[t];
         }
       }
     }

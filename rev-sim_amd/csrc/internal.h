@@ -17,9 +17,10 @@ struct DevBufs {
   double4* ud;     // FP64 mode: accum_C / m
   double4* O64;    // O position at the last list build, sorted
   int4* Oq;        // the same, quantized (2^32 per box edge), for the list build's FP32 screen
-  // molecule list, ELL by wave: nbr[((gtid/64)*capc + k)*64 + gtid%64], gtid = i*CH + c
-  int *nbr, *ncnt;
-  int* nbr_tmp;    // list-build scratch, same layout as nbr (per-lane discovery order)
+  // molecule list, ELL by wave and distance shell (list_base in common.h); per-lane lengths
+  int* nbr;
+  int4* ncnt;
+  int* nbr_tmp;    // all-pairs list-build scratch: [((gtid/64)*capc + k)*64 + gtid%64]
   int* flags;
   double* slots;   // per-block θ-VJP partials [nblocks * 8]
   double* vjb;     // bonded θ-VJP per molecule [3M]: (k_bond, r0, k_angle)

@@ -10,6 +10,11 @@
 // no atomics, deterministic. CH lanes cooperate on one molecule; lane c owns sub-list c, built
 // here by the same lane mapping the pair kernel uses (gtid = i*CH + c), in ELL-by-wave layout so
 // that the pair kernel's list reads are 256-byte coalesced.
+//
+// Each molecule's list is split into three shells by the O-O distance at the build: which atom
+// pairs of a listed molecule can come within r_c before the next build follows from it (O-O only
+// up to r_c + skin, O-H only up to r_c + h_ext + skin), so the pair kernels evaluate nine, eight or
+// four atom pairs per entry, shell by shell, each shell in its own loop.
 #include "internal.h"
 
 namespace {
@@ -156,9 +161,10 @@ template <int CH>
 __global__ __launch_bounds__(RS_BLOCK) void k_list_build(SysParams p, const int* __restrict__ start,
                                                          const double4* __restrict__ O64,
                                                          const int4* __restrict__ Oq, int* tmp,
-                                                         int* nbr, int* ncnt, int* flags) {
+                                                         int* nbr, int4* ncnt, int* flags) {
   // Pass 1: lane c tests every CH-th candidate of the cell stencil (or of all molecules) and
-  // keeps the accepted ones in discovery order in its slot column of tmp.
+  // keeps the accepted ones in discovery order in its slot column of tmp. Everything goes to
+  // shell 0 (all nine atom pairs evaluated): this path serves boxes too small for cells.
   const int gtid = blockIdx.x * RS_BLOCK + threadIdx.x;
   const int i = gtid / CH, c = gtid % CH;
   const bool act = i < p.M;
@@ -256,21 +262,22 @@ __global__ __launch_bounds__(RS_BLOCK) void k_list_build(SysParams p, const int*
           d = q + 1;
           before = incl_of[q];
         }
-      nbr[wbase + (size_t)slot * RS_WAVE + lane] = src0[(size_t)(g - before) * RS_WAVE + d];
+      nbr[list_base(p, gtid, 0) + (size_t)slot * RS_WAVE] = src0[(size_t)(g - before) * RS_WAVE + d];
     }
   }
   if (act) {
     if (ovf) atomicOr(&flags[FLAG_LIST_OVERFLOW], 1);
-    ncnt[gtid] = mine;
+    ncnt[gtid] = make_int4(mine, 0, 0, 0);
   }
 }
 
 // Cell-centric build (the cell-stencil path): one block per home cell. The stencil's quantized O
 // positions are staged in LDS in chunks, in stencil order (rows (dy, dz), then cells along x, then
 // sorted index); each wave takes the cell's molecules in turn and tests 64 staged candidates per
-// iteration. A wave ballot ranks the accepted ones in stencil order, so candidate g of molecule i
-// goes straight to its dealt place, sub-list g % CH, slot g / CH, of the pair kernels' ELL-by-wave
-// layout: no scratch, deterministic, sub-lists balanced to within one entry.
+// iteration. Per shell, a wave ballot ranks the accepted ones in stencil order, so candidate g of
+// molecule i's shell s goes straight to its dealt place, sub-list g % CH, slot g / CH of shell s in
+// the pair kernels' ELL-by-wave layout: no scratch, deterministic, sub-lists balanced to within one
+// entry in every shell. The shell test errs only towards lower shells (more pairs evaluated).
 constexpr int LC_CHUNK = 1536;  // staged candidates per pass (a 98k-atom stencil holds ~1,200)
 constexpr int LC_MAXH = 512;    // molecules per home cell
 constexpr int LC_MAXR = 2 * RS_WAVE;  // stencil ranges: (2 cs + 1)^2 <= 25 rows, two slots each
@@ -279,11 +286,11 @@ template <int CH>
 __global__ __launch_bounds__(RS_BLOCK) void k_list_cells(SysParams p, const int* __restrict__ start,
                                                          const double4* __restrict__ O64,
                                                          const int4* __restrict__ Oq, int* nbr,
-                                                         int* ncnt, int* flags) {
+                                                         int4* ncnt, int* flags) {
   __shared__ int4 sq[LC_CHUNK];
   __shared__ int sj[LC_CHUNK];
   __shared__ int rbeg[LC_MAXR], roff[LC_MAXR + 1];
-  __shared__ int Ts[LC_MAXH];
+  __shared__ int Ts[LIST_SHELLS][LC_MAXH];
   __shared__ int nr;
   const int h = blockIdx.x;
   const int h0 = start[h], nh = start[h + 1] - h0;
@@ -344,7 +351,7 @@ __global__ __launch_bounds__(RS_BLOCK) void k_list_cells(SysParams p, const int*
       nr = 2 * W * W;
     }
   }
-  for (int m = threadIdx.x; m < nh; m += RS_BLOCK) Ts[m] = 0;
+  for (int m = threadIdx.x; m < nh; m += RS_BLOCK) Ts[0][m] = Ts[1][m] = Ts[2][m] = 0;
   __syncthreads();
   const int total = roff[nr];
   const int wave = threadIdx.x / RS_WAVE, lane = threadIdx.x & (RS_WAVE - 1);
@@ -368,8 +375,8 @@ __global__ __launch_bounds__(RS_BLOCK) void k_list_cells(SysParams p, const int*
       const int4 qi = Oq[i];
       const double4 oi = O64[i];
       const size_t g0 = (size_t)i * CH;
-      int* out = nbr + (g0 / RS_WAVE) * p.capc * RS_WAVE + (g0 & (RS_WAVE - 1));
-      int T = Ts[m];
+      int* out0 = nbr + list_base(p, g0, 0);
+      unsigned T0 = Ts[0][m], T1 = Ts[1][m], T2 = Ts[2][m];
 #pragma unroll 2
       for (int c0 = 0; c0 < nchunk; c0 += RS_WAVE) {
         const int t = c0 + lane;
@@ -384,32 +391,55 @@ __global__ __launch_bounds__(RS_BLOCK) void k_list_cells(SysParams p, const int*
                                  qdiffq((uint32_t)qi.y, (uint32_t)qj.y),
                                  qdiffq((uint32_t)qi.z, (uint32_t)qj.z));
         bool in = valid && r2q < p.rlo2q;
+        // shells from the quantized distance (cubic box), else from the FP64 one, which a
+        // non-cubic box computes for every candidate (its band is unbounded)
+        bool s0 = r2q <= p.sh2q[0], s01 = r2q <= p.sh2q[1];
         if (valid && !in && r2q <= p.rhi2q) {
           const double4 oj = O64[j];
           const double dx = mic1(oj.x - oi.x, p.L[0], p.invL[0]);
           const double dy = mic1(oj.y - oi.y, p.L[1], p.invL[1]);
           const double dz = mic1(oj.z - oi.z, p.L[2], p.invL[2]);
-          in = r2_of(dx, dy, dz) <= p.rlist2;
+          const double r2 = r2_of(dx, dy, dz);
+          in = r2 <= p.rlist2;
+          if (p.rlo2q < 0.0f) {
+            s0 = r2 <= p.sh2[0];
+            s01 = r2 <= p.sh2[1];
+          }
         }
-        const unsigned long long bal = __ballot(in);
+        // nested ballots: shell 0, shells 0-1, all; a lane's rank in its shell is the difference
+        // of its prefix counts
+        const unsigned long long ba = __ballot(in), b0 = __ballot(in && s0), b01 = __ballot(in && s01);
         if (in) {
-          const int g = T + __popcll(bal & below);
-          const int slot = g / CH;
-          if (slot < p.capc) out[(size_t)slot * RS_WAVE + (g % CH)] = j;
+          const unsigned c0 = __popcll(b0 & below), c01 = __popcll(b01 & below), ca = __popcll(ba & below);
+          const unsigned sh = s0 ? 0u : (s01 ? 1u : 2u);
+          const unsigned g = s0 ? T0 + c0 : (s01 ? T1 + c01 - c0 : T2 + ca - c01);
+          const unsigned slot = g / CH;
+          if (slot < (unsigned)p.capc) out0[(sh * (unsigned)p.capc + slot) * RS_WAVE + (g % CH)] = j;
         }
-        T += __popcll(bal);
+        const unsigned n0 = __popcll(b0), n01 = __popcll(b01);
+        T0 += n0;
+        T1 += n01 - n0;
+        T2 += __popcll(ba) - n01;
       }
-      if (lane == 0) Ts[m] = T;
+      if (lane == 0) {
+        Ts[0][m] = T0;
+        Ts[1][m] = T1;
+        Ts[2][m] = T2;
+      }
     }
     __syncthreads();
   }
   for (int m = wave; m < nh; m += RS_BLOCK / RS_WAVE) {
-    const int i = h0 + m, T = Ts[m];
-    if (lane < CH) {
-      int mine = (T - lane + CH - 1) / CH;
-      ncnt[(size_t)i * CH + lane] = mine < p.capc ? mine : p.capc;
+    const int i = h0 + m;
+    int mine[LIST_SHELLS];
+    bool ovf = false;
+    for (int s = 0; s < LIST_SHELLS; ++s) {
+      const int T = Ts[s][m];
+      mine[s] = min((T - lane + CH - 1) / CH, p.capc);
+      ovf = ovf || T > CH * p.capc;
     }
-    if (lane == 0 && T > CH * p.capc) atomicOr(&flags[FLAG_LIST_OVERFLOW], 1);
+    if (lane < CH) ncnt[(size_t)i * CH + lane] = make_int4(mine[0], mine[1], mine[2], 0);
+    if (lane == 0 && ovf) atomicOr(&flags[FLAG_LIST_OVERFLOW], 1);
   }
 }
 

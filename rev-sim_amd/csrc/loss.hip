@@ -57,7 +57,7 @@ __device__ __forceinline__ void contrib64(const RdfArgs& r, double rd, unsigned 
 template <int CH>
 __global__ __launch_bounds__(RS_BLOCK) void k_rdf_hist(SysParams p, RdfArgs r,
                                                         const int* __restrict__ nbr,
-                                                        const int* __restrict__ ncnt,
+                                                        const int4* __restrict__ ncnt,
                                                         const int* __restrict__ order,
                                                         const double* __restrict__ x,
                                                         long long* hist) {
@@ -67,17 +67,20 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rdf_hist(SysParams p, RdfArgs r,
   const int gtid = blockIdx.x * RS_BLOCK + threadIdx.x;
   const int i = gtid / CH;
   if (i < p.M) {
-    const int n = ncnt[gtid];
-    const int* L = nbr + (size_t)(gtid / RS_WAVE) * p.capc * RS_WAVE + (gtid & (RS_WAVE - 1));
-    const double* xi = x + 9 * (size_t)order[i];
-    for (int k = 0; k < n; ++k) {
-      const int j = L[(size_t)k * RS_WAVE];
-      if (j <= i) continue;
-      const double* xj = x + 9 * (size_t)order[j];
-      const double dx = mic1(xj[0] - xi[0], p.L[0], p.invL[0]);
-      const double dy = mic1(xj[1] - xi[1], p.L[1], p.invL[1]);
-      const double dz = mic1(xj[2] - xi[2], p.L[2], p.invL[2]);
-      contrib64(r, sqrt(r2_of(dx, dy, dz)), lh);
+    const int4 nc4 = ncnt[gtid];
+    for (int s = 0; s < LIST_SHELLS; ++s) {
+      const int n = s == 0 ? nc4.x : (s == 1 ? nc4.y : nc4.z);
+      const int* L = nbr + list_base(p, gtid, s);
+      const double* xi = x + 9 * (size_t)order[i];
+      for (int k = 0; k < n; ++k) {
+        const int j = L[(size_t)k * RS_WAVE];
+        if (j <= i) continue;
+        const double* xj = x + 9 * (size_t)order[j];
+        const double dx = mic1(xj[0] - xi[0], p.L[0], p.invL[0]);
+        const double dy = mic1(xj[1] - xi[1], p.L[1], p.invL[1]);
+        const double dz = mic1(xj[2] - xi[2], p.L[2], p.invL[2]);
+        contrib64(r, sqrt(r2_of(dx, dy, dz)), lh);
+      }
     }
   }
   __syncthreads();
@@ -112,7 +115,7 @@ __global__ void k_rdf_final(RdfArgs r, const long long* hist, const double* targ
 template <int CH>
 __global__ __launch_bounds__(RS_BLOCK) void k_rdf_grad(SysParams p, RdfArgs r,
                                                         const int* __restrict__ nbr,
-                                                        const int* __restrict__ ncnt,
+                                                        const int4* __restrict__ ncnt,
                                                         const int* __restrict__ order,
                                                         const double* __restrict__ x,
                                                         const double* __restrict__ G,
@@ -124,31 +127,34 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rdf_grad(SysParams p, RdfArgs r,
   const int mi = act ? order[i] : 0;
   if (act) {
     const double* xi = x + 9 * (size_t)mi;
-    const int n = ncnt[gtid];
-    const int* L = nbr + (size_t)(gtid / RS_WAVE) * p.capc * RS_WAVE + (gtid & (RS_WAVE - 1));
-    const double dxb = r.rmax / r.nb, iw = 1.0 / r.width, rmaxw = r.rmax + r.window;
-    for (int k = 0; k < n; ++k) {
-      const int j = L[(size_t)k * RS_WAVE];
-      const double* xj = x + 9 * (size_t)order[j];
-      double d[3];
-      for (int t = 0; t < 3; ++t) d[t] = mic1(xj[t] - xi[t], p.L[t], p.invL[t]);
-      const double rr = sqrt(r2_of(d[0], d[1], d[2]));
-      if (rr > rmaxw + 0.01) continue;
-      int lo, hi;
-      window64(r, rr, lo, hi);
-      if (lo > hi) continue;
-      double den = 0.0, dden = 0.0, s1 = 0.0, s2 = 0.0;
-      for (int bb = lo; bb <= hi; ++bb) {
-        double t = ((double)bb + 0.5) * dxb - rr;
-        double e = rs_exp(-(t * t) * iw);
-        double de = e * 2.0 * t * iw;
-        den += e;
-        dden += de;
-        s1 += G[bb] * de;
-        s2 += G[bb] * e;
+    const int4 nc4 = ncnt[gtid];
+    for (int s = 0; s < LIST_SHELLS; ++s) {
+      const int n = s == 0 ? nc4.x : (s == 1 ? nc4.y : nc4.z);
+      const int* L = nbr + list_base(p, gtid, s);
+      const double dxb = r.rmax / r.nb, iw = 1.0 / r.width, rmaxw = r.rmax + r.window;
+      for (int k = 0; k < n; ++k) {
+        const int j = L[(size_t)k * RS_WAVE];
+        const double* xj = x + 9 * (size_t)order[j];
+        double d[3];
+        for (int t = 0; t < 3; ++t) d[t] = mic1(xj[t] - xi[t], p.L[t], p.invL[t]);
+        const double rr = sqrt(r2_of(d[0], d[1], d[2]));
+        if (rr > rmaxw + 0.01) continue;
+        int lo, hi;
+        window64(r, rr, lo, hi);
+        if (lo > hi) continue;
+        double den = 0.0, dden = 0.0, s1 = 0.0, s2 = 0.0;
+        for (int bb = lo; bb <= hi; ++bb) {
+          double t = ((double)bb + 0.5) * dxb - rr;
+          double e = rs_exp(-(t * t) * iw);
+          double de = e * 2.0 * t * iw;
+          den += e;
+          dden += de;
+          s1 += G[bb] * de;
+          s2 += G[bb] * e;
+        }
+        const double dLdr = s1 / den - dden * s2 / (den * den);
+        for (int t = 0; t < 3; ++t) acc[t] -= dLdr * d[t] / rr;
       }
-      const double dLdr = s1 / den - dden * s2 / (den * den);
-      for (int t = 0; t < 3; ++t) acc[t] -= dLdr * d[t] / rr;
     }
   }
   for (int t = 0; t < 3; ++t) acc[t] = chunk_sum<CH>(acc[t]);

@@ -9,8 +9,11 @@
 //   w = accum_A dt^2/(2m) (water/water.jl:1517-1519), θ = (σ_O, ε_O, q_H, k_b, r0, k_a).
 //
 // Work decomposition: CH consecutive lanes own one molecule i (sorted order); lane c walks
-// sub-list c of i's full molecule list, evaluates the 3x3 atom pairs of each listed molecule j and
-// accumulates only i's side. The CH partial sums are combined by xor-shuffles in a fixed order, so
+// sub-list c of i's full molecule list, evaluates the atom pairs of each listed molecule j and
+// accumulates only i's side. The list's three distance shells are walked in turn, each in its own
+// loop with the atom pairs that can be within r_c there: all nine (shell 0, with the O-O
+// Lennard-Jones and MLP terms), the eight without O-O (shell 1), the four H-H (shell 2). A pair
+// left out would contribute exact zeros, so the sums are the same as over all nine. The CH partial sums are combined by xor-shuffles in a fixed order, so
 // the result is deterministic without atomics. Lane c == 0 then adds them to the molecule's
 // outputs in the caller's atom order.
 //
@@ -33,7 +36,7 @@ namespace {
 struct PairArgs {
   SysParams p;
   const int* __restrict__ nbr;
-  const int* __restrict__ ncnt;
+  const int4* __restrict__ ncnt;
   const int* __restrict__ order;
   const void* __restrict__ pos;
   const void* __restrict__ u;
@@ -110,7 +113,9 @@ struct P64 {
 // The FP32 forward variant fits 64 VGPRs, i.e. 8 waves per SIMD, when asked to (unconstrained it
 // takes 66 and drops to 7); the other variants are left to the compiler.
 template <class P, int CH, bool ADJ>
-constexpr int pair_waves() { return (!ADJ && sizeof(typename P::R) == 4 && CH == 16) ? 8 : 1; }
+constexpr int pair_waves() {
+  return sizeof(typename P::R) == 4 && CH == 16 ? (ADJ ? 4 : 8) : 1;
+}
 
 template <class P, int CH, bool ADJ>
 __global__ __launch_bounds__(RS_BLOCK) __attribute__((amdgpu_waves_per_eu(pair_waves<P, CH, ADJ>())))
@@ -168,114 +173,127 @@ void k_pair(PairArgs A) {
   }
   const R mrlo = (R)A.mrlo, mh = (R)A.mh, minvh = (R)A.minvh;
 
-  const int n = act ? A.ncnt[gtid] : 0;
-  const int* __restrict__ L = A.nbr + (size_t)(gtid / RS_WAVE) * p.capc * RS_WAVE + (gtid & (RS_WAVE - 1));
-  for (int k = 0; k < n; ++k) {
-    const int j = L[(size_t)k * RS_WAVE];
-    Pos pj[3];
-    R uj[3][3];
+  const int4 ncnt = act ? A.ncnt[gtid] : make_int4(0, 0, 0, 0);
+  auto shell = [&](auto tag) {
+    constexpr int S = decltype(tag)::value;
+    constexpr int b0 = S == 2 ? 1 : 0;  // first atom of j that can interact
+    const int n = S == 0 ? ncnt.x : (S == 1 ? ncnt.y : ncnt.z);
+    const int* __restrict__ L = A.nbr + list_base(p, gtid, S);
+    // the next entry's index is loaded while this one is gathered and evaluated (the last entry
+    // is read again at the end), so an iteration waits on one memory latency, not two
+    int jn = n > 0 ? L[0] : 0;
+    for (int k = 0; k < n; ++k) {
+      const int j = jn;
+      Pos pj[3];
+      R uj[3][3];
 #pragma unroll
-    for (int b = 0; b < 3; ++b) {
-      pj[b] = P::ld_pos(A.pos, 3 * j + b);
-      if (ADJ) {
-        U t = P::ld_u(A.u, 3 * j + b);
-        uj[b][0] = t.x;
-        uj[b][1] = t.y;
-        uj[b][2] = t.z;
-      }
-    }
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-#pragma unroll
-      for (int b = 0; b < 3; ++b) {
-        R d[3];
-        P::diff(p, own[a], pj[b], d[0], d[1], d[2]);
-        const R r2 = P::r2(d[0], d[1], d[2]);
-        // r2 == 0 (coincident atoms) within the cutoff gives non-finite forces, which are flagged.
-        // Outside the cutoff 1/r and the reaction-field constant are zeroed, which zeroes every
-        // term below (all carry a power of 1/r) without further selects.
-        const bool in = r2 <= rc2;
-        const R ri = in ? P::rinv(r2) : (R)0;
-        const R ri2 = ri * ri, ri3 = ri2 * ri;
-        const int ty = (a > 0) + (b > 0);
-        const R fr = P::fma_(kA[ty], ri3, in ? -kB[ty] : (R)0);
-#pragma unroll
-        for (int t = 0; t < 3; ++t) F[a][t] -= fr * d[t];
-        R du[3], dot = 0;
+      for (int b = b0; b < 3; ++b) {
+        pj[b] = P::ld_pos(A.pos, 3 * j + b);
         if (ADJ) {
-#pragma unroll
-          for (int t = 0; t < 3; ++t) du[t] = uo[a][t] - uj[b][t];
-          dot = P::fma_(d[2], du[2], P::fma_(d[1], du[1], d[0] * du[0]));
-          // (k3A r'^-3) r'^-2: the FP32 path's r'^-5 alone would be subnormal
-          const R gd = ((k3A[ty] * ri3) * ri2) * dot;
-#pragma unroll
-          for (int t = 0; t < 3; ++t) hh[a][t] = P::fma_(gd, d[t], P::fma_(fr, du[t], hh[a][t]));
+          U t = P::ld_u(A.u, 3 * j + b);
+          uj[b][0] = t.x;
+          uj[b][1] = t.y;
+          uj[b][2] = t.z;
         }
-        if (a == 0 && b == 0) {
-          const R s2 = sig2 * ri2, s6 = s2 * s2 * s2, s12 = s6 * s6;
-          const R frl = eps24 * ri2 * ((R)2 * s12 - s6);
+      }
+      jn = L[(size_t)min(k + 1, n - 1) * RS_WAVE];
 #pragma unroll
-          for (int t = 0; t < 3; ++t) F[0][t] -= frl * d[t];
+      for (int a = b0; a < 3; ++a) {
+#pragma unroll
+        for (int b = b0; b < 3; ++b) {
+          if (S == 1 && a == 0 && b == 0) continue;
+          R d[3];
+          P::diff(p, own[a], pj[b], d[0], d[1], d[2]);
+          const R r2 = P::r2(d[0], d[1], d[2]);
+          // r2 == 0 (coincident atoms) within the cutoff gives non-finite forces, which are flagged.
+          // Outside the cutoff 1/r and the reaction-field constant are zeroed, which zeroes every
+          // term below (all carry a power of 1/r) without further selects.
+          const bool in = r2 <= rc2;
+          const R ri = in ? P::rinv(r2) : (R)0;
+          const R ri2 = ri * ri, ri3 = ri2 * ri;
+          const int ty = (a > 0) + (b > 0);
+          const R fr = P::fma_(kA[ty], ri3, in ? -kB[ty] : (R)0);
+#pragma unroll
+          for (int t = 0; t < 3; ++t) F[a][t] -= fr * d[t];
+          R du[3], dot = 0;
           if (ADJ) {
-            const R gdl = -eps24 * ri2 * ri2 * ((R)28 * s12 - (R)8 * s6) * dot;
-            const R fs = epss * ri2 * ((R)4 * s12 - s6);
 #pragma unroll
-            for (int t = 0; t < 3; ++t) {
-              hh[0][t] = P::fma_(gdl, d[t], P::fma_(frl, du[t], hh[0][t]));
-              Flj[t] = P::fma_(-frl, d[t], Flj[t]);
-              Fs[t] = P::fma_(-fs, d[t], Fs[t]);
-            }
+            for (int t = 0; t < 3; ++t) du[t] = uo[a][t] - uj[b][t];
+            dot = P::fma_(d[2], du[2], P::fma_(d[1], du[1], d[0] * du[0]));
+            // (k3A r'^-3) r'^-2: the FP32 path's r'^-5 alone would be subnormal
+            const R gd = ((k3A[ty] * ri3) * ri2) * dot;
+#pragma unroll
+            for (int t = 0; t < 3; ++t) hh[a][t] = P::fma_(gd, d[t], P::fma_(fr, du[t], hh[a][t]));
           }
-          if (A.mlp) {
-            // f(r) = -u'(r) by cubic Hermite between table nodes k, k+1
-            const R rr = (r2 * ri) * lu;
-            R tk = (rr - mrlo) * minvh;
-            if (in && tk < (R)0) atomicOr(&A.flags[FLAG_MLP_RANGE], 1);
-            tk = tk < (R)0 ? (R)0 : tk;
-            int kk = (int)tk;
-            kk = kk > MLP_NT - 2 ? MLP_NT - 2 : kk;
-            R tt = tk - (R)kk;
-            tt = tt > (R)1 ? (R)1 : tt;
-            R fk0, dk0, fk1, dk1;
-            if (sizeof(R) == 4) {
-              const float2* tb = (const float2*)A.mtab;
-              const float2 e0 = tb[kk], e1 = tb[kk + 1];
-              fk0 = (R)e0.x; dk0 = (R)e0.y; fk1 = (R)e1.x; dk1 = (R)e1.y;
-            } else {
-              const double2* tb = (const double2*)A.mtab;
-              const double2 e0 = tb[kk], e1 = tb[kk + 1];
-              fk0 = (R)e0.x; dk0 = (R)e0.y; fk1 = (R)e1.x; dk1 = (R)e1.y;
-            }
-            R hb[4], hdb[4];
-            hermite<R>(tt, hb, hdb);
-            const R f = hb[0] * fk0 + hb[1] * mh * dk0 + hb[2] * fk1 + hb[3] * mh * dk1;
-            const R fp = (hdb[0] * fk0 + hdb[1] * mh * dk0 + hdb[2] * fk1 + hdb[3] * mh * dk1) * minvh;
-            const R frm = f * ri;
+          if (S == 0 && a == 0 && b == 0) {
+            const R s2 = sig2 * ri2, s6 = s2 * s2 * s2, s12 = s6 * s6;
+            const R frl = eps24 * ri2 * ((R)2 * s12 - s6);
 #pragma unroll
-            for (int t = 0; t < 3; ++t) F[0][t] -= frm * d[t];
+            for (int t = 0; t < 3; ++t) F[0][t] -= frl * d[t];
             if (ADJ) {
-              const R g2m = (fp * lu - f * ri) * ri2;
+              const R gdl = -eps24 * ri2 * ri2 * ((R)28 * s12 - (R)8 * s6) * dot;
+              const R fs = epss * ri2 * ((R)4 * s12 - s6);
 #pragma unroll
               for (int t = 0; t < 3; ++t) {
-                hh[0][t] += frm * du[t] + (g2m * dot) * d[t];
-                Fml[t] -= frm * d[t];
+                hh[0][t] = P::fma_(gdl, d[t], P::fma_(frl, du[t], hh[0][t]));
+                Flj[t] = P::fma_(-frl, d[t], Flj[t]);
+                Fs[t] = P::fma_(-fs, d[t], Fs[t]);
               }
-              // weight VJP: w_O . dF_O/d(table) = -(w_O . r̂) ∂f/∂(table)
-              if (in) {
-                const double q = -(double)((wO[0] * d[0] + wO[1] * d[1] + wO[2] * d[2]) * ri) * gscale;
-                atomicAdd((unsigned long long*)&lg[2 * kk], (unsigned long long)__double2ll_rn(q * (double)hb[0]));
-                atomicAdd((unsigned long long*)&lg[2 * kk + 1],
-                          (unsigned long long)__double2ll_rn(q * (double)(mh * hb[1])));
-                atomicAdd((unsigned long long*)&lg[2 * kk + 2], (unsigned long long)__double2ll_rn(q * (double)hb[2]));
-                atomicAdd((unsigned long long*)&lg[2 * kk + 3],
-                          (unsigned long long)__double2ll_rn(q * (double)(mh * hb[3])));
+            }
+            if (A.mlp) {
+              // f(r) = -u'(r) by cubic Hermite between table nodes k, k+1
+              const R rr = (r2 * ri) * lu;
+              R tk = (rr - mrlo) * minvh;
+              if (in && tk < (R)0) atomicOr(&A.flags[FLAG_MLP_RANGE], 1);
+              tk = tk < (R)0 ? (R)0 : tk;
+              int kk = (int)tk;
+              kk = kk > MLP_NT - 2 ? MLP_NT - 2 : kk;
+              R tt = tk - (R)kk;
+              tt = tt > (R)1 ? (R)1 : tt;
+              R fk0, dk0, fk1, dk1;
+              if (sizeof(R) == 4) {
+                const float2* tb = (const float2*)A.mtab;
+                const float2 e0 = tb[kk], e1 = tb[kk + 1];
+                fk0 = (R)e0.x; dk0 = (R)e0.y; fk1 = (R)e1.x; dk1 = (R)e1.y;
+              } else {
+                const double2* tb = (const double2*)A.mtab;
+                const double2 e0 = tb[kk], e1 = tb[kk + 1];
+                fk0 = (R)e0.x; dk0 = (R)e0.y; fk1 = (R)e1.x; dk1 = (R)e1.y;
+              }
+              R hb[4], hdb[4];
+              hermite<R>(tt, hb, hdb);
+              const R f = hb[0] * fk0 + hb[1] * mh * dk0 + hb[2] * fk1 + hb[3] * mh * dk1;
+              const R fp = (hdb[0] * fk0 + hdb[1] * mh * dk0 + hdb[2] * fk1 + hdb[3] * mh * dk1) * minvh;
+              const R frm = f * ri;
+#pragma unroll
+              for (int t = 0; t < 3; ++t) F[0][t] -= frm * d[t];
+              if (ADJ) {
+                const R g2m = (fp * lu - f * ri) * ri2;
+#pragma unroll
+                for (int t = 0; t < 3; ++t) {
+                  hh[0][t] += frm * du[t] + (g2m * dot) * d[t];
+                  Fml[t] -= frm * d[t];
+                }
+                // weight VJP: w_O . dF_O/d(table) = -(w_O . r̂) ∂f/∂(table)
+                if (in) {
+                  const double q = -(double)((wO[0] * d[0] + wO[1] * d[1] + wO[2] * d[2]) * ri) * gscale;
+                  atomicAdd((unsigned long long*)&lg[2 * kk], (unsigned long long)__double2ll_rn(q * (double)hb[0]));
+                  atomicAdd((unsigned long long*)&lg[2 * kk + 1],
+                            (unsigned long long)__double2ll_rn(q * (double)(mh * hb[1])));
+                  atomicAdd((unsigned long long*)&lg[2 * kk + 2], (unsigned long long)__double2ll_rn(q * (double)hb[2]));
+                  atomicAdd((unsigned long long*)&lg[2 * kk + 3],
+                            (unsigned long long)__double2ll_rn(q * (double)(mh * hb[3])));
+                }
               }
             }
           }
         }
       }
     }
-  }
+  };
+  shell(std::integral_constant<int, 0>());
+  shell(std::integral_constant<int, 1>());
+  shell(std::integral_constant<int, 2>());
 
   // combine the CH lanes of the molecule (fixed xor tree, FP64)
   double Fd[9], hd[9], Fljd[3], Fsd[3], Fmld[3];
