@@ -204,7 +204,7 @@ int alloc_all(rs_ctx* c) {
       (r = dalloc(c, &b.cell_start, p.ncell + 1)) || (r = dalloc(c, &b.cell_fill, p.ncell + 1)) ||
       (r = dalloc(c, &b.tmp, M)) || (r = dalloc(c, &b.order, M)) || (r = dalloc(c, &b.rank, M)))
     return r;
-  if ((r = dalloc(c, &b.posq, 3 * N)) || (r = dalloc(c, &b.pos64, N)) || (r = dalloc(c, &b.uf, 3 * N)) ||
+  if ((r = dalloc(c, &b.posq, 3 * N)) || (r = dalloc(c, &b.pos64, N)) || (r = dalloc(c, &b.pu, 5 * M)) ||
       (r = dalloc(c, &b.ud, N)) || (r = dalloc(c, &b.O64, M)) || (r = dalloc(c, &b.Oq, M)))
     return r;
   if ((r = dalloc(c, &b.ncnt, lanes)) || (r = dalloc(c, &b.flags, FLAG_COUNT)) ||
@@ -865,7 +865,7 @@ int rs_destroy(rs_ctx* c) {
   free_records(c);
   DevBufs& b = c->L.b;
   void* ptrs[] = {b.x, b.v, b.F, b.h, b.A, b.B, b.C, b.D, b.grad, b.cell, b.cell_count,
-                  b.cell_start, b.cell_fill, b.tmp, b.order, b.rank, b.posq, b.pos64, b.uf,
+                  b.cell_start, b.cell_fill, b.tmp, b.order, b.rank, b.posq, b.pos64, b.pu,
                   b.ud, b.O64, b.Oq, b.nbr, b.nbr_tmp, b.ncnt, b.flags, b.slots, b.vjb, b.dth, b.scal, b.wmax, b.hist, b.rdfG,
                   b.target, b.rdf_g, b.frames, b.msd_del, b.msd_acc, b.msd_grad, b.ext_noise,
                   b.hpart, b.mlp_w, b.mlp_wb, b.mlp_tab, b.mlp_slot, b.mlp_G, b.mlp_part, b.mlp_dW,

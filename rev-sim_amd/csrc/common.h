@@ -154,6 +154,18 @@ __device__ __forceinline__ void normals3(uint64_t seed, int step, int atom, doub
   n[2] = rad2 * cos(twopi * u4);
 }
 
+// Word offset of atom a (0 = O) in a molecule's 20-word adjoint pair record: H1 at 0, H2 at 6,
+// O at 12, each as 3 position words then 3 accum_C/m words, so the H-H shell reads 3 aligned
+// 16-byte words and the others 5.
+__device__ __forceinline__ int pair_rec(int a) { return a == 0 ? 12 : 6 * (a - 1); }
+__device__ __forceinline__ void write_pair_rec(float4* pu, int mol, int a, const int q[3], const double u[3]) {
+  float* r = (float*)(pu + 5 * (size_t)mol) + pair_rec(a);
+  for (int k = 0; k < 3; ++k) {
+    r[k] = __int_as_float(q[k]);
+    r[3 + k] = (float)u[k];
+  }
+}
+
 // Three consecutive 32-bit values (one atom of the packed FP32 pair inputs: 9 values per
 // molecule) as a 4-vector; the fourth component is unused.
 __device__ __forceinline__ int4 ld3i(const int* p) { return make_int4(p[0], p[1], p[2], 0); }

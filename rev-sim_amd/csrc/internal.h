@@ -13,7 +13,9 @@ struct DevBufs {
   // sorted pair inputs
   int* posq;       // FP32 mode: quantized positions, 9 ints per molecule (O, H1, H2)
   double4* pos64;  // FP64 mode: positions, 3 per molecule
-  float* uf;       // FP32 mode: accum_C / m, 9 floats per molecule
+  // FP32 mode, adjoint pair inputs: one 80-byte record per molecule, H1, H2, O, each as its
+  // quantized position (int bits) then accum_C / m, and two unused words (pair_rec in common.h)
+  float4* pu;
   double4* ud;     // FP64 mode: accum_C / m
   double4* O64;    // O position at the last list build, sorted
   int4* Oq;        // the same, quantized (2^32 per box edge), for the list build's FP32 screen

@@ -54,34 +54,49 @@ __global__ __launch_bounds__(RS_BLOCK) void k_cells_assign(SysParams p, const do
       if (hist[c]) atomicAdd(&count[c], hist[c]);
 }
 
-// Exclusive scan of cell counts (one block); clears the fill counters, and the counts for the
-// next build (they start at zero when allocated).
-__global__ void k_cells_scan(int ncell, int* count, int* start, int* fill) {
-  __shared__ int part[RS_BLOCK];
-  int t = threadIdx.x;
-  int per = (ncell + RS_BLOCK - 1) / RS_BLOCK;
-  int b0 = t * per, b1 = min(ncell, b0 + per);
-  int s = 0;
-  for (int c = b0; c < b1; ++c) s += count[c];
-  part[t] = s;
-  __syncthreads();
-  if (t == 0) {
-    int acc = 0;
-    for (int u = 0; u < RS_BLOCK; ++u) {
-      int v = part[u];
-      part[u] = acc;
-      acc += v;
+// Exclusive scan of cell counts (one 1,024-thread block); clears the fill counters, and the
+// counts for the next build (they start at zero when allocated). Thread t takes SCAN_PER
+// consecutive cells per round, loaded together; waves scan by shuffles, the block through LDS.
+constexpr int SCAN_BLOCK = 1024, SCAN_PER = 4;
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_cells_scan(int ncell, int* count, int* start, int* fill) {
+  __shared__ int wsum[SCAN_BLOCK / RS_WAVE];
+  __shared__ int carry;
+  const int t = threadIdx.x, lane = t & (RS_WAVE - 1), w = t / RS_WAVE;
+  if (t == 0) carry = 0;
+  for (int base = 0; base < ncell; base += SCAN_BLOCK * SCAN_PER) {
+    const int c0 = base + t * SCAN_PER;
+    int v[SCAN_PER], s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; ++k) {
+      v[k] = c0 + k < ncell ? count[c0 + k] : 0;
+      s += v[k];
     }
-    start[ncell] = acc;
+    int incl = s;
+#pragma unroll
+    for (int o = 1; o < RS_WAVE; o <<= 1) {
+      const int u = __shfl_up(incl, o, RS_WAVE);
+      if (lane >= o) incl += u;
+    }
+    __syncthreads();  // carry of the previous round is final; wsum free
+    if (lane == RS_WAVE - 1) wsum[w] = incl;
+    __syncthreads();
+    int before = carry;
+    for (int q = 0; q < w; ++q) before += wsum[q];
+    int acc = before + incl - s;
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; ++k)
+      if (c0 + k < ncell) {
+        start[c0 + k] = acc;
+        fill[c0 + k] = 0;
+        count[c0 + k] = 0;
+        acc += v[k];
+      }
+    __syncthreads();  // every thread has read carry
+    if (t == SCAN_BLOCK - 1) carry = acc;
   }
   __syncthreads();
-  int acc = part[t];
-  for (int c = b0; c < b1; ++c) {
-    start[c] = acc;
-    fill[c] = 0;
-    acc += count[c];
-    count[c] = 0;
-  }
+  if (t == 0) start[ncell] = carry;
 }
 
 // Places each molecule into its cell's range (any order inside the cell; k_cells_order then sorts
@@ -128,7 +143,7 @@ __global__ void k_cells_order(SysParams p, const int* start, const int* tmp, int
 // Sorted copies of the pair inputs for the current permutation.
 __global__ void k_pack(SysParams p, int precision, const int* __restrict__ order,
                        const double* __restrict__ x, const double* __restrict__ C, int* posq,
-                       double4* pos64, float* uf, double4* ud, double4* O64, int4* Oq) {
+                       double4* pos64, float4* pu, double4* ud, double4* O64, int4* Oq) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;  // sorted atom slot 3*i + a
   if (t >= p.N) return;
   int i = t / 3, a = t - 3 * i;
@@ -150,9 +165,11 @@ __global__ void k_pack(SysParams p, int precision, const int* __restrict__ order
   if (C) {
     double im = 1.0 / (a == 0 ? p.mO : p.mH);
     const double* ca = C + 9 * m + 3 * a;
-    if (precision == RS_PRECISION_FP32)
-      for (int k = 0; k < 3; ++k) uf[3 * t + k] = (float)(ca[k] * im);
-    else
+    if (precision == RS_PRECISION_FP32) {
+      const int q[3] = {(int)quant1(x0, p.qscale[0]), (int)quant1(x1, p.qscale[1]), (int)quant1(x2, p.qscale[2])};
+      const double u[3] = {ca[0] * im, ca[1] * im, ca[2] * im};
+      write_pair_rec(pu, i, a, q, u);
+    } else
       ud[t] = make_double4(ca[0] * im, ca[1] * im, ca[2] * im, 0.0);
   }
 }
@@ -452,13 +469,13 @@ void launch_list_build(const Launch& L, bool with_u) {
   const DevBufs& b = L.b;
   int gm = (p.M + RS_BLOCK - 1) / RS_BLOCK;
   k_cells_assign<<<gm, RS_BLOCK, 0, L.s>>>(p, b.x, b.cell, b.cell_count);
-  k_cells_scan<<<1, RS_BLOCK, 0, L.s>>>(p.ncell, b.cell_count, b.cell_start, b.cell_fill);
+  k_cells_scan<<<1, SCAN_BLOCK, 0, L.s>>>(p.ncell, b.cell_count, b.cell_start, b.cell_fill);
   k_cells_scatter<<<gm, RS_BLOCK, 0, L.s>>>(p, b.cell, b.cell_start, b.cell_fill, b.tmp);
   int gw = (p.ncell * RS_WAVE + RS_BLOCK - 1) / RS_BLOCK;
   k_cells_order<<<gw, RS_BLOCK, 0, L.s>>>(p, b.cell_start, b.tmp, b.order, b.rank);
   int ga = (p.N + RS_BLOCK - 1) / RS_BLOCK;
   k_pack<<<ga, RS_BLOCK, 0, L.s>>>(p, L.precision, b.order, b.x, with_u ? b.C : nullptr, b.posq,
-                                    b.pos64, b.uf, b.ud, b.O64, b.Oq);
+                                    b.pos64, b.pu, b.ud, b.O64, b.Oq);
   if (!p.brute) {
     switch (p.CH) {
       case 4: k_list_cells<4><<<p.ncell, RS_BLOCK, 0, L.s>>>(p, b.cell_start, b.O64, b.Oq, b.nbr, b.ncnt, b.flags); break;

@@ -40,6 +40,7 @@ struct PairArgs {
   const int* __restrict__ order;
   const void* __restrict__ pos;
   const void* __restrict__ u;
+  const float4* __restrict__ pu;  // FP32 adjoint: per-molecule records (write_pair_rec)
   const double* __restrict__ wsrc;
   double wscale[2];
   const double* __restrict__ vjb;
@@ -134,12 +135,48 @@ void k_pair(PairArgs A) {
   const int i = gtid / CH, c = gtid % CH;
   const bool act = i < p.M;
 
+  // the FP32 adjoint reads each molecule's positions and u as one record (write_pair_rec)
+  constexpr bool REC = ADJ && sizeof(R) == 4;
+  // molecule j's inputs, atoms b0..2 (b0 = 1: the H atoms only)
+  auto gather = [&](int j, int b0, Pos* pp, R (*uu)[3]) {
+    if constexpr (REC) {
+      const float4* r = A.pu + 5 * (size_t)j;
+      const float4 v0 = r[0], v1 = r[1], v2 = r[2];
+      pp[1] = make_int4(__float_as_int(v0.x), __float_as_int(v0.y), __float_as_int(v0.z), 0);
+      uu[1][0] = v0.w;
+      uu[1][1] = v1.x;
+      uu[1][2] = v1.y;
+      pp[2] = make_int4(__float_as_int(v1.z), __float_as_int(v1.w), __float_as_int(v2.x), 0);
+      uu[2][0] = v2.y;
+      uu[2][1] = v2.z;
+      uu[2][2] = v2.w;
+      if (b0 == 0) {
+        const float4 v3 = r[3], v4 = r[4];
+        pp[0] = make_int4(__float_as_int(v3.x), __float_as_int(v3.y), __float_as_int(v3.z), 0);
+        uu[0][0] = v3.w;
+        uu[0][1] = v4.x;
+        uu[0][2] = v4.y;
+      }
+    } else {
+#pragma unroll
+      for (int b = b0; b < 3; ++b) {
+        pp[b] = P::ld_pos(A.pos, 3 * j + b);
+        if (ADJ) {
+          U t = P::ld_u(A.u, 3 * j + b);
+          uu[b][0] = t.x;
+          uu[b][1] = t.y;
+          uu[b][2] = t.z;
+        }
+      }
+    }
+  };
+
   Pos own[3];
   R uo[3][3];
   R F[3][3], hh[3][3], Flj[3], Fs[3], Fml[3];
+  gather(act ? i : 0, 0, own, uo);
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    own[a] = P::ld_pos(A.pos, act ? 3 * i + a : 0);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       F[a][k] = 0;
@@ -148,12 +185,6 @@ void k_pair(PairArgs A) {
     Flj[a] = 0;
     Fs[a] = 0;
     Fml[a] = 0;
-    if (ADJ) {
-      U t = P::ld_u(A.u, act ? 3 * i + a : 0);
-      uo[a][0] = t.x;
-      uo[a][1] = t.y;
-      uo[a][2] = t.z;
-    }
   }
   const R rc2 = (R)A.rc2;
   const R kA[3] = {(R)A.kA[0], (R)A.kA[1], (R)A.kA[2]};
@@ -185,18 +216,6 @@ void k_pair(PairArgs A) {
     constexpr int b0 = S == 2 ? 1 : 0;  // first atom of j that can interact
     const int n = S == 0 ? ncnt.x : (S == 1 ? ncnt.y : ncnt.z);
     const int* __restrict__ L = A.nbr + list_base(p, gtid, S);
-    auto gather = [&](int j, Pos* pp, R (*uu)[3]) {
-#pragma unroll
-      for (int b = b0; b < 3; ++b) {
-        pp[b] = P::ld_pos(A.pos, 3 * j + b);
-        if (ADJ) {
-          U t = P::ld_u(A.u, 3 * j + b);
-          uu[b][0] = t.x;
-          uu[b][1] = t.y;
-          uu[b][2] = t.z;
-        }
-      }
-    };
     // Software pipeline: the next entry's index (and, where registers allow, its inputs) is
     // loaded while this entry is evaluated; past the end the last entry is read again.
     constexpr bool PF = pair_prefetch<P, CH, ADJ>();
@@ -205,7 +224,7 @@ void k_pair(PairArgs A) {
     R un[3][3];
     int jn = j0;
     if constexpr (PF) {
-      gather(j0, pn, un);
+      gather(j0, b0, pn, un);
       jn = n > 1 ? L[RS_WAVE] : j0;
     }
     for (int k = 0; k < n; ++k) {
@@ -218,10 +237,10 @@ void k_pair(PairArgs A) {
           if (ADJ)
             for (int t = 0; t < 3; ++t) uj[b][t] = un[b][t];
         }
-        gather(jn, pn, un);
+        gather(jn, b0, pn, un);
         jn = L[(size_t)min(k + 2, n - 1) * RS_WAVE];
       } else {
-        gather(jn, pj, uj);
+        gather(jn, b0, pj, uj);
         jn = L[(size_t)min(k + 1, n - 1) * RS_WAVE];
       }
 #pragma unroll
@@ -465,7 +484,8 @@ void launch_pair(const Launch& L, bool adjoint, const double* wsrc, double wscal
   a.hscale = 1.0 / lu;
   if (f32) {
     a.pos = L.b.posq;
-    a.u = L.b.uf;
+    a.u = nullptr;
+    a.pu = L.b.pu;
     if (adjoint) dispatch<P32, true>(L, a);
     else dispatch<P32, false>(L, a);
   } else {

@@ -232,9 +232,11 @@ __global__ __launch_bounds__(RS_WAVE) void k_step_rev(SysParams p, int precision
     gather_mol(L, wa, wm);
     if (L.act) {
       write_sorted(p, precision, 3 * rk + L.a, xa, b.posq, b.pos64);
-      if (precision == RS_PRECISION_FP32)
-        for (int k = 0; k < 3; ++k) b.uf[3 * (3 * rk + L.a) + k] = (float)ua[k];
-      else
+      if (precision == RS_PRECISION_FP32) {
+        int q[3];
+        for (int k = 0; k < 3; ++k) q[k] = (int)quant1(xa[k], p.qscale[k]);
+        write_pair_rec(b.pu, rk, L.a, q, ua);
+      } else
         b.ud[3 * rk + L.a] = make_double4(ua[0], ua[1], ua[2], 0.0);
       wv = fmax(fabs(wm[0]), fmax(fabs(wm[1]), fabs(wm[2])));
       double F[9], h[9], vj[3];
