@@ -372,7 +372,6 @@ __global__ __launch_bounds__(RS_BLOCK) void k_list_cells(SysParams p, const int*
   __syncthreads();
   const int total = roff[nr];
   const int wave = threadIdx.x / RS_WAVE, lane = threadIdx.x & (RS_WAVE - 1);
-  const unsigned long long below = (1ull << lane) - 1ull;
   for (int base = 0; base < total; base += LC_CHUNK) {
     const int nchunk = min(LC_CHUNK, total - base);
     // one range per wave at a time, lanes along it (no search for an element's range)
@@ -393,6 +392,7 @@ __global__ __launch_bounds__(RS_BLOCK) void k_list_cells(SysParams p, const int*
       const double4 oi = O64[i];
       const size_t g0 = (size_t)i * CH;
       int* out0 = nbr + list_base(p, g0, 0);
+      const unsigned shell_words = (unsigned)p.capc * RS_WAVE;  // one shell of a wave's list
       unsigned T0 = Ts[0][m], T1 = Ts[1][m], T2 = Ts[2][m];
 #pragma unroll 2
       for (int c0 = 0; c0 < nchunk; c0 += RS_WAVE) {
@@ -423,20 +423,21 @@ __global__ __launch_bounds__(RS_BLOCK) void k_list_cells(SysParams p, const int*
             s01 = r2 <= p.sh2[1];
           }
         }
-        // nested ballots: shell 0, shells 0-1, all; a lane's rank in its shell is the difference
-        // of its prefix counts
+        // one ballot per shell (the masks are wave-uniform); a lane's rank in its shell counts
+        // the mask's bits below it (mbcnt)
         const unsigned long long ba = __ballot(in), b0 = __ballot(in && s0), b01 = __ballot(in && s01);
+        const unsigned long long m1 = b01 & ~b0, m2 = ba & ~b01;
         if (in) {
-          const unsigned c0 = __popcll(b0 & below), c01 = __popcll(b01 & below), ca = __popcll(ba & below);
-          const unsigned sh = s0 ? 0u : (s01 ? 1u : 2u);
-          const unsigned g = s0 ? T0 + c0 : (s01 ? T1 + c01 - c0 : T2 + ca - c01);
+          const unsigned long long ms = s0 ? b0 : (s01 ? m1 : m2);
+          const unsigned g = (s0 ? T0 : (s01 ? T1 : T2)) +
+                             __builtin_amdgcn_mbcnt_hi((unsigned)(ms >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)ms, 0u));
           const unsigned slot = g / CH;
-          if (slot < (unsigned)p.capc) out0[(sh * (unsigned)p.capc + slot) * RS_WAVE + (g % CH)] = j;
+          const unsigned shoff = s0 ? 0u : (s01 ? shell_words : 2u * shell_words);
+          if (slot < (unsigned)p.capc) out0[shoff + slot * RS_WAVE + (g % CH)] = j;
         }
-        const unsigned n0 = __popcll(b0), n01 = __popcll(b01);
-        T0 += n0;
-        T1 += n01 - n0;
-        T2 += __popcll(ba) - n01;
+        T0 += __popcll(b0);
+        T1 += __popcll(m1);
+        T2 += __popcll(m2);
       }
       if (lane == 0) {
         Ts[0][m] = T0;
