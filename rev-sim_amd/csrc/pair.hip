@@ -63,8 +63,16 @@ struct PairArgs {
 };
 
 // Fixed-point scale of the per-block table-gradient sums: 2^(50 - e) with max|w_O| < 2^e, so one
-// block's sum (≤ 2^11 pair contributions bounded by max|w_O|) stays below 2^61.
+// block's sum (≤ 2^11 pair contributions bounded by max|w_O|) stays below 2^61, and each
+// contribution (|w_O . r̂| times a Hermite weight ≤ 1) below sqrt(3) 2^50 < 2^51.
 // Called by every lane of a wave (uniform branch): the wave reduces the 64 slots.
+//
+// fix_rn(v) = round-to-nearest-even of v as a 64-bit integer for |v| < 2^51: adding 1.5 * 2^52
+// leaves the rounded integer in the low mantissa bits (the exponent stays 52), three
+// instructions against the general conversion's dozen.
+__device__ __forceinline__ unsigned long long fix_rn(double v) {
+  return (unsigned long long)(__double_as_longlong(v + 6755399441055744.0) - 0x4338000000000000LL);
+}
 __device__ __forceinline__ double mlp_scale(const double* wmax) {
   double m = wmax[threadIdx.x & 63];
   for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
@@ -193,18 +201,24 @@ void k_pair(PairArgs A) {
   const R sig2 = (R)A.sig2A, eps24 = (R)A.eps24A, epss = (R)A.epssA;
   const R lu = (R)A.lu;
 
-  // MLP: own O's w (adjoint), table-gradient accumulator in LDS
+  // MLP: the (f, f') table in LDS (FP32 path; 4 KB), own O's w and the table-gradient
+  // accumulator in LDS (adjoint)
   __shared__ long long lg[ADJ ? MLP_NG : 1];
+  __shared__ float2 stab[sizeof(R) == 4 ? MLP_NT : 1];
   R wO[3] = {0, 0, 0};
   double gscale = 0.0;
-  if (ADJ && A.mlp) {
-    for (int q = threadIdx.x; q < MLP_NG; q += RS_BLOCK) lg[q] = 0;
-    gscale = mlp_scale(A.wmax);
-    if (act) {
-      const size_t mo = 9 * (size_t)A.order[i];
-      wO[0] = (R)(A.wsrc[mo] * A.wscale[0]);
-      wO[1] = (R)(A.wsrc[mo + 1] * A.wscale[0]);
-      wO[2] = (R)(A.wsrc[mo + 2] * A.wscale[0]);
+  if (A.mlp) {
+    if (sizeof(R) == 4)
+      for (int q = threadIdx.x; q < MLP_NT; q += RS_BLOCK) stab[q] = ((const float2*)A.mtab)[q];
+    if (ADJ) {
+      for (int q = threadIdx.x; q < MLP_NG; q += RS_BLOCK) lg[q] = 0;
+      gscale = mlp_scale(A.wmax);
+      if (act) {
+        const size_t mo = 9 * (size_t)A.order[i];
+        wO[0] = (R)(A.wsrc[mo] * A.wscale[0]);
+        wO[1] = (R)(A.wsrc[mo + 1] * A.wscale[0]);
+        wO[2] = (R)(A.wsrc[mo + 2] * A.wscale[0]);
+      }
     }
     __syncthreads();
   }
@@ -298,8 +312,7 @@ void k_pair(PairArgs A) {
               tt = tt > (R)1 ? (R)1 : tt;
               R fk0, dk0, fk1, dk1;
               if (sizeof(R) == 4) {
-                const float2* tb = (const float2*)A.mtab;
-                const float2 e0 = tb[kk], e1 = tb[kk + 1];
+                const float2 e0 = stab[kk], e1 = stab[kk + 1];
                 fk0 = (R)e0.x; dk0 = (R)e0.y; fk1 = (R)e1.x; dk1 = (R)e1.y;
               } else {
                 const double2* tb = (const double2*)A.mtab;
@@ -323,12 +336,10 @@ void k_pair(PairArgs A) {
                 // weight VJP: w_O . dF_O/d(table) = -(w_O . r̂) ∂f/∂(table)
                 if (in) {
                   const double q = -(double)((wO[0] * d[0] + wO[1] * d[1] + wO[2] * d[2]) * ri) * gscale;
-                  atomicAdd((unsigned long long*)&lg[2 * kk], (unsigned long long)__double2ll_rn(q * (double)hb[0]));
-                  atomicAdd((unsigned long long*)&lg[2 * kk + 1],
-                            (unsigned long long)__double2ll_rn(q * (double)(mh * hb[1])));
-                  atomicAdd((unsigned long long*)&lg[2 * kk + 2], (unsigned long long)__double2ll_rn(q * (double)hb[2]));
-                  atomicAdd((unsigned long long*)&lg[2 * kk + 3],
-                            (unsigned long long)__double2ll_rn(q * (double)(mh * hb[3])));
+                  atomicAdd((unsigned long long*)&lg[2 * kk], fix_rn(q * (double)hb[0]));
+                  atomicAdd((unsigned long long*)&lg[2 * kk + 1], fix_rn(q * (double)(mh * hb[1])));
+                  atomicAdd((unsigned long long*)&lg[2 * kk + 2], fix_rn(q * (double)hb[2]));
+                  atomicAdd((unsigned long long*)&lg[2 * kk + 3], fix_rn(q * (double)(mh * hb[3])));
                 }
               }
             }
