@@ -214,6 +214,7 @@ def main():
     achieved = flop / (avg_ms * 1e-3) / 1e12
     fp = tim["fwd_pair"]
     fwd_avg = fp["ms"] / max(fp["launches"], 1)
+    traffic = TRAFFIC_REV_PAIR if a.workload == "water32768" and prec == revsim.RS_PRECISION_FP32 else None
     roof = {
         "kernel": "k_pair<P32,CH,adjoint> (reverse pair + HVP + θ-VJP)",
         "bound": "valu",
@@ -222,8 +223,11 @@ def main():
         "unit": "TFLOP/s",
         "frac": round(achieved / PEAK_FP32, 4),
         "frac_of_nonpacked_peak": round(achieved / PEAK_FP32_NONPACKED, 4),
-        "traffic": TRAFFIC_REV_PAIR if a.workload == "water32768" and prec == revsim.RS_PRECISION_FP32 else None,
+        "traffic": traffic,
         "traffic_unit": "bytes per launch",
+        # the same kernel's HBM rate (measured bytes per launch / event-timed launch) against 8 TB/s
+        "hbm_gb_s": round(traffic / (avg_ms * 1e-3) / 1e9, 1) if traffic else None,
+        "hbm_frac": round(traffic / (avg_ms * 1e-3) / 1e9 / PEAK_HBM, 4) if traffic else None,
         "avg_launch_ms": round(avg_ms, 5),
         "flop_per_launch": flop,
         "pairs_within_rc": n_pairs,
@@ -256,6 +260,9 @@ def main():
         "loss": loss,
         "dL_dtheta": grad.tolist(),
         "phases_ms_one_trajectory": {k: round(vv["ms"], 3) for k, vv in tim.items()},
+        # atom pairs within r_c evaluated per second (each gradient step evaluates them forward and
+        # in the adjoint)
+        "pair_evals_per_s": round(2 * n_pairs * value, -6),
         "roofline": roof,
     }
 
