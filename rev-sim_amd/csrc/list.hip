@@ -4,7 +4,8 @@
 // Membership is by molecule: j is listed for i when the O-O minimum-image distance at build time
 // is <= r_list = r_c + 2*h_ext + skin (FP64 rule, identical to the oracle's), which contains every
 // atom pair within r_c as long as each O moves less than skin/2 and each O-H stays below h_ext
-// until the next build (both checked every step in the pair kernel).
+// until the next build (both checked every step by the integrator kernels, which evaluate the
+// bonded terms).
 //
 // The list is full (both i->j and j->i) so each molecule's forces are summed by its own lanes:
 // no atomics, deterministic. CH lanes cooperate on one molecule; lane c owns sub-list c, built
@@ -175,13 +176,12 @@ __global__ void k_pack(SysParams p, int precision, const int* __restrict__ order
 }
 
 template <int CH>
-__global__ __launch_bounds__(RS_BLOCK) void k_list_build(SysParams p, const int* __restrict__ start,
-                                                         const double4* __restrict__ O64,
+__global__ __launch_bounds__(RS_BLOCK) void k_list_build(SysParams p, const double4* __restrict__ O64,
                                                          const int4* __restrict__ Oq, int* tmp,
                                                          int* nbr, int4* ncnt, int* flags) {
-  // Pass 1: lane c tests every CH-th candidate of the cell stencil (or of all molecules) and
+  // All-pairs build for boxes too small for cells. Pass 1: lane c tests every CH-th molecule and
   // keeps the accepted ones in discovery order in its slot column of tmp. Everything goes to
-  // shell 0 (all nine atom pairs evaluated): this path serves boxes too small for cells.
+  // shell 0 (all nine atom pairs evaluated).
   const int gtid = blockIdx.x * RS_BLOCK + threadIdx.x;
   const int i = gtid / CH, c = gtid % CH;
   const bool act = i < p.M;
@@ -212,39 +212,7 @@ __global__ __launch_bounds__(RS_BLOCK) void k_list_build(SysParams p, const int*
         ++n;
       }
     };
-    if (p.brute) {
-      for (int j = c; j < p.M; j += CH) test(j);
-    } else {
-      int ci = cell_index(p, oi.x, oi.y, oi.z);
-      int cx = ci % p.nc[0], cy = (ci / p.nc[0]) % p.nc[1], cz = ci / (p.nc[0] * p.nc[1]);
-      // The stencil's cells along x are consecutive cell indices, so each (dy, dz) row is one
-      // contiguous range of sorted molecules, or two where it wraps around the box.
-      int off = 0;
-      const int S = p.cs;
-      auto range = [&](int e0, int e1) {  // cells e0..e1 inclusive
-        const int s = start[e0], cnt = start[e1 + 1] - s;
-        const int t0 = ((c - off) % CH + CH) % CH;
-        for (int t = t0; t < cnt; t += CH) test(s + t);
-        off += cnt;
-      };
-      for (int dz = -S; dz <= S; ++dz)
-        for (int dy = -S; dy <= S; ++dy) {
-          int ey = cy + dy, ez = cz + dz;
-          ey += (ey < 0) ? p.nc[1] : (ey >= p.nc[1] ? -p.nc[1] : 0);
-          ez += (ez < 0) ? p.nc[2] : (ez >= p.nc[2] ? -p.nc[2] : 0);
-          const int row = (ez * p.nc[1] + ey) * p.nc[0];
-          const int x0 = cx - S, x1 = cx + S;
-          if (x0 < 0) {
-            range(row + x0 + p.nc[0], row + p.nc[0] - 1);
-            range(row, row + x1);
-          } else if (x1 >= p.nc[0]) {
-            range(row + x0, row + p.nc[0] - 1);
-            range(row, row + x1 - p.nc[0]);
-          } else {
-            range(row + x0, row + x1);
-          }
-        }
-    }
+    for (int j = c; j < p.M; j += CH) test(j);
   }
   // pass 2 reads entries other lanes of the wave wrote in pass 1
   __threadfence_block();
@@ -488,9 +456,9 @@ void launch_list_build(const Launch& L, bool with_u) {
   }
   int gl = pair_blocks(p);
   switch (p.CH) {
-    case 4: k_list_build<4><<<gl, RS_BLOCK, 0, L.s>>>(p, b.cell_start, b.O64, b.Oq, b.nbr_tmp, b.nbr, b.ncnt, b.flags); break;
-    case 8: k_list_build<8><<<gl, RS_BLOCK, 0, L.s>>>(p, b.cell_start, b.O64, b.Oq, b.nbr_tmp, b.nbr, b.ncnt, b.flags); break;
-    case 16: k_list_build<16><<<gl, RS_BLOCK, 0, L.s>>>(p, b.cell_start, b.O64, b.Oq, b.nbr_tmp, b.nbr, b.ncnt, b.flags); break;
-    default: k_list_build<32><<<gl, RS_BLOCK, 0, L.s>>>(p, b.cell_start, b.O64, b.Oq, b.nbr_tmp, b.nbr, b.ncnt, b.flags); break;
+    case 4: k_list_build<4><<<gl, RS_BLOCK, 0, L.s>>>(p, b.O64, b.Oq, b.nbr_tmp, b.nbr, b.ncnt, b.flags); break;
+    case 8: k_list_build<8><<<gl, RS_BLOCK, 0, L.s>>>(p, b.O64, b.Oq, b.nbr_tmp, b.nbr, b.ncnt, b.flags); break;
+    case 16: k_list_build<16><<<gl, RS_BLOCK, 0, L.s>>>(p, b.O64, b.Oq, b.nbr_tmp, b.nbr, b.ncnt, b.flags); break;
+    default: k_list_build<32><<<gl, RS_BLOCK, 0, L.s>>>(p, b.O64, b.Oq, b.nbr_tmp, b.nbr, b.ncnt, b.flags); break;
   }
 }
