@@ -13,9 +13,10 @@
 // accumulates only i's side. The list's three distance shells are walked in turn, each in its own
 // loop with the atom pairs that can be within r_c there: all nine (shell 0, with the O-O
 // Lennard-Jones and MLP terms), the eight without O-O (shell 1), the four H-H (shell 2). A pair
-// left out would contribute exact zeros, so the sums are the same as over all nine. The CH partial sums are combined by xor-shuffles in a fixed order, so
-// the result is deterministic without atomics. Lane c == 0 then adds them to the molecule's
-// outputs in the caller's atom order.
+// left out would contribute exact zeros, so the sums are the same as over all nine. The CH
+// partial sums are combined by xor-shuffles in a fixed order, so the result is deterministic
+// without atomics. Lane c == 0 then adds them to the molecule's outputs in the caller's atom
+// order.
 //
 // Pair arithmetic: FP32 on quantized positions (the int32 difference of 2^32-periodic positions is
 // the minimum image by construction) or FP64 (parity mode). The FP32 path computes in the quantized
@@ -66,13 +67,6 @@ struct PairArgs {
 // block's sum (≤ 2^11 pair contributions bounded by max|w_O|) stays below 2^61, and each
 // contribution (|w_O . r̂| times a Hermite weight ≤ 1) below sqrt(3) 2^50 < 2^51.
 // Called by every lane of a wave (uniform branch): the wave reduces the 64 slots.
-//
-// fix_rn(v) = round-to-nearest-even of v as a 64-bit integer for |v| < 2^51: adding 1.5 * 2^52
-// leaves the rounded integer in the low mantissa bits (the exponent stays 52), three
-// instructions against the general conversion's dozen.
-__device__ __forceinline__ unsigned long long fix_rn(double v) {
-  return (unsigned long long)(__double_as_longlong(v + 6755399441055744.0) - 0x4338000000000000LL);
-}
 __device__ __forceinline__ double mlp_scale(const double* wmax) {
   double m = wmax[threadIdx.x & 63];
   for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
@@ -80,6 +74,13 @@ __device__ __forceinline__ double mlp_scale(const double* wmax) {
   int e;
   frexp(m, &e);
   return ldexp(1.0, 50 - e);
+}
+
+// fix_rn(v) = round-to-nearest-even of v as a 64-bit integer for |v| < 2^51: adding 1.5 * 2^52
+// leaves the rounded integer in the low mantissa bits (the exponent stays 52), three
+// instructions against the general conversion's dozen.
+__device__ __forceinline__ unsigned long long fix_rn(double v) {
+  return (unsigned long long)(__double_as_longlong(v + 6755399441055744.0) - 0x4338000000000000LL);
 }
 
 struct P32 {
