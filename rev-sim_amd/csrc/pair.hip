@@ -120,21 +120,24 @@ struct P64 {
 
 // ------------------------------------------------------------------ the pair kernel
 
-// Occupancy targets of the FP32 variants (the others are left to the compiler): the forward loop
-// gathers one entry ahead in 72 VGPRs, 7 waves per SIMD (8 waves without the prefetch ran 5 %
-// slower); the adjoint fits 128 VGPRs, 4 waves (prefetching there needs 154 VGPRs, 3 waves, and
-// ran 1 % slower).
-template <class P, int CH, bool ADJ>
+// Occupancy targets of the FP32 variants (the others are left to the compiler). The forward loop
+// gathers one entry ahead: without the MLP term in 64 VGPRs, 8 waves per SIMD, so its 8,192 waves
+// at 98k atoms run in one round (2 % faster than 7 waves); with it in 72 VGPRs, 7 waves (at 8 it
+// spills inside the loops). The adjoint fits 128 VGPRs, 4 waves (prefetching there needs 154
+// VGPRs, 3 waves, and ran 1 % slower).
+template <class P, int CH, bool ADJ, bool FMLP>
 constexpr int pair_waves() {
-  return sizeof(typename P::R) == 4 && CH == 16 ? (ADJ ? 4 : 7) : 1;
+  return sizeof(typename P::R) == 4 && CH == 16 ? (ADJ ? 4 : (FMLP ? 7 : 8)) : 1;
 }
 
 // Whether the pair loop gathers the next entry's inputs one iteration ahead.
 template <class P, int CH, bool ADJ>
 constexpr bool pair_prefetch() { return sizeof(typename P::R) == 4 && CH == 16 && !ADJ; }
 
-template <class P, int CH, bool ADJ>
-__global__ __launch_bounds__(RS_BLOCK) __attribute__((amdgpu_waves_per_eu(pair_waves<P, CH, ADJ>())))
+// FMLP: the forward variant's MLP term as a compile-time switch (the plain forward kernel then
+// holds none of its state); the adjoint variants follow the runtime flag.
+template <class P, int CH, bool ADJ, bool FMLP>
+__global__ __launch_bounds__(RS_BLOCK) __attribute__((amdgpu_waves_per_eu(pair_waves<P, CH, ADJ, FMLP>())))
 void k_pair(PairArgs A) {
   typedef typename P::R R;
   typedef typename P::Pos Pos;
@@ -208,7 +211,8 @@ void k_pair(PairArgs A) {
   __shared__ float2 stab[sizeof(R) == 4 ? MLP_NT : 1];
   R wO[3] = {0, 0, 0};
   double gscale = 0.0;
-  if (A.mlp) {
+  const bool mlp = ADJ ? (bool)A.mlp : FMLP;
+  if (mlp) {
     if (sizeof(R) == 4)
       for (int q = threadIdx.x; q < MLP_NT; q += RS_BLOCK) stab[q] = ((const float2*)A.mtab)[q];
     if (ADJ) {
@@ -301,7 +305,7 @@ void k_pair(PairArgs A) {
                 Fs[t] = P::fma_(-fs, d[t], Fs[t]);
               }
             }
-            if (A.mlp) {
+            if (mlp) {
               // f(r) = -u'(r) by cubic Hermite between table nodes k, k+1
               const R rr = (r2 * ri) * lu;
               R tk = (rr - mrlo) * minvh;
@@ -365,7 +369,7 @@ void k_pair(PairArgs A) {
     if (ADJ) {
       Fljd[a] = chunk_sum<CH>((double)Flj[a]);
       Fsd[a] = chunk_sum<CH>((double)Fs[a]);
-      Fmld[a] = A.mlp ? chunk_sum<CH>((double)Fml[a]) : 0.0;
+      Fmld[a] = mlp ? chunk_sum<CH>((double)Fml[a]) : 0.0;
     }
   }
 
@@ -421,7 +425,7 @@ void k_pair(PairArgs A) {
       for (int q = 0; q < RS_BLOCK / CH; ++q) s += vjs[q][threadIdx.x];
       A.slots[(size_t)blockIdx.x * 8 + threadIdx.x] += s;  // this block's slot, summed over steps
     }
-    if (A.mlp) {
+    if (mlp) {
       // this block's table gradient of the step, accumulated over the trajectory in its own slot
       const double inv = 1.0 / gscale;
       double* sl = A.mslot + (size_t)blockIdx.x * MLP_NG;
@@ -452,8 +456,14 @@ template <class P, bool ADJ>
 void dispatch(const Launch& L, const PairArgs& a) {
   int g = pair_blocks(L.p);
   switch (L.p.CH) {
-    case 8: k_pair<P, 8, ADJ><<<g, RS_BLOCK, 0, L.s>>>(a); break;
-    default: k_pair<P, 16, ADJ><<<g, RS_BLOCK, 0, L.s>>>(a); break;
+    case 8:
+      if (!ADJ && a.mlp) k_pair<P, 8, ADJ, !ADJ><<<g, RS_BLOCK, 0, L.s>>>(a);
+      else k_pair<P, 8, ADJ, false><<<g, RS_BLOCK, 0, L.s>>>(a);
+      break;
+    default:
+      if (!ADJ && a.mlp) k_pair<P, 16, ADJ, !ADJ><<<g, RS_BLOCK, 0, L.s>>>(a);
+      else k_pair<P, 16, ADJ, false><<<g, RS_BLOCK, 0, L.s>>>(a);
+      break;
   }
 }
 
