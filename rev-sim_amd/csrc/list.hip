@@ -263,8 +263,8 @@ __global__ __launch_bounds__(RS_BLOCK) void k_list_build(SysParams p, const doub
 // molecule i's shell s goes straight to its dealt place, sub-list g % CH, slot g / CH of shell s in
 // the pair kernels' ELL-by-wave layout: no scratch, deterministic, sub-lists balanced to within one
 // entry in every shell. The shell test errs only towards lower shells (more pairs evaluated).
-constexpr int LC_CHUNK = 1536;  // staged candidates per pass (a 98k-atom stencil holds ~1,200)
-constexpr int LC_MAXH = 512;    // molecules per home cell
+constexpr int LC_CHUNK = 1280;  // staged candidates per pass (a 98k-atom stencil holds ~1,200)
+constexpr int LC_MAXH = 256;    // molecules per home cell
 constexpr int LC_MAXR = 2 * RS_WAVE;  // stencil ranges: (2 cs + 1)^2 <= 25 rows, two slots each
 
 template <int CH>
@@ -272,7 +272,11 @@ __global__ __launch_bounds__(RS_BLOCK) void k_list_cells(SysParams p, const int*
                                                          const double4* __restrict__ O64,
                                                          const int4* __restrict__ Oq, int* nbr,
                                                          int4* ncnt, int* flags) {
-  __shared__ int4 sq[LC_CHUNK];
+  // quantized O positions as three ints (24 KB of LDS per block in all: six blocks per CU)
+  struct Q3 {
+    int x, y, z;
+  };
+  __shared__ Q3 sq[LC_CHUNK];
   __shared__ int sj[LC_CHUNK];
   __shared__ int rbeg[LC_MAXR], roff[LC_MAXR + 1];
   __shared__ int Ts[LIST_SHELLS][LC_MAXH];
@@ -349,7 +353,8 @@ __global__ __launch_bounds__(RS_BLOCK) void k_list_cells(SysParams p, const int*
         const int gi = o + t;
         if (gi >= 0 && gi < nchunk) {
           sj[gi] = b + t;
-          sq[gi] = Oq[b + t];
+          const int4 q = Oq[b + t];
+          sq[gi] = Q3{q.x, q.y, q.z};
         }
       }
     }
@@ -368,7 +373,7 @@ __global__ __launch_bounds__(RS_BLOCK) void k_list_cells(SysParams p, const int*
         // read at a clamped index and mask, so the LDS reads do not wait on a branch
         const int tc = t < nchunk ? t : nchunk - 1;
         const int j = sj[tc];
-        const int4 qj = sq[tc];
+        const Q3 qj = sq[tc];
         const bool valid = t < nchunk && j != i;
         // FP32 screen in quantized units (cubic box: one unit for all axes); the FP64 rule
         // decides inside the uncertainty band
