@@ -180,6 +180,15 @@ __device__ __host__ __forceinline__ size_t list_base(const SysParams& p, size_t 
   return (((gtid / RS_WAVE) * LIST_SHELLS + s) * (size_t)p.capc) * RS_WAVE + (gtid & (RS_WAVE - 1));
 }
 
+// XCD-aware block order. The dispatcher deals consecutive workgroups round-robin over MI355X's 8
+// XCDs, each with its own L2; remapping workgroup b to logical block (b mod 8)*(nb/8) + b/8 gives
+// each XCD one contiguous eighth of the logical blocks, so blocks over neighbouring molecules (the
+// cell-sorted order) share an L2. A bijection when nb is a multiple of 8; otherwise the identity.
+constexpr int RS_XCDS = 8;
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+  return nb % RS_XCDS == 0 ? (b % RS_XCDS) * (nb / RS_XCDS) + b / RS_XCDS : b;
+}
+
 // Wave-level reductions over lanes that share a molecule (CH consecutive lanes).
 template <int CH>
 __device__ __forceinline__ float chunk_sum(float v) {

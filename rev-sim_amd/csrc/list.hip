@@ -281,7 +281,9 @@ __global__ __launch_bounds__(RS_BLOCK) void k_list_cells(SysParams p, const int*
   __shared__ int rbeg[LC_MAXR], roff[LC_MAXR + 1];
   __shared__ int Ts[LIST_SHELLS][LC_MAXH];
   __shared__ int nr;
-  const int h = blockIdx.x;
+  // home cell, XCD-aware: the grid is rounded up to a multiple of 8 and the padding exits
+  const int h = xcd_block(blockIdx.x, gridDim.x);
+  if (h >= p.ncell) return;
   const int h0 = start[h], nh = start[h + 1] - h0;
   if (nh == 0) return;  // uniform for the block
   if (nh > LC_MAXH) {
@@ -452,10 +454,10 @@ void launch_list_build(const Launch& L, bool with_u) {
                                     b.pos64, b.pu, b.ud, b.O64, b.Oq);
   if (!p.brute) {
     switch (p.CH) {
-      case 4: k_list_cells<4><<<p.ncell, RS_BLOCK, 0, L.s>>>(p, b.cell_start, b.O64, b.Oq, b.nbr, b.ncnt, b.flags); break;
-      case 8: k_list_cells<8><<<p.ncell, RS_BLOCK, 0, L.s>>>(p, b.cell_start, b.O64, b.Oq, b.nbr, b.ncnt, b.flags); break;
-      case 16: k_list_cells<16><<<p.ncell, RS_BLOCK, 0, L.s>>>(p, b.cell_start, b.O64, b.Oq, b.nbr, b.ncnt, b.flags); break;
-      default: k_list_cells<32><<<p.ncell, RS_BLOCK, 0, L.s>>>(p, b.cell_start, b.O64, b.Oq, b.nbr, b.ncnt, b.flags); break;
+      case 4: k_list_cells<4><<<(p.ncell + RS_XCDS - 1) / RS_XCDS * RS_XCDS, RS_BLOCK, 0, L.s>>>(p, b.cell_start, b.O64, b.Oq, b.nbr, b.ncnt, b.flags); break;
+      case 8: k_list_cells<8><<<(p.ncell + RS_XCDS - 1) / RS_XCDS * RS_XCDS, RS_BLOCK, 0, L.s>>>(p, b.cell_start, b.O64, b.Oq, b.nbr, b.ncnt, b.flags); break;
+      case 16: k_list_cells<16><<<(p.ncell + RS_XCDS - 1) / RS_XCDS * RS_XCDS, RS_BLOCK, 0, L.s>>>(p, b.cell_start, b.O64, b.Oq, b.nbr, b.ncnt, b.flags); break;
+      default: k_list_cells<32><<<(p.ncell + RS_XCDS - 1) / RS_XCDS * RS_XCDS, RS_BLOCK, 0, L.s>>>(p, b.cell_start, b.O64, b.Oq, b.nbr, b.ncnt, b.flags); break;
     }
     return;
   }

@@ -143,7 +143,8 @@ void k_pair(PairArgs A) {
   typedef typename P::Pos Pos;
   typedef typename P::U U;
   const SysParams& p = A.p;
-  const int gtid = blockIdx.x * RS_BLOCK + threadIdx.x;
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  const int gtid = blk * RS_BLOCK + threadIdx.x;
   const int i = gtid / CH, c = gtid % CH;
   const bool act = i < p.M;
 
@@ -423,12 +424,12 @@ void k_pair(PairArgs A) {
     if (threadIdx.x < 6) {
       double s = 0.0;
       for (int q = 0; q < RS_BLOCK / CH; ++q) s += vjs[q][threadIdx.x];
-      A.slots[(size_t)blockIdx.x * 8 + threadIdx.x] += s;  // this block's slot, summed over steps
+      A.slots[(size_t)blk * 8 + threadIdx.x] += s;  // this block's slot, summed over steps
     }
     if (mlp) {
       // this block's table gradient of the step, accumulated over the trajectory in its own slot
       const double inv = 1.0 / gscale;
-      double* sl = A.mslot + (size_t)blockIdx.x * MLP_NG;
+      double* sl = A.mslot + (size_t)blk * MLP_NG;
       for (int q = threadIdx.x; q < MLP_NG; q += RS_BLOCK)
         if (lg[q]) sl[q] += (double)lg[q] * inv;
     }
