@@ -33,7 +33,7 @@ FLOP_FWD = (36, 15)
 FLOP_REV = (72, 33)
 # HBM bytes per launch of the adjoint pair kernel at this workload: FETCH_SIZE x 2.000 + WRITE_SIZE
 # from rocprofv3 counter runs, scaled by the library's copy kernel (profiles/kernels_water32768.md)
-TRAFFIC_REV_PAIR = 124.5e6
+TRAFFIC_REV_PAIR = 107.8e6
 
 
 def log(*a):
