@@ -362,3 +362,42 @@ def test_truncated_blocks_and_snapshots_run(box216):
     # of 10 steps started from the state at step 50 reproduces the last block's contribution
     xs, vs, _ = o.forward(box216.coords, box216.velocities, 13, 60)
     assert np.all(np.isfinite(xs))
+
+
+def _free_config(n, box, rc, **kw):
+    """No interactions (Coulomb constant, ε, bond and angle constants all zero), so the integrator
+    alone moves the atoms."""
+    import revsim
+
+    cfg = revsim.make_config(n, box, rc, w_rdf=0.0, **kw)
+    cfg.coulomb_k = 0.0
+    th = np.array(revsim.DEFAULT_THETA, dtype=np.float64)
+    th[1] = th[3] = th[5] = 0.0  # ε_O, k_bond, k_angle
+    return cfg, th
+
+
+def free_particle_closed_form(x0, v0, n, dt, gamma, box):
+    """Zero force and zero noise: v_n = e^n v_0 and, from the two half-step drifts of every step,
+    x_n = x_0 + (dt/2) v_0 (1 + e)(1 - e^n)/(1 - e), e = exp(-γ dt) (a geometric series)."""
+    e = np.exp(-gamma * dt)
+    v = v0 * e ** n
+    x = x0 + 0.5 * dt * v0 * (1.0 + e) * (1.0 - e ** n) / (1.0 - e)
+    return np.mod(x.reshape(-1, 3), box).reshape(-1), v
+
+
+def test_free_particle_closed_form(oracle_mod, box216):
+    """The integrator's friction and half-step drifts against the closed form; molecules move
+    rigidly (one velocity for their three atoms) so no bond is needed to keep them whole."""
+    cfg, th = _free_config(216, 1.862, 0.8)
+    o = oracle_mod.Oracle(cfg)
+    o.set_theta(th)
+    rng = np.random.default_rng(4)
+    v0 = np.repeat(rng.normal(0.0, 0.4, (216, 1, 3)), 3, axis=1).reshape(-1)
+    n = 50
+    o.set_external_noise(np.zeros(n * v0.size), n)
+    x, v, _ = o.forward(box216.coords, v0, 1, n)
+    xc, vc = free_particle_closed_form(box216.coords, v0, n, cfg.dt, cfg.gamma, 1.862)
+    d = x - xc
+    d -= 1.862 * np.round(d / 1.862)
+    assert np.abs(d).max() < 1e-12
+    assert np.abs(v - vc).max() < 1e-13
