@@ -195,3 +195,21 @@ def test_reversibility_fp64(gpu_lib, box216):
     d -= 1.862 * np.round(d / 1.862)
     assert np.max(np.abs(d)) < 1e-9  # the reference's own bound, water/water.jl:1613
     assert abs(lr - lf) < 1e-7  # water/water.jl:1614
+
+
+def test_reversibility_fp64_98k_atoms(gpu_lib):
+    """The reference's own reversibility bounds (water/water.jl:1613-1614) at the 98k-atom size,
+    200 + 200 steps, FP64 path: x0 recovered within 1e-9 nm, |L_rev - L_fwd| < 1e-7."""
+    import revsim
+
+    wb = revsim.water_box(32768, 9.95)
+    sim = revsim.Simulator(revsim.make_config(32768, 9.95, 1.0, precision=FP64), device=0)
+    sim.set_theta(revsim.DEFAULT_THETA)
+    sim.set_target_rdf(revsim.perturbed_target(sim.rdf(wb.coords)[0]))
+    xN, vN, lf = sim.forward(wb.coords, wb.velocities, 13, 200)
+    lr, g, x0, v0 = sim.reverse(xN, vN, 13, 200)
+    d = x0 - np.mod(wb.coords, 9.95)
+    d -= 9.95 * np.round(d / 9.95)
+    assert np.max(np.abs(d)) < 1e-9
+    assert abs(lr - lf) < 1e-7
+    assert np.all(np.isfinite(g))
