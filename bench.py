@@ -125,7 +125,7 @@ def grad_error_by_horizon(wl, extra, x, v, theta, target, seed, horizons, mlp=No
     errs = {}
     for horizon in horizons:
         ex = fit_msd_window(extra, horizon)
-        g = {}
+        g, secs = {}, {}
         for p_ in (revsim.RS_PRECISION_FP32, revsim.RS_PRECISION_FP64):
             c_ = revsim.make_config(wl["n_molecules"], wl["box"], wl["cutoff"], precision=p_,
                                     use_mlp=int(mlp is not None), **ex)
@@ -134,12 +134,17 @@ def grad_error_by_horizon(wl, extra, x, v, theta, target, seed, horizons, mlp=No
                 s_.set_target_rdf(target)
                 if mlp is not None:
                     s_.set_mlp(mlp)
+                t0 = time.perf_counter()  # first call of a context: includes its list set-up
                 g[p_] = s_.value_and_grad(x, v, seed, horizon)[1]
+                secs[p_] = time.perf_counter() - t0
         g32, g64 = g[revsim.RS_PRECISION_FP32], g[revsim.RS_PRECISION_FP64]
         errs[str(horizon)] = {
             "normwise": float(np.linalg.norm(g32 - g64) / np.linalg.norm(g64)),
             "per_param": (np.abs(g32 - g64) / np.maximum(np.abs(g64), 1e-300)).tolist(),
             "grad_norm_fp64": float(np.linalg.norm(g64)),
+            # gradient steps/s of each path on this (single, untimed-warm-up) trajectory
+            "steps_per_s": {"fp32": round(horizon / secs[revsim.RS_PRECISION_FP32], 1),
+                            "fp64": round(horizon / secs[revsim.RS_PRECISION_FP64], 1)},
         }
         log(f"[bench] dL/dθ FP32 vs FP64 at {horizon} steps{' (+MLP)' if mlp is not None else ''}: "
             f"{errs[str(horizon)]['normwise']:.3e}")
