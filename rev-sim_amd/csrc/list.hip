@@ -126,8 +126,44 @@ __global__ __launch_bounds__(RS_BLOCK) void k_cells_scatter(SysParams p, const i
   }
 }
 
-// Orders each cell's members by molecule index (one wave per cell): deterministic sorted order.
-__global__ void k_cells_order(SysParams p, const int* start, const int* tmp, int* order, int* rank) {
+// Sorted copies of atom a of molecule m, now at sorted index i: the pair inputs for the current
+// permutation.
+__device__ __forceinline__ void pack_atom(const SysParams& p, int precision, int i, int a, int m,
+                                          const double* __restrict__ x, const double* __restrict__ C,
+                                          int* posq, double4* pos64, float4* pu, double4* ud,
+                                          double4* O64, int4* Oq) {
+  const int t = 3 * i + a;  // sorted atom slot
+  const double* xa = x + 9 * (size_t)m + 3 * a;
+  const double x0 = xa[0], x1 = xa[1], x2 = xa[2];
+  const int q[3] = {(int)quant1(x0, p.qscale[0]), (int)quant1(x1, p.qscale[1]), (int)quant1(x2, p.qscale[2])};
+  if (precision == RS_PRECISION_FP32) {
+    posq[3 * t] = q[0];
+    posq[3 * t + 1] = q[1];
+    posq[3 * t + 2] = q[2];
+  } else {
+    pos64[t] = make_double4(x0, x1, x2, 0.0);
+  }
+  if (a == 0) {
+    O64[i] = make_double4(x0, x1, x2, 0.0);
+    Oq[i] = make_int4(q[0], q[1], q[2], 0);
+  }
+  if (C) {
+    const double im = 1.0 / (a == 0 ? p.mO : p.mH);
+    const double* ca = C + 9 * (size_t)m + 3 * a;
+    const double u[3] = {ca[0] * im, ca[1] * im, ca[2] * im};
+    if (precision == RS_PRECISION_FP32)
+      write_pair_rec(pu, i, a, q, u);
+    else
+      ud[t] = make_double4(u[0], u[1], u[2], 0.0);
+  }
+}
+
+// Orders each cell's members by molecule index (one wave per cell): deterministic sorted order;
+// the lane that places a molecule also writes its packed pair inputs (C: accum_C for the adjoint
+// records, or null).
+__global__ void k_cells_order(SysParams p, int precision, const int* start, const int* tmp, int* order,
+                              int* rank, const double* __restrict__ x, const double* __restrict__ C,
+                              int* posq, double4* pos64, float4* pu, double4* ud, double4* O64, int4* Oq) {
   int wave = (blockIdx.x * blockDim.x + threadIdx.x) / RS_WAVE;
   int lane = threadIdx.x & (RS_WAVE - 1);
   if (wave >= p.ncell) return;
@@ -138,40 +174,8 @@ __global__ void k_cells_order(SysParams p, const int* start, const int* tmp, int
     for (int u = 0; u < n; ++u) r += (tmp[s + u] < m);
     order[s + r] = m;
     rank[m] = s + r;
-  }
-}
-
-// Sorted copies of the pair inputs for the current permutation.
-__global__ void k_pack(SysParams p, int precision, const int* __restrict__ order,
-                       const double* __restrict__ x, const double* __restrict__ C, int* posq,
-                       double4* pos64, float4* pu, double4* ud, double4* O64, int4* Oq) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;  // sorted atom slot 3*i + a
-  if (t >= p.N) return;
-  int i = t / 3, a = t - 3 * i;
-  int m = order[i];
-  const double* xa = x + 9 * m + 3 * a;
-  double x0 = xa[0], x1 = xa[1], x2 = xa[2];
-  if (precision == RS_PRECISION_FP32) {
-    posq[3 * t] = (int)quant1(x0, p.qscale[0]);
-    posq[3 * t + 1] = (int)quant1(x1, p.qscale[1]);
-    posq[3 * t + 2] = (int)quant1(x2, p.qscale[2]);
-  } else {
-    pos64[t] = make_double4(x0, x1, x2, 0.0);
-  }
-  if (a == 0) {
-    O64[i] = make_double4(x0, x1, x2, 0.0);
-    Oq[i] = make_int4((int)quant1(x0, p.qscale[0]), (int)quant1(x1, p.qscale[1]),
-                      (int)quant1(x2, p.qscale[2]), 0);
-  }
-  if (C) {
-    double im = 1.0 / (a == 0 ? p.mO : p.mH);
-    const double* ca = C + 9 * m + 3 * a;
-    if (precision == RS_PRECISION_FP32) {
-      const int q[3] = {(int)quant1(x0, p.qscale[0]), (int)quant1(x1, p.qscale[1]), (int)quant1(x2, p.qscale[2])};
-      const double u[3] = {ca[0] * im, ca[1] * im, ca[2] * im};
-      write_pair_rec(pu, i, a, q, u);
-    } else
-      ud[t] = make_double4(ca[0] * im, ca[1] * im, ca[2] * im, 0.0);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) pack_atom(p, precision, s + r, a, m, x, C, posq, pos64, pu, ud, O64, Oq);
   }
 }
 
@@ -448,10 +452,8 @@ void launch_list_build(const Launch& L, bool with_u) {
   k_cells_scan<<<1, SCAN_BLOCK, 0, L.s>>>(p.ncell, b.cell_count, b.cell_start, b.cell_fill);
   k_cells_scatter<<<gm, RS_BLOCK, 0, L.s>>>(p, b.cell, b.cell_start, b.cell_fill, b.tmp);
   int gw = (p.ncell * RS_WAVE + RS_BLOCK - 1) / RS_BLOCK;
-  k_cells_order<<<gw, RS_BLOCK, 0, L.s>>>(p, b.cell_start, b.tmp, b.order, b.rank);
-  int ga = (p.N + RS_BLOCK - 1) / RS_BLOCK;
-  k_pack<<<ga, RS_BLOCK, 0, L.s>>>(p, L.precision, b.order, b.x, with_u ? b.C : nullptr, b.posq,
-                                    b.pos64, b.pu, b.ud, b.O64, b.Oq);
+  k_cells_order<<<gw, RS_BLOCK, 0, L.s>>>(p, L.precision, b.cell_start, b.tmp, b.order, b.rank, b.x,
+                                          with_u ? b.C : nullptr, b.posq, b.pos64, b.pu, b.ud, b.O64, b.Oq);
   if (!p.brute) {
     switch (p.CH) {
       case 4: k_list_cells<4><<<(p.ncell + RS_XCDS - 1) / RS_XCDS * RS_XCDS, RS_BLOCK, 0, L.s>>>(p, b.cell_start, b.O64, b.Oq, b.nbr, b.ncnt, b.flags); break;
