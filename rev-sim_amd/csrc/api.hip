@@ -301,7 +301,7 @@ struct BarostatState {
 
 double box_volume(const rs_ctx* c) { return c->cfg.box[0] * c->cfg.box[1] * c->cfg.box[2]; }
 double kT_of(const rs_ctx* c);
-void rebuild(rs_ctx* c, bool with_u);
+void rebuild(rs_ctx* c, bool with_u, bool filed = false);
 
 // One barostat move at step n (apply_coupling!, water/water.jl:811-865): U, a trial volume
 // V + dV with dV = s (2u - 1), molecule centres scaled by (V'/V)^(1/3), the list rebuilt at the
@@ -435,15 +435,18 @@ void collect_timing(rs_ctx* c) {
   c->ev_used = 0;
 }
 
-void rebuild(rs_ctx* c, bool with_u) {
+void rebuild(rs_ctx* c, bool with_u, bool filed) {
   tbegin(c, RS_PHASE_LIST);
-  launch_list_build(c->L, with_u);
+  launch_list_build(c->L, with_u, filed);
   tend(c);
   c->since_build = 0;
 }
 
+// Start of a call: clear the status flags, and the cell counts (an earlier call that ended
+// between an integrator step that filed molecules and its list build leaves them counted).
 int reset_flags(rs_ctx* c) {
   HIPCHK(c, hipMemsetAsync(c->L.b.flags, 0, sizeof(int) * FLAG_COUNT, c->s));
+  HIPCHK(c, hipMemsetAsync(c->L.b.cell_count, 0, sizeof(int) * (size_t)c->L.p.ncell, c->s));
   return 0;
 }
 
@@ -544,9 +547,10 @@ int forward_impl(rs_ctx* c, const double* x0, const double* v0, uint64_t seed, i
       launch_pair(c->L, false, nullptr, 0, 0);
       tend(c);
       tbegin(c, RS_PHASE_FWD_STEP);
-      launch_step_fwd(c->L, n, seed, dt, e, s, kT);
+      const bool build = n % g.nl_every == 0;
+      launch_step_fwd(c->L, n, seed, dt, e, s, kT, build);
       tend(c);
-      if (n % g.nl_every == 0) rebuild(c, false);
+      if (build) rebuild(c, false, true);
       take_snap(n);
       if (n % P.n_loss == 0 && g.w_rdf != 0.0) {
         tbegin(c, RS_PHASE_LOSS);
@@ -658,10 +662,12 @@ int reverse_impl(rs_ctx* c, const double* xN, const double* vN, uint64_t seed, i
       ++trunc_count;
       if (trunc_count <= P.n_trunc) {
         tbegin(c, RS_PHASE_REV_STEP);
-        launch_step_rev(c->L, pending, true, n, seed, dt, e, s, kT, nslots, pending && clip);
+        const bool build = c->since_build + 1 >= g.nl_every;
+        launch_step_rev(c->L, pending, true, n, seed, dt, e, s, kT, nslots, pending && clip, build);
         tend(c);
         pending = false;
-        if (++c->since_build >= g.nl_every) rebuild(c, true);
+        ++c->since_build;
+        if (build) rebuild(c, true, true);
         tbegin(c, RS_PHASE_REV_PAIR);
         c->L.parity = n & 1;
         launch_pair(c->L, true, b.A, ws[0], ws[1]);

@@ -92,6 +92,27 @@ __device__ __forceinline__ double wrap1(double x, double L) {
   return x - floor(x / L) * L;
 }
 
+// Cell of a molecule from its O position (cells of side L/nc along each axis).
+__device__ __forceinline__ int cell_index(const SysParams& p, double x, double y, double z) {
+  int c[3];
+  double w[3] = {x, y, z};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    int ck = (int)(wrap1(w[k], p.L[k]) * p.invL[k] * p.nc[k]);
+    c[k] = ck < 0 ? 0 : (ck >= p.nc[k] ? p.nc[k] - 1 : ck);
+  }
+  return (c[2] * p.nc[1] + c[1]) * p.nc[0] + c[0];
+}
+
+// First stage of a list build: molecule m's cell, counted in the per-cell totals (which the
+// build's scan clears again).
+__device__ __forceinline__ void file_molecule(const SysParams& p, int* cell, int* count, int m,
+                                              double x, double y, double z) {
+  const int c = p.brute ? 0 : cell_index(p, x, y, z);
+  cell[m] = c;
+  atomicAdd(&count[c], 1);
+}
+
 // ----------------------------------------------------------------------------------------------
 // exp(t) for t <= 0: Cody-Waite reduction t = k ln2 + f with fdlibm's two-part ln2, Taylor
 // polynomial in f by fused multiply-adds, exact scaling by 2^k. Used for the RDF smoothing

@@ -61,19 +61,20 @@ struct Launch {
   int parity;      // step parity selecting the max|w_O| slot of the adjoint MLP accumulation
 };
 
-// list
-void launch_list_build(const Launch& L, bool with_u);
+// list (filed: the integrator kernel that wrote the positions has already filed each molecule
+// into its cell and counted it, so the build starts at the scan)
+void launch_list_build(const Launch& L, bool with_u, bool filed = false);
 // pair
 void launch_pair(const Launch& L, bool adjoint, const double* wsrc, double wscale_O, double wscale_H);
 void launch_vjp_reduce(const Launch& L);
 void launch_bonded(const Launch& L, bool adjoint, const double* csrc, double cO, double cH,
                    const double* wsrc, double wO, double wH);
-// integrator
+// integrator (assign: file each molecule into its cell for the list build that follows)
 void launch_wrap_state(const Launch& L);
 void launch_step_fwd(const Launch& L, int step, uint64_t seed, double dt, double e, double s,
-                     double kT);
+                     double kT, bool assign = false);
 void launch_step_rev(const Launch& L, bool do_r2, bool do_r1, int step, uint64_t seed, double dt,
-                     double e, double s, double kT, int nslots, bool clip);
+                     double e, double s, double kT, int nslots, bool clip, bool assign = false);
 void launch_seed(const Launch& L, bool reset, double e);
 void launch_noise(const Launch& L, int step, uint64_t seed, double kT, double* xi);
 void launch_hnorm(const Launch& L, double clip);

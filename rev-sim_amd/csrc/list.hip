@@ -20,19 +20,9 @@
 
 namespace {
 
-__device__ __forceinline__ int cell_index(const SysParams& p, double x, double y, double z) {
-  int c[3];
-  double w[3] = {x, y, z};
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    int ck = (int)(wrap1(w[k], p.L[k]) * p.invL[k] * p.nc[k]);
-    c[k] = ck < 0 ? 0 : (ck >= p.nc[k] ? p.nc[k] - 1 : ck);
-  }
-  return (c[2] * p.nc[1] + c[1]) * p.nc[0] + c[0];
-}
-
 // Cell of each molecule's O and the per-cell counts. Counts go through a block-private LDS
 // histogram (molecules of one block share few cells) and one global atomic per nonzero bin.
+// Builds that follow an integrator step skip this kernel: the step files the molecules itself.
 constexpr int LDS_CELLS = 8192;
 
 __global__ __launch_bounds__(RS_BLOCK) void k_cells_assign(SysParams p, const double* __restrict__ x,
@@ -444,11 +434,11 @@ __global__ __launch_bounds__(RS_BLOCK) void k_list_cells(SysParams p, const int*
 
 int pair_blocks(const SysParams& p) { return (p.M * p.CH + RS_BLOCK - 1) / RS_BLOCK; }
 
-void launch_list_build(const Launch& L, bool with_u) {
+void launch_list_build(const Launch& L, bool with_u, bool filed) {
   const SysParams& p = L.p;
   const DevBufs& b = L.b;
   int gm = (p.M + RS_BLOCK - 1) / RS_BLOCK;
-  k_cells_assign<<<gm, RS_BLOCK, 0, L.s>>>(p, b.x, b.cell, b.cell_count);
+  if (!filed) k_cells_assign<<<gm, RS_BLOCK, 0, L.s>>>(p, b.x, b.cell, b.cell_count);
   k_cells_scan<<<1, SCAN_BLOCK, 0, L.s>>>(p.ncell, b.cell_count, b.cell_start, b.cell_fill);
   k_cells_scatter<<<gm, RS_BLOCK, 0, L.s>>>(p, b.cell, b.cell_start, b.cell_fill, b.tmp);
   int gw = (p.ncell * RS_WAVE + RS_BLOCK - 1) / RS_BLOCK;

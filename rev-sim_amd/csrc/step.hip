@@ -92,7 +92,7 @@ __device__ __forceinline__ void gather_mol(const AtomLane& L, const double v[3],
 // kernel (it adds the pair forces).
 __global__ __launch_bounds__(RS_WAVE) void k_step_fwd(SysParams p, int precision, DevBufs b,
                                                       int step, uint64_t seed, double dt,
-                                                      double e, double s, double kT) {
+                                                      double e, double s, double kT, int assign) {
 #pragma clang fp contract(off)
   const AtomLane L = atom_lane(p);
   double xa[3] = {0.0, 0.0, 0.0};
@@ -115,6 +115,7 @@ __global__ __launch_bounds__(RS_WAVE) void k_step_fwd(SysParams p, int precision
       fin = fin && isfinite(x) && isfinite(v);
     }
     if (!fin) atomicOr(&b.flags[FLAG_NONFINITE], 1);
+    if (assign && L.a == 0) file_molecule(p, b.cell, b.cell_count, L.mol, xa[0], xa[1], xa[2]);
   }
   double xm[9];
   gather_mol(L, xa, xm);
@@ -134,7 +135,7 @@ __global__ __launch_bounds__(RS_WAVE) void k_step_rev(SysParams p, int precision
                                                       int do_r2, int do_r1, int step,
                                                       uint64_t seed, double dt, double e,
                                                       double s, double kT, int nslots, int clip,
-                                                      int mlp) {
+                                                      int mlp, int assign) {
 #pragma clang fp contract(off)
   const AtomLane L = atom_lane(p);
   const double dt2 = dt * dt;
@@ -223,6 +224,7 @@ __global__ __launch_bounds__(RS_WAVE) void k_step_rev(SysParams p, int precision
       b.v[o] = vn[k];
     }
     if (!fin) atomicOr(&b.flags[FLAG_NONFINITE], 1);
+    if (assign && do_r1 && L.a == 0) file_molecule(p, b.cell, b.cell_count, L.mol, xa[0], xa[1], xa[2]);
   }
   double wv = 0.0;  // max |w| on this molecule's O (MLP scale)
   if (do_r1) {
@@ -358,9 +360,9 @@ void launch_wrap_state(const Launch& L) {
 }
 
 void launch_step_fwd(const Launch& L, int step, uint64_t seed, double dt, double e, double s,
-                     double kT) {
+                     double kT, bool assign) {
   k_step_fwd<<<(L.p.M + ATOM_MOLS - 1) / ATOM_MOLS, RS_WAVE, 0, L.s>>>(L.p, L.precision, L.b, step,
-                                                                       seed, dt, e, s, kT);
+                                                                       seed, dt, e, s, kT, assign);
 }
 
 void launch_bonded(const Launch& L, bool adjoint, const double* csrc, double cO, double cH,
@@ -373,9 +375,10 @@ void launch_bonded(const Launch& L, bool adjoint, const double* csrc, double cO,
 }
 
 void launch_step_rev(const Launch& L, bool do_r2, bool do_r1, int step, uint64_t seed, double dt,
-                     double e, double s, double kT, int nslots, bool clip) {
+                     double e, double s, double kT, int nslots, bool clip, bool assign) {
   k_step_rev<<<(L.p.M + ATOM_MOLS - 1) / ATOM_MOLS, RS_WAVE, 0, L.s>>>(
-      L.p, L.precision, L.b, do_r2, do_r1, step, seed, dt, e, s, kT, nslots, clip, L.use_mlp);
+      L.p, L.precision, L.b, do_r2, do_r1, step, seed, dt, e, s, kT, nslots, clip, L.use_mlp,
+      assign && do_r1);
 }
 
 void launch_seed(const Launch& L, bool reset, double e) {
