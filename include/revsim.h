@@ -199,7 +199,9 @@ int rs_molecule_pairs(rs_ctx* ctx, const double* x, int32_t* pairs, int64_t capa
                       int64_t* n_pairs);
 /* List radius used by rs_molecule_pairs and the force kernels (nm). */
 double rs_list_radius(const rs_ctx* ctx);
-/* Per-phase device time (ms) accumulated since the last call: see RS_PHASE_* */
+/* Per-phase device time (ms) accumulated since the last call: see RS_PHASE_*. The forward pass
+   runs the pair forces and the integrator step as one kernel per step, timed as FWD_PAIR
+   (FWD_STEP stays zero). */
 enum { RS_PHASE_FWD_PAIR = 0, RS_PHASE_FWD_STEP, RS_PHASE_REV_PAIR, RS_PHASE_REV_STEP,
        RS_PHASE_LIST, RS_PHASE_LOSS, RS_PHASE_MLP, RS_PHASE_COUNT };
 int rs_timings(rs_ctx* ctx, double* ms_by_phase, int32_t n);

@@ -204,7 +204,8 @@ int alloc_all(rs_ctx* c) {
       (r = dalloc(c, &b.cell_start, p.ncell + 1)) || (r = dalloc(c, &b.cell_fill, p.ncell + 1)) ||
       (r = dalloc(c, &b.tmp, M)) || (r = dalloc(c, &b.order, M)) || (r = dalloc(c, &b.rank, M)))
     return r;
-  if ((r = dalloc(c, &b.posq, 3 * N)) || (r = dalloc(c, &b.pos64, N)) || (r = dalloc(c, &b.pu, 5 * M)) ||
+  if ((r = dalloc(c, &b.posq, 3 * N)) || (r = dalloc(c, &b.pos64, N)) || (r = dalloc(c, &b.posq_alt, 3 * N)) ||
+      (r = dalloc(c, &b.pos64_alt, N)) || (r = dalloc(c, &b.pu, 5 * M)) ||
       (r = dalloc(c, &b.ud, N)) || (r = dalloc(c, &b.O64, M)) || (r = dalloc(c, &b.Oq, M)))
     return r;
   if ((r = dalloc(c, &b.ncnt, lanes)) || (r = dalloc(c, &b.flags, FLAG_COUNT)) ||
@@ -543,12 +544,12 @@ int forward_impl(rs_ctx* c, const double* x0, const double* v0, uint64_t seed, i
     };
     take_snap(0);
     for (int n = 1; n <= n_steps; ++n) {
+      // one kernel per step: the pair forces at x_{n-1}, then step n and the bonded forces at x_n
       tbegin(c, RS_PHASE_FWD_PAIR);
-      launch_pair(c->L, false, nullptr, 0, 0);
-      tend(c);
-      tbegin(c, RS_PHASE_FWD_STEP);
       const bool build = n % g.nl_every == 0;
-      launch_step_fwd(c->L, n, seed, dt, e, s, kT, build);
+      launch_pair_step(c->L, n, seed, dt, e, s, kT, build);
+      std::swap(b.posq, b.posq_alt);
+      std::swap(b.pos64, b.pos64_alt);
       tend(c);
       if (build) rebuild(c, false, true);
       take_snap(n);
@@ -871,7 +872,7 @@ int rs_destroy(rs_ctx* c) {
   free_records(c);
   DevBufs& b = c->L.b;
   void* ptrs[] = {b.x, b.v, b.F, b.h, b.A, b.B, b.C, b.D, b.grad, b.cell, b.cell_count,
-                  b.cell_start, b.cell_fill, b.tmp, b.order, b.rank, b.posq, b.pos64, b.pu,
+                  b.cell_start, b.cell_fill, b.tmp, b.order, b.rank, b.posq, b.pos64, b.posq_alt, b.pos64_alt, b.pu,
                   b.ud, b.O64, b.Oq, b.nbr, b.nbr_tmp, b.ncnt, b.flags, b.slots, b.vjb, b.dth, b.scal, b.wmax, b.hist, b.rdfG,
                   b.target, b.rdf_g, b.frames, b.msd_del, b.msd_acc, b.msd_grad, b.ext_noise,
                   b.hpart, b.mlp_w, b.mlp_wb, b.mlp_tab, b.mlp_slot, b.mlp_G, b.mlp_part, b.mlp_dW,

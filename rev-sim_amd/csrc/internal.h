@@ -13,6 +13,10 @@ struct DevBufs {
   // sorted pair inputs
   int* posq;       // FP32 mode: quantized positions, 9 ints per molecule (O, H1, H2)
   double4* pos64;  // FP64 mode: positions, 3 per molecule
+  // the other buffer of each, written by the pair kernel that ends with the integrator step; the
+  // host swaps the pair after such a launch
+  int* posq_alt;
+  double4* pos64_alt;
   // FP32 mode, adjoint pair inputs: one 80-byte record per molecule, H1, H2, O, each as its
   // quantized position (int bits) then accum_C / m, and two unused words (pair_rec in common.h)
   float4* pu;
@@ -66,13 +70,15 @@ struct Launch {
 void launch_list_build(const Launch& L, bool with_u, bool filed = false);
 // pair
 void launch_pair(const Launch& L, bool adjoint, const double* wsrc, double wscale_O, double wscale_H);
+// forward pair kernel followed, in the same kernel, by the forward step n (launch_step_fwd's
+// arguments); writes the new sorted positions to posq_alt / pos64_alt, which the caller swaps in
+void launch_pair_step(const Launch& L, int step, uint64_t seed, double dt, double e, double s,
+                      double kT, bool assign);
 void launch_vjp_reduce(const Launch& L);
 void launch_bonded(const Launch& L, bool adjoint, const double* csrc, double cO, double cH,
                    const double* wsrc, double wO, double wH);
 // integrator (assign: file each molecule into its cell for the list build that follows)
 void launch_wrap_state(const Launch& L);
-void launch_step_fwd(const Launch& L, int step, uint64_t seed, double dt, double e, double s,
-                     double kT, bool assign = false);
 void launch_step_rev(const Launch& L, bool do_r2, bool do_r1, int step, uint64_t seed, double dt,
                      double e, double s, double kT, int nslots, bool clip, bool assign = false);
 void launch_seed(const Launch& L, bool reset, double e);

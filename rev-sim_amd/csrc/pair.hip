@@ -2,8 +2,10 @@
 // the molecule list, forces, Hessian-vector product and θ-VJP in one pass.
 //
 // Forward variant (calc_F!, water/water.jl:877-975): F(x). The intramolecular terms (FP64, see
-// bonded.h) are evaluated by the preceding integrator kernel, which seeds F (and h and the bonded
-// θ-VJP in the adjoint variant) at the same positions; this kernel adds the pair terms.
+// bonded.h) seed F (and h and the bonded θ-VJP in the adjoint variant) at the same positions:
+// evaluated by the preceding integrator kernel in the adjoint, by the previous launch's step
+// epilogue in the forward loop, whose kernel ends with the Langevin step (STEP variant). This
+// kernel adds the pair terms.
 // Adjoint variant (calc_F_and_grads!, water/water.jl:1212-1433): F(x), the Hessian-vector product
 //   h = (dF/dx)^T u with u = accum_C / m, and the θ-VJP sum_dof w_dof dF_dof/dθ with
 //   w = accum_A dt^2/(2m) (water/water.jl:1517-1519), θ = (σ_O, ε_O, q_H, k_b, r0, k_a).
@@ -30,6 +32,8 @@
 //          with (f' - f/r)/r^2 = -3 k q_a q_b / r^5   [- 24ε/r^4 (28 s^12 - 8 s^6) for O-O]
 //   q-VJP: dF/dq_H = 2 F_RF / q_H; ε-VJP: dF/dε = F_LJ/ε; σ-VJP: dF_a/dσ = -(144ε/(σ r^2))(4s^12-s^6) d
 #include "internal.h"
+#include "bonded.h"
+#include "integrator.h"
 #include "mlp.h"
 
 namespace {
@@ -136,9 +140,12 @@ constexpr bool pair_prefetch() { return sizeof(typename P::R) == 4 && CH == 16 &
 
 // FMLP: the forward variant's MLP term as a compile-time switch (the plain forward kernel then
 // holds none of its state); the adjoint variants follow the runtime flag.
-template <class P, int CH, bool ADJ, bool FMLP>
+// STEP (forward only): the kernel ends with the integrator step of its molecules (St), so the
+// forward loop runs one kernel per step. The new sorted positions go to the other buffer
+// (St.posq_next / St.pos64_next): other blocks may still be reading this step's.
+template <class P, int CH, bool ADJ, bool FMLP, bool STEP>
 __global__ __launch_bounds__(RS_BLOCK) __attribute__((amdgpu_waves_per_eu(pair_waves<P, CH, ADJ, FMLP>())))
-void k_pair(PairArgs A) {
+void k_pair(PairArgs A, StepArgs St) {
   typedef typename P::R R;
   typedef typename P::Pos Pos;
   typedef typename P::U U;
@@ -375,7 +382,55 @@ void k_pair(PairArgs A) {
   }
 
   __shared__ double vjs[RS_BLOCK / CH][6];
-  if (c == 0) {
+  if constexpr (STEP) {
+    // Forward step n of the block's molecules (the step kernel's arithmetic), then the bonded
+    // forces at the new positions, which seed F for the next pair kernel. The pair forces go
+    // through LDS to as few waves as hold three lanes per molecule (one wave for CH = 16): the
+    // step's FP64 work then runs on 48 of 64 lanes rather than 3 of every 16.
+    static_assert(!ADJ, "the step variant is forward-only");
+    constexpr int NM = RS_BLOCK / CH, MPW = RS_WAVE / 3;  // molecules per block, per step wave
+    __shared__ double fpair[NM][9];
+    if (c == 0) {
+#pragma unroll
+      for (int t = 0; t < 9; ++t) fpair[threadIdx.x / CH][t] = Fd[t];
+    }
+    __syncthreads();
+    const int w = threadIdx.x / RS_WAVE, l = threadIdx.x % RS_WAVE;
+    if (w * MPW < NM) {  // wave-uniform
+      const bool lv = l < 3 * MPW;
+      const int ml = w * MPW + (lv ? l / 3 : 0), a = lv ? l % 3 : 0;
+      const int im = blk * NM + ml;
+      const bool al = lv && ml < NM && im < p.M;
+      const int m = al ? A.order[im] : 0;
+      const int atom = 3 * m + a;
+      const double mass = a == 0 ? p.mO : p.mH;
+      double xa[3] = {0.0, 0.0, 0.0};
+      if (al) {
+        double xi[3];
+        atom_noise(St.b, p, St.step, St.seed, atom, mass, St.kT, xi);
+        double Ft[3];
+        bool fin = true;
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+          Ft[t] = A.F[3 * (size_t)atom + t] + fpair[ml][3 * a + t];
+          fin = fin && isfinite(Ft[t]);
+        }
+        fin = step_fwd_atom(p, St.b, atom, mass, Ft, xi, St.dt, St.e, St.s, xa) && fin;
+        if (!fin) atomicOr(&A.flags[FLAG_NONFINITE], 1);
+        if (St.assign && a == 0) file_molecule(p, St.b.cell, St.b.cell_count, m, xa[0], xa[1], xa[2]);
+      }
+      double xm[9];
+      gather_mol3(lv ? 3 * (l / 3) : 0, xa, xm);
+      if (al) {
+        write_sorted(p, sizeof(R) == 4 ? RS_PRECISION_FP32 : RS_PRECISION_FP64, 3 * im + a, xa,
+                     St.posq_next, St.pos64_next);
+        double Fb[9], hb[9], vj[3];
+        bonded_mol<false>(p, xm, nullptr, nullptr, Fb, hb, vj, A.flags, St.b.O64[im]);
+#pragma unroll
+        for (int t = 0; t < 3; ++t) A.F[3 * (size_t)atom + t] = a == 0 ? Fb[t] : (a == 1 ? Fb[3 + t] : Fb[6 + t]);
+      }
+    }
+  } else if (c == 0) {
     double vj[6] = {0, 0, 0, 0, 0, 0};
     if (act) {
       const int m = A.order[i];
@@ -453,24 +508,33 @@ __global__ __launch_bounds__(RS_BLOCK) void k_vjp_reduce(const double* __restric
   if (threadIdx.x == 0) dth[k] += red[0];
 }
 
-template <class P, bool ADJ>
-void dispatch(const Launch& L, const PairArgs& a) {
-  int g = pair_blocks(L.p);
-  switch (L.p.CH) {
-    case 8:
-      if (!ADJ && a.mlp) k_pair<P, 8, ADJ, !ADJ><<<g, RS_BLOCK, 0, L.s>>>(a);
-      else k_pair<P, 8, ADJ, false><<<g, RS_BLOCK, 0, L.s>>>(a);
-      break;
-    default:
-      if (!ADJ && a.mlp) k_pair<P, 16, ADJ, !ADJ><<<g, RS_BLOCK, 0, L.s>>>(a);
-      else k_pair<P, 16, ADJ, false><<<g, RS_BLOCK, 0, L.s>>>(a);
-      break;
+template <class P, int CH, bool ADJ>
+void dispatch_ch(const Launch& L, const PairArgs& a, const StepArgs* st) {
+  const int g = pair_blocks(L.p);
+  StepArgs none{};
+  if constexpr (!ADJ) {
+    if (st) {
+      if (a.mlp) k_pair<P, CH, false, true, true><<<g, RS_BLOCK, 0, L.s>>>(a, *st);
+      else k_pair<P, CH, false, false, true><<<g, RS_BLOCK, 0, L.s>>>(a, *st);
+      return;
+    }
   }
+  if (!ADJ && a.mlp) k_pair<P, CH, ADJ, !ADJ, false><<<g, RS_BLOCK, 0, L.s>>>(a, none);
+  else k_pair<P, CH, ADJ, false, false><<<g, RS_BLOCK, 0, L.s>>>(a, none);
+}
+
+template <class P, bool ADJ>
+void dispatch(const Launch& L, const PairArgs& a, const StepArgs* st) {
+  if (L.p.CH == 8) dispatch_ch<P, 8, ADJ>(L, a, st);
+  else dispatch_ch<P, 16, ADJ>(L, a, st);
 }
 
 }  // namespace
 
-void launch_pair(const Launch& L, bool adjoint, const double* wsrc, double wscale_O, double wscale_H) {
+namespace {
+
+void pair_launch(const Launch& L, bool adjoint, const double* wsrc, double wscale_O,
+                 double wscale_H, const StepArgs* st) {
   PairArgs a;
   a.mlp = L.use_mlp;
   a.mtab = L.b.mlp_tab;
@@ -509,14 +573,36 @@ void launch_pair(const Launch& L, bool adjoint, const double* wsrc, double wscal
     a.pos = L.b.posq;
     a.u = nullptr;
     a.pu = L.b.pu;
-    if (adjoint) dispatch<P32, true>(L, a);
-    else dispatch<P32, false>(L, a);
+    if (adjoint) dispatch<P32, true>(L, a, nullptr);
+    else dispatch<P32, false>(L, a, st);
   } else {
     a.pos = L.b.pos64;
     a.u = L.b.ud;
-    if (adjoint) dispatch<P64, true>(L, a);
-    else dispatch<P64, false>(L, a);
+    if (adjoint) dispatch<P64, true>(L, a, nullptr);
+    else dispatch<P64, false>(L, a, st);
   }
+}
+
+}  // namespace
+
+void launch_pair(const Launch& L, bool adjoint, const double* wsrc, double wscale_O, double wscale_H) {
+  pair_launch(L, adjoint, wsrc, wscale_O, wscale_H, nullptr);
+}
+
+void launch_pair_step(const Launch& L, int step, uint64_t seed, double dt, double e, double s,
+                      double kT, bool assign) {
+  StepArgs st;
+  st.b = L.b;
+  st.posq_next = L.b.posq_alt;
+  st.pos64_next = L.b.pos64_alt;
+  st.step = step;
+  st.assign = assign;
+  st.seed = seed;
+  st.dt = dt;
+  st.e = e;
+  st.s = s;
+  st.kT = kT;
+  pair_launch(L, false, nullptr, 0.0, 0.0, &st);
 }
 
 void launch_vjp_reduce(const Launch& L) {

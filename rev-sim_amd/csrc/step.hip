@@ -1,7 +1,8 @@
-// step.hip — streaming kernels of the forward Langevin step and of the reverse-time adjoint.
+// step.hip — streaming kernel of the reverse-time adjoint step, and the per-molecule bonded,
+// seed, noise and clipping kernels.
 //
-// Forward step n (forward_sim loop, water/water.jl:1030-1037), FP64 state, one lane per atom:
-//   v += F dt/m; x += v dt/2; v = e v + s ξ_n; x += v dt/2; wrap       e = exp(-γdt), s = √(1-e²)
+// The forward step n (forward_sim loop, water/water.jl:1030-1037; integrator.h) runs at the end
+// of the forward pair kernel. It is inverted here, FP64 state, one lane per atom:
 // Reverse step (reverse_sim, water/water.jl:1496-1521), fused as [part 2 of step n+1][part 1 of n]:
 //   part 2: v -= F dt/m; B = e B + dt² h; A += B   (dL/dθ is accumulated by the pair kernel)
 //   part 1: x -= v dt/2; v = (v - s ξ_n)/e; x -= v dt/2; wrap; D = e D + dt² h (1+e)/2; C += D
@@ -10,6 +11,7 @@
 // Expressions follow the reference's broadcast order and are evaluated without contraction.
 #include "internal.h"
 #include "bonded.h"
+#include "integrator.h"
 
 namespace {
 
@@ -26,36 +28,10 @@ __device__ __forceinline__ void wmax_fold(double v, double* slots) {
     atomicMax((unsigned long long*)&slots[blockIdx.x & 63], (unsigned long long)__double_as_longlong(v));
 }
 
-__device__ __forceinline__ void write_sorted(const SysParams& p, int precision, int slot,
-                                             const double x[3], int* posq, double4* pos64) {
-  if (precision == RS_PRECISION_FP32)
-    for (int k = 0; k < 3; ++k) posq[3 * slot + k] = (int)quant1(x[k], p.qscale[k]);
-  else
-    pos64[slot] = make_double4(x[0], x[1], x[2], 0.0);
-}
-
 __global__ void k_wrap(SysParams p, double* x) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= 3 * p.N) return;
   x[t] = wrap1(x[t], p.L[t % 3]);
-}
-
-__device__ __forceinline__ void atom_noise(const DevBufs& b, const SysParams& p, int step,
-                                           uint64_t seed, int atom, double mass, double kT,
-                                           double xi[3]) {
-  if (b.ext_noise) {
-    const double* src = b.ext_noise + (size_t)(step - 1) * 3 * p.N + 3 * (size_t)atom;
-    xi[0] = src[0];
-    xi[1] = src[1];
-    xi[2] = src[2];
-  } else {
-    double nn[3];
-    normals3(seed, step, atom, nn);
-    const double sd = sqrt(kT / mass);
-    xi[0] = sd * nn[0];
-    xi[1] = sd * nn[1];
-    xi[2] = sd * nn[2];
-  }
 }
 
 // The step kernels run one lane per atom: a 64-lane block holds ATOM_MOLS molecules in lanes
@@ -82,51 +58,7 @@ __device__ __forceinline__ AtomLane atom_lane(const SysParams& p) {
 
 // The molecule's nine values (O, H1, H2) of a per-atom 3-vector held by its three lanes.
 __device__ __forceinline__ void gather_mol(const AtomLane& L, const double v[3], double out[9]) {
-#pragma unroll
-  for (int b = 0; b < 3; ++b)
-#pragma unroll
-    for (int k = 0; k < 3; ++k) out[3 * b + k] = __shfl(v[k], L.base + b, RS_WAVE);
-}
-
-// Forward step n; then the bonded forces at the new positions, which seed F for the next pair
-// kernel (it adds the pair forces).
-__global__ __launch_bounds__(RS_WAVE) void k_step_fwd(SysParams p, int precision, DevBufs b,
-                                                      int step, uint64_t seed, double dt,
-                                                      double e, double s, double kT, int assign) {
-#pragma clang fp contract(off)
-  const AtomLane L = atom_lane(p);
-  double xa[3] = {0.0, 0.0, 0.0};
-  if (L.act) {
-    const double mass = L.a == 0 ? p.mO : p.mH;
-    double xi[3];
-    atom_noise(b, p, step, seed, L.atom, mass, kT, xi);
-    bool fin = true;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      const size_t o = 3 * (size_t)L.atom + k;
-      double v = b.v[o] + (b.F[o] * dt) / mass;
-      double x = b.x[o] + (v * dt) / 2.0;
-      v = v * e + xi[k] * s;
-      x = x + (v * dt) / 2.0;
-      x = wrap1(x, p.L[k]);
-      b.v[o] = v;
-      b.x[o] = x;
-      xa[k] = x;
-      fin = fin && isfinite(x) && isfinite(v);
-    }
-    if (!fin) atomicOr(&b.flags[FLAG_NONFINITE], 1);
-    if (assign && L.a == 0) file_molecule(p, b.cell, b.cell_count, L.mol, xa[0], xa[1], xa[2]);
-  }
-  double xm[9];
-  gather_mol(L, xa, xm);
-  if (L.act) {
-    const int rk = b.rank[L.mol];
-    write_sorted(p, precision, 3 * rk + L.a, xa, b.posq, b.pos64);
-    double F[9], h[9], vj[3];
-    bonded_mol<false>(p, xm, nullptr, nullptr, F, h, vj, b.flags, b.O64[rk]);
-#pragma unroll
-    for (int k = 0; k < 3; ++k) b.F[3 * (size_t)L.atom + k] = F[3 * L.a + k];
-  }
+  gather_mol3(L.base, v, out);
 }
 
 // Reverse step: [part 2 of step n+1][part 1 of step n]; after part 1 the bonded forces, HVP and
@@ -357,12 +289,6 @@ __global__ void k_hnorm_final(int nparts, const double* part, double clip, doubl
 
 void launch_wrap_state(const Launch& L) {
   k_wrap<<<(3 * L.p.N + RS_BLOCK - 1) / RS_BLOCK, RS_BLOCK, 0, L.s>>>(L.p, L.b.x);
-}
-
-void launch_step_fwd(const Launch& L, int step, uint64_t seed, double dt, double e, double s,
-                     double kT, bool assign) {
-  k_step_fwd<<<(L.p.M + ATOM_MOLS - 1) / ATOM_MOLS, RS_WAVE, 0, L.s>>>(L.p, L.precision, L.b, step,
-                                                                       seed, dt, e, s, kT, assign);
 }
 
 void launch_bonded(const Launch& L, bool adjoint, const double* csrc, double cO, double cH,
