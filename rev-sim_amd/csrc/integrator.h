@@ -36,18 +36,19 @@ __device__ __forceinline__ void atom_noise(const DevBufs& b, const SysParams& p,
   }
 }
 
-// The forward step of one atom given its total force F; updates x and v in place and returns
-// the new position in xa. False when x or v is not finite.
+// The forward step of one atom from its position x0, velocity v0 and total force F; stores the
+// new x and v and returns the new position in xa. False when x or v is not finite.
 __device__ __forceinline__ bool step_fwd_atom(const SysParams& p, const DevBufs& b, int atom,
-                                              double mass, const double F[3], const double xi[3],
-                                              double dt, double e, double s, double xa[3]) {
+                                              double mass, const double x0[3], const double v0[3],
+                                              const double F[3], const double xi[3], double dt,
+                                              double e, double s, double xa[3]) {
 #pragma clang fp contract(off)
   bool fin = true;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     const size_t o = 3 * (size_t)atom + k;
-    double v = b.v[o] + (F[k] * dt) / mass;
-    double x = b.x[o] + (v * dt) / 2.0;
+    double v = v0[k] + (F[k] * dt) / mass;
+    double x = x0[k] + (v * dt) / 2.0;
     v = v * e + xi[k] * s;
     x = x + (v * dt) / 2.0;
     x = wrap1(x, p.L[k]);

@@ -191,6 +191,36 @@ void k_pair(PairArgs A, StepArgs St) {
     }
   };
 
+  // STEP: the step's inputs that do not depend on this kernel's results (molecule index, x, v,
+  // the F seed, the noise) are read and generated by the step waves before their pair loops,
+  // while the other waves keep the SIMDs busy, and wait in LDS for the epilogue
+  constexpr int NM = RS_BLOCK / CH, MPW = RS_WAVE / 3;  // molecules per block, per step wave
+  struct StepIn {
+    double x[3], v[3], f[3], xi[3];
+  };
+  __shared__ StepIn stin[STEP ? 3 * NM : 1];
+  __shared__ int smol[STEP ? NM : 1];
+  const int sw = threadIdx.x / RS_WAVE, sl = threadIdx.x % RS_WAVE;  // step wave and lane
+  const bool slv = sl < 3 * MPW;
+  const int sml = sw * MPW + (slv ? sl / 3 : 0), sa = slv ? sl % 3 : 0;  // molecule, atom
+  const int sim = blk * NM + sml;                                        // its sorted index
+  const bool sal = slv && sml < NM && sim < p.M;
+  if constexpr (STEP) {
+    if (sal) {
+      const int m = A.order[sim];
+      const int atom = 3 * m + sa;
+      StepIn& q = stin[3 * sml + sa];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        q.x[k] = St.b.x[3 * (size_t)atom + k];
+        q.v[k] = St.b.v[3 * (size_t)atom + k];
+        q.f[k] = A.F[3 * (size_t)atom + k];
+      }
+      atom_noise(St.b, p, St.step, St.seed, atom, sa == 0 ? p.mO : p.mH, St.kT, q.xi);
+      if (sa == 0) smol[sml] = m;
+    }
+  }
+
   Pos own[3];
   R uo[3][3];
   R F[3][3], hh[3][3], Flj[3], Fs[3], Fml[3];
@@ -388,46 +418,39 @@ void k_pair(PairArgs A, StepArgs St) {
     // through LDS to as few waves as hold three lanes per molecule (one wave for CH = 16): the
     // step's FP64 work then runs on 48 of 64 lanes rather than 3 of every 16.
     static_assert(!ADJ, "the step variant is forward-only");
-    constexpr int NM = RS_BLOCK / CH, MPW = RS_WAVE / 3;  // molecules per block, per step wave
     __shared__ double fpair[NM][9];
     if (c == 0) {
 #pragma unroll
       for (int t = 0; t < 9; ++t) fpair[threadIdx.x / CH][t] = Fd[t];
     }
     __syncthreads();
-    const int w = threadIdx.x / RS_WAVE, l = threadIdx.x % RS_WAVE;
-    if (w * MPW < NM) {  // wave-uniform
-      const bool lv = l < 3 * MPW;
-      const int ml = w * MPW + (lv ? l / 3 : 0), a = lv ? l % 3 : 0;
-      const int im = blk * NM + ml;
-      const bool al = lv && ml < NM && im < p.M;
-      const int m = al ? A.order[im] : 0;
-      const int atom = 3 * m + a;
-      const double mass = a == 0 ? p.mO : p.mH;
+    if (sw * MPW < NM) {  // wave-uniform
+      const int m = sal ? smol[sml] : 0;
+      const int atom = 3 * m + sa;
+      const double mass = sa == 0 ? p.mO : p.mH;
       double xa[3] = {0.0, 0.0, 0.0};
-      if (al) {
-        double xi[3];
-        atom_noise(St.b, p, St.step, St.seed, atom, mass, St.kT, xi);
+      if (sal) {
+        const StepIn& q = stin[3 * sml + sa];
         double Ft[3];
         bool fin = true;
 #pragma unroll
         for (int t = 0; t < 3; ++t) {
-          Ft[t] = A.F[3 * (size_t)atom + t] + fpair[ml][3 * a + t];
+          Ft[t] = q.f[t] + fpair[sml][3 * sa + t];
           fin = fin && isfinite(Ft[t]);
         }
-        fin = step_fwd_atom(p, St.b, atom, mass, Ft, xi, St.dt, St.e, St.s, xa) && fin;
+        fin = step_fwd_atom(p, St.b, atom, mass, q.x, q.v, Ft, q.xi, St.dt, St.e, St.s, xa) && fin;
         if (!fin) atomicOr(&A.flags[FLAG_NONFINITE], 1);
-        if (St.assign && a == 0) file_molecule(p, St.b.cell, St.b.cell_count, m, xa[0], xa[1], xa[2]);
+        if (St.assign && sa == 0) file_molecule(p, St.b.cell, St.b.cell_count, m, xa[0], xa[1], xa[2]);
       }
       double xm[9];
-      gather_mol3(lv ? 3 * (l / 3) : 0, xa, xm);
-      if (al) {
-        write_sorted(p, sizeof(R) == 4 ? RS_PRECISION_FP32 : RS_PRECISION_FP64, 3 * im + a, xa,
+      gather_mol3(slv ? 3 * (sl / 3) : 0, xa, xm);
+      if (sal) {
+        write_sorted(p, sizeof(R) == 4 ? RS_PRECISION_FP32 : RS_PRECISION_FP64, 3 * sim + sa, xa,
                      St.posq_next, St.pos64_next);
         double Fb[9], hb[9], vj[3];
-        bonded_mol<false>(p, xm, nullptr, nullptr, Fb, hb, vj, A.flags, St.b.O64[im]);
+        bonded_mol<false>(p, xm, nullptr, nullptr, Fb, hb, vj, A.flags, St.b.O64[sim]);
 #pragma unroll
-        for (int t = 0; t < 3; ++t) A.F[3 * (size_t)atom + t] = a == 0 ? Fb[t] : (a == 1 ? Fb[3 + t] : Fb[6 + t]);
+        for (int t = 0; t < 3; ++t) A.F[3 * (size_t)atom + t] = sa == 0 ? Fb[t] : (sa == 1 ? Fb[3 + t] : Fb[6 + t]);
       }
     }
   } else if (c == 0) {
