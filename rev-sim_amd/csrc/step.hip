@@ -7,7 +7,8 @@
 //   part 2: v -= F dt/m; B = e B + dt² h; A += B   (dL/dθ is accumulated by the pair kernel)
 //   part 1: x -= v dt/2; v = (v - s ξ_n)/e; x -= v dt/2; wrap; D = e D + dt² h (1+e)/2; C += D
 // ξ_n is regenerated from the Philox counter (step n, atom) — the same bits forward and reverse.
-// Every lane also writes its atom's slot of the sorted pair inputs for the next pair kernel.
+// Every lane also writes its atom's slot of the sorted adjoint pair inputs for the next pair
+// kernel.
 // Expressions follow the reference's broadcast order and are evaluated without contraction.
 #include "internal.h"
 #include "bonded.h"
@@ -165,13 +166,16 @@ __global__ __launch_bounds__(RS_WAVE) void k_step_rev(SysParams p, int precision
     gather_mol(L, ua, um);
     gather_mol(L, wa, wm);
     if (L.act) {
-      write_sorted(p, precision, 3 * rk + L.a, xa, b.posq, b.pos64);
+      // the adjoint pair inputs: FP32, the record (positions and u) alone; posq is not read
+      // before the next list build repacks it
       if (precision == RS_PRECISION_FP32) {
         int q[3];
         for (int k = 0; k < 3; ++k) q[k] = (int)quant1(xa[k], p.qscale[k]);
         write_pair_rec(b.pu, rk, L.a, q, ua);
-      } else
+      } else {
+        b.pos64[3 * rk + L.a] = make_double4(xa[0], xa[1], xa[2], 0.0);
         b.ud[3 * rk + L.a] = make_double4(ua[0], ua[1], ua[2], 0.0);
+      }
       wv = fmax(fabs(wm[0]), fmax(fabs(wm[1]), fabs(wm[2])));
       double F[9], h[9], vj[3];
       bonded_mol<true>(p, xm, um, wm, F, h, vj, b.flags, ob);
