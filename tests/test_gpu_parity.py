@@ -183,6 +183,29 @@ def test_value_and_grad(gpu_lib, prec, tol, box216):
     assert rel_err(g, go) < tol, (g, go)
 
 
+@pytest.mark.parametrize("prec,tol", [(FP64, 1e-8), (FP32, 1e-4)])
+def test_eight_lanes_per_molecule(gpu_lib, prec, tol, box216):
+    """pair_chunks = 8: the 8-lane pair kernels, whose forward kernel runs the integrator step on
+    two waves per block (32 molecules; 216 waters leave the last block partly filled). The forward
+    trajectory and the loss gradient against the oracle, to the 16-lane tolerances."""
+    import revsim
+
+    sim, orc = _pair(prec, pair_chunks=8)
+    x, v, _ = sim.forward(box216.coords, box216.velocities, 13, 50)
+    xo, vo, _ = orc.forward(box216.coords, box216.velocities, 13, 50)
+    dx = x - xo
+    dx -= 1.862 * np.round(dx / 1.862)
+    assert np.max(np.abs(dx)) < (1e-10 if prec == FP64 else 1e-5)
+    g0, _, _, _ = orc.rdf(box216.coords)
+    tgt = revsim.perturbed_target(g0)
+    sim.set_target_rdf(tgt)
+    orc.set_target_rdf(tgt)
+    L, g = sim.value_and_grad(box216.coords, box216.velocities, 13, 50)
+    Lo, go = orc.value_and_grad(box216.coords, box216.velocities, 13, 50)
+    assert abs(L - Lo) <= tol * abs(Lo)
+    assert rel_err(g, go) < tol, (g, go)
+
+
 def test_reversibility_fp64(gpu_lib, box216):
     import revsim
 
