@@ -174,7 +174,7 @@ def main():
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
-        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
+        print(json.dumps({"error": "bench.py measures one GPU; runs on more than one GPU are not supported",
                           "n_gpus": a.gpus}))
         sys.exit(2)
 

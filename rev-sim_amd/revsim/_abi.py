@@ -75,7 +75,7 @@ class RsConfig(ctypes.Structure):
 
 
 def make_config(n_molecules: int, box, cutoff: float, **kw) -> RsConfig:
-    """Water defaults (water/water.jl:27-49 constants, BASELINE physics) with overrides."""
+    """Water defaults (water/water.jl:27-49 constants) with overrides."""
     c = RsConfig()
     c.abi_version = RS_ABI_VERSION
     c.n_molecules = int(n_molecules)

@@ -5,7 +5,7 @@ the MLP disabled.
 The target RDF comes from a "teacher" liquid: the classical force field at the starting θ plus a
 hidden O-O pair term (an MLP of the same architecture, another seed, output layer x8) that the six
 classical parameters cannot represent. Plain θ training can only partly compensate; the MLP
-correction learns the hidden term, and both are judged on a held-out loss (fresh seeds, a 300-step
+correction learns the hidden term, and both are compared on a held-out loss (fresh seeds, a 300-step
 re-equilibration under the trained model, then a 500-step loss run)."""
 import numpy as np
 import pytest
