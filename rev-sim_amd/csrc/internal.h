@@ -70,8 +70,10 @@ struct Launch {
 void launch_list_build(const Launch& L, bool with_u, bool filed = false);
 // pair
 void launch_pair(const Launch& L, bool adjoint, const double* wsrc, double wscale_O, double wscale_H);
-// forward pair kernel followed, in the same kernel, by the forward step n (launch_step_fwd's
-// arguments); writes the new sorted positions to posq_alt / pos64_alt, which the caller swaps in
+// forward pair kernel followed, in the same kernel, by the forward step n (Philox seed, dt,
+// e = exp(-γ dt), s = sqrt(1 - e^2), kT; assign: file each molecule into its cell for the list
+// build that follows); writes the new sorted positions to posq_alt / pos64_alt, which the caller
+// swaps in
 void launch_pair_step(const Launch& L, int step, uint64_t seed, double dt, double e, double s,
                       double kT, bool assign);
 void launch_vjp_reduce(const Launch& L);
