@@ -137,6 +137,11 @@ int rs_set_external_noise(rs_ctx* ctx, const double* xi, int32_t n_steps);
    needs. x_out/v_out (nullable) receive the final state, loss_out (nullable) the loss. */
 int rs_forward(rs_ctx* ctx, const double* x0, const double* v0, uint64_t seed, int32_t n_steps,
                double* x_out, double* v_out, double* loss_out);
+/* The reverse-only arguments of reverse_sim (water/water.jl:1435-1436): n_steps_trunc (0: none)
+   and clip_norm_threshold (<= 0 or inf: none), replacing the config's n_steps_trunc and clip_norm.
+   The records of the last rs_forward stay valid, so this may come between rs_forward and
+   rs_reverse. */
+int rs_set_reverse_options(rs_ctx* ctx, int32_t n_steps_trunc, double clip_norm);
 /* Reverse-time adjoint from the final state of the last rs_forward (same seed and n_steps).
    Returns the loss, dL/dtheta (6), and (nullable) the reconstructed initial state. */
 int rs_reverse(rs_ctx* ctx, const double* xN, const double* vN, uint64_t seed, int32_t n_steps,
@@ -144,6 +149,21 @@ int rs_reverse(rs_ctx* ctx, const double* xN, const double* vN, uint64_t seed, i
 /* rs_forward followed by rs_reverse. dL_dmlp (nullable) receives the MLP weight gradient. */
 int rs_value_and_grad(rs_ctx* ctx, const double* x0, const double* v0, uint64_t seed,
                       int32_t n_steps, double* loss_out, double* dL_dtheta6, float* dL_dmlp);
+/* The loss terms of the last rs_forward / rs_reverse / rs_value_and_grad: terms4 =
+   (l_rdf, l_ev, l_msd, D), the extra return values of the reference's reverse_sim
+   (sys, l, dl_dp, l_rdf, l_ev), water/water.jl:1549-1557, and of the diffusion variant
+   (sys, loss, dl_dp, D, l_diff, l_rdf, l_ev), diffusion/diffusion.jl:1605-1616: the RDF and
+   enthalpy-of-vaporization terms averaged over loss blocks (weights applied), the MSD term, and the
+   diffusion coefficient D in m^2/s. l_rdf + l_ev + l_msd is the loss. After rs_reverse the first
+   two are the reverse pass's own block losses. */
+int rs_loss_terms(const rs_ctx* ctx, double* terms4);
+/* Counters (no counterpart in the reference): stats[RS_STAT_RERUNS_CALL] list-driven reruns of the
+   last call that ran device work (a neighbour-list capacity overflow, a stale list, an O-H bond
+   beyond the list's extent or a crowded cell made it widen the list and run the call again),
+   stats[RS_STAT_BUILDS_CALL] its list builds (all attempts), stats[RS_STAT_RERUNS_TOTAL] reruns
+   since the context was created. n: entries wanted (<= RS_STAT_COUNT). */
+enum { RS_STAT_RERUNS_CALL = 0, RS_STAT_BUILDS_CALL, RS_STAT_RERUNS_TOTAL, RS_STAT_COUNT };
+int rs_stats(const rs_ctx* ctx, int64_t* stats, int32_t n);
 
 /* --- radial-basis MLP correction on O-O pairs (rs_config.use_mlp = 1) ---------------------
    u(r) = w3.SiLU(W2 SiLU(W1 φ(r) + b1) + b2) + b3 per O-O pair within the cutoff, 16 Gaussian

@@ -57,6 +57,12 @@ typedef struct {
   double* v;
 } snap_t;
 
+/* growable list of atom pairs (a, b): one per thread while a neighbour list is built */
+typedef struct pairbuf {
+  int32_t *a, *b;
+  int64_t n, cap;
+} pairbuf_t;
+
 typedef struct or_ctx {
   rs_config cfg;
   int M, N, N3;
@@ -91,6 +97,16 @@ typedef struct or_ctx {
   int use_mlp;
   double* mlp;      /* NW weights: W1[64][16], b1[64], W2[64][64], b2[64], w3[64], b3 */
   double* dW;       /* dL/dW of the last reverse pass */
+  /* per-thread force / HVP buffers of eval_terms and the u = c/m scratch, allocated once per
+     context (grown when the thread count grows) and zeroed by their own threads */
+  double* tbuf;
+  int tbuf_threads;
+  pairbuf_t* qbuf; /* per-thread list-build buffers, kept between builds */
+  int qbuf_threads;
+  double* ubuf;
+  /* loss terms of the last trajectory call: RDF, enthalpy of vaporization, MSD (averaged over
+     blocks, weights applied) and D (nm^2/ps) */
+  double terms[4];
   char err[256];
 } or_ctx;
 
@@ -191,6 +207,13 @@ int or_destroy(or_ctx* o) {
   free(o->pa);
   free(o->pb);
   free(o->cnt);
+  free(o->tbuf);
+  free(o->ubuf);
+  for (int t = 0; t < o->qbuf_threads; ++t) {
+    free(o->qbuf[t].a);
+    free(o->qbuf[t].b);
+  }
+  free(o->qbuf);
   free(o->target);
   free(o->ext);
   free(o);
@@ -264,105 +287,152 @@ int or_set_external_noise(or_ctx* o, const double* xi, int32_t n_steps) {
 /* ---------------------------------------------------------------- neighbour list (atoms) */
 
 /* Half list of atom pairs in different molecules with FP64 minimum-image distance <= rlist.
-   Plays the role of Molly's CellListMap finder (water/water.jl:1025,1038). Cells of edge >= rlist;
-   fewer than 3 cells along any axis falls back to all pairs. Order: by a, then cell order, then
-   b ascending — deterministic. */
+   Plays the role of Molly's CellListMap finder (water/water.jl:1025,1038). Built by molecule: an
+   atom pair within rlist has an O-O distance within R = rlist + 2 h, h the largest O-H distance
+   at this state (triangle inequality), so molecule pairs i<j with O-O <= R (cells of edge >= R/2,
+   a 5^3 stencil; 3^3 or all pairs in small boxes) are expanded to their nine atom pairs, each
+   kept if within rlist. One pass: each thread takes a contiguous range of i (static schedule) and
+   appends to its own buffer; the buffers are joined in thread order. Order: by i, stencil order,
+   j ascending within a cell, then atom order — deterministic for a given thread count. */
+static void pairbuf_grow(pairbuf_t* q, int64_t need) {
+  if (need <= q->cap) return;
+  int64_t cap = q->cap ? 2 * q->cap : 65536;
+  while (cap < need) cap *= 2;
+  q->a = (int32_t*)realloc(q->a, sizeof(int32_t) * cap);
+  q->b = (int32_t*)realloc(q->b, sizeof(int32_t) * cap);
+  q->cap = cap;
+}
+
 static void build_list(or_ctx* o, const double* x) {
-  const int N = o->N;
-  int nc[3], all = 0;
+  const int M = o->M;
+  double hmax = 0.0;
+#pragma omp parallel for schedule(static) reduction(max : hmax)
+  for (int m = 0; m < M; ++m)
+    for (int a = 1; a <= 2; ++a) {
+      double dr[3];
+      mic3(o, x + 9 * (size_t)m, x + 9 * (size_t)m + 3 * a, dr);
+      const double r = sqrt(r2_of(dr));
+      if (r > hmax) hmax = r;
+    }
+  const double R = o->rlist + 2.0 * hmax * (1.0 + 1e-12);
+  const double R2 = R * R, rl2 = o->rlist * o->rlist;
+  int nc[3], all = 0, S = 2;
+  for (int k = 0; k < 3; ++k)
+    if ((int)floor(o->L[k] * 2.0 / R) < 5) S = 1;
   for (int k = 0; k < 3; ++k) {
-    nc[k] = (int)floor(o->L[k] / o->rlist);
-    if (nc[k] < 3) all = 1;
+    nc[k] = (int)floor(o->L[k] * S / R);
+    if (nc[k] < 2 * S + 1) all = 1;
   }
-  double rl2 = o->rlist * o->rlist;
-  if (!o->cnt) o->cnt = (int64_t*)malloc(sizeof(int64_t) * (N + 1));
   int* cell_of = NULL;
   int *cstart = NULL, *cidx = NULL;
   int ncell = 1;
   if (!all) {
     ncell = nc[0] * nc[1] * nc[2];
-    cell_of = (int*)malloc(sizeof(int) * N);
+    cell_of = (int*)malloc(sizeof(int) * M);
     cstart = (int*)calloc(ncell + 1, sizeof(int));
-    cidx = (int*)malloc(sizeof(int) * N);
-    for (int a = 0; a < N; ++a) {
+    cidx = (int*)malloc(sizeof(int) * M);
+#pragma omp parallel for schedule(static)
+    for (int m = 0; m < M; ++m) {
       int c[3];
       for (int k = 0; k < 3; ++k) {
-        double w = wrap1(x[3 * a + k], o->L[k]);
+        double w = wrap1(x[9 * (size_t)m + k], o->L[k]);
         c[k] = (int)(w / o->L[k] * nc[k]);
         if (c[k] >= nc[k]) c[k] = nc[k] - 1;
         if (c[k] < 0) c[k] = 0;
       }
-      cell_of[a] = (c[2] * nc[1] + c[1]) * nc[0] + c[0];
-      cstart[cell_of[a] + 1]++;
+      cell_of[m] = (c[2] * nc[1] + c[1]) * nc[0] + c[0];
     }
+    for (int m = 0; m < M; ++m) cstart[cell_of[m] + 1]++;
     for (int c = 0; c < ncell; ++c) cstart[c + 1] += cstart[c];
     int* fill = (int*)malloc(sizeof(int) * ncell);
     memcpy(fill, cstart, sizeof(int) * ncell);
-    for (int a = 0; a < N; ++a) cidx[fill[cell_of[a]]++] = a; /* ascending a within a cell */
+    for (int m = 0; m < M; ++m) cidx[fill[cell_of[m]]++] = m; /* ascending within a cell */
     free(fill);
   }
-  for (int pass = 0; pass < 2; ++pass) {
-#pragma omp parallel for schedule(dynamic, 64)
-    for (int a = 0; a < N; ++a) {
-      int64_t n = 0, base = pass ? o->cnt[a] : 0;
-      int ma = a / 3;
-      if (all) {
-        for (int b = a + 1; b < N; ++b) {
-          if (b / 3 == ma) continue;
-          double dr[3];
-          mic3(o, x + 3 * a, x + 3 * b, dr);
-          if (r2_of(dr) <= rl2) {
-            if (pass) {
-              o->pa[base + n] = a;
-              o->pb[base + n] = b;
+  int nth = 1;
+#ifdef _OPENMP
+  nth = omp_get_max_threads();
+#endif
+  if (o->qbuf_threads < nth) {
+    o->qbuf = (pairbuf_t*)realloc(o->qbuf, sizeof(pairbuf_t) * nth);
+    memset(o->qbuf + o->qbuf_threads, 0, sizeof(pairbuf_t) * (nth - o->qbuf_threads));
+    o->qbuf_threads = nth;
+  }
+  pairbuf_t* qb = o->qbuf;
+  for (int t = 0; t < nth; ++t) qb[t].n = 0;
+#pragma omp parallel num_threads(nth)
+  {
+    int t = 0;
+#ifdef _OPENMP
+    t = omp_get_thread_num();
+#endif
+    pairbuf_t* q = qb + t;
+#pragma omp for schedule(static)
+    for (int i = 0; i < M; ++i) {
+      const double* xi = x + 9 * (size_t)i;
+      const int ci = all ? 0 : cell_of[i];
+      const int cx = ci % nc[0], cy = (ci / nc[0]) % nc[1], cz = ci / (nc[0] * nc[1]);
+      const int span = all ? 0 : S;
+      for (int dz = -span; dz <= span; ++dz)
+        for (int dy = -span; dy <= span; ++dy)
+          for (int dx = -span; dx <= span; ++dx) {
+            int lo = i + 1, hi = M;
+            if (!all) {
+              const int ex = (cx + dx + nc[0]) % nc[0], ey = (cy + dy + nc[1]) % nc[1],
+                        ez = (cz + dz + nc[2]) % nc[2];
+              const int e = (ez * nc[1] + ey) * nc[0] + ex;
+              lo = cstart[e];
+              hi = cstart[e + 1];
             }
-            ++n;
-          }
-        }
-      } else {
-        int c = cell_of[a];
-        int cx = c % nc[0], cy = (c / nc[0]) % nc[1], cz = c / (nc[0] * nc[1]);
-        for (int dz = -1; dz <= 1; ++dz)
-          for (int dy = -1; dy <= 1; ++dy)
-            for (int dx = -1; dx <= 1; ++dx) {
-              int ex = (cx + dx + nc[0]) % nc[0], ey = (cy + dy + nc[1]) % nc[1],
-                  ez = (cz + dz + nc[2]) % nc[2];
-              int e = (ez * nc[1] + ey) * nc[0] + ex;
-              for (int t = cstart[e]; t < cstart[e + 1]; ++t) {
-                int b = cidx[t];
-                if (b <= a || b / 3 == ma) continue;
-                double dr[3];
-                mic3(o, x + 3 * a, x + 3 * b, dr);
-                if (r2_of(dr) <= rl2) {
-                  if (pass) {
-                    o->pa[base + n] = a;
-                    o->pb[base + n] = b;
-                  }
-                  ++n;
+            for (int u = lo; u < hi; ++u) {
+              const int j = all ? u : cidx[u];
+              if (j <= i) continue;
+              const double* xj = x + 9 * (size_t)j;
+              double dr[3];
+              mic3(o, xi, xj, dr);
+              if (r2_of(dr) > R2) continue;
+              pairbuf_grow(q, q->n + 9);
+              int64_t k = q->n;
+              for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) {
+                  mic3(o, xi + 3 * a, xj + 3 * b, dr);
+                  q->a[k] = 3 * i + a;
+                  q->b[k] = 3 * j + b;
+                  k += r2_of(dr) <= rl2;
                 }
-              }
+              q->n = k;
             }
-      }
-      if (!pass) o->cnt[a] = n;
-    }
-    if (!pass) {
-      int64_t tot = 0;
-      for (int a = 0; a < N; ++a) {
-        int64_t c = o->cnt[a];
-        o->cnt[a] = tot;
-        tot += c;
-      }
-      o->cnt[N] = tot;
-      if (tot > o->cap) {
-        free(o->pa);
-        free(o->pb);
-        o->cap = tot + tot / 4 + 16;
-        o->pa = (int32_t*)malloc(sizeof(int32_t) * o->cap);
-        o->pb = (int32_t*)malloc(sizeof(int32_t) * o->cap);
-      }
-      o->npairs = tot;
+          }
     }
   }
+  int64_t tot = 0;
+  for (int t = 0; t < nth; ++t) tot += qb[t].n;
+  if (tot > o->cap) {
+    free(o->pa);
+    free(o->pb);
+    o->cap = tot + tot / 4 + 16;
+    o->pa = (int32_t*)malloc(sizeof(int32_t) * o->cap);
+    o->pb = (int32_t*)malloc(sizeof(int32_t) * o->cap);
+  }
+  /* join the thread buffers in thread order, each thread copying its own */
+#pragma omp parallel num_threads(nth)
+  {
+    int t = 0;
+#ifdef _OPENMP
+    t = omp_get_thread_num();
+#endif
+    int T = 1;
+#ifdef _OPENMP
+    T = omp_get_num_threads();
+#endif
+    for (int w = t; w < nth; w += T) {
+      int64_t off = 0;
+      for (int u = 0; u < w; ++u) off += qb[u].n;
+      memcpy(o->pa + off, qb[w].a, sizeof(int32_t) * qb[w].n);
+      memcpy(o->pb + off, qb[w].b, sizeof(int32_t) * qb[w].n);
+    }
+  }
+  o->npairs = tot;
   free(cell_of);
   free(cstart);
   free(cidx);
@@ -671,8 +741,14 @@ static void eval_terms(or_ctx* o, const double* x, const double* c, const double
 #ifdef _OPENMP
   nth = omp_get_max_threads();
 #endif
-  double* Fb = (double*)calloc((size_t)N3 * nth, sizeof(double));
-  double* hb = adj ? (double*)calloc((size_t)N3 * nth, sizeof(double)) : NULL;
+  if (o->tbuf_threads < nth) {
+    free(o->tbuf);
+    o->tbuf = (double*)malloc(sizeof(double) * 2 * (size_t)N3 * nth);
+    o->tbuf_threads = nth;
+  }
+  if (!o->ubuf) o->ubuf = (double*)malloc(sizeof(double) * N3);
+  double* Fb = o->tbuf;
+  double* hb = adj ? o->tbuf + (size_t)N3 * nth : NULL;
   double* vb = (double*)calloc((size_t)NTH * nth, sizeof(double));
   double* eb = (double*)calloc((size_t)nth, sizeof(double));
   double* db = (double*)calloc((size_t)NTH * nth, sizeof(double)); /* dU/dθ per thread */
@@ -680,11 +756,12 @@ static void eval_terms(or_ctx* o, const double* x, const double* c, const double
   double* dWb = (adj && mlp_on && dW) ? (double*)calloc((size_t)MLP_NW * nth, sizeof(double)) : NULL;
   double* u = NULL;
   if (adj) {
-    u = (double*)malloc(sizeof(double) * N3);
+    u = o->ubuf;
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < N3; ++i) u[i] = c[i] / o->mass3[i];
   }
   const int64_t np = o->npairs;
-#pragma omp parallel
+#pragma omp parallel num_threads(nth)
   {
     int t = 0;
 #ifdef _OPENMP
@@ -692,6 +769,17 @@ static void eval_terms(or_ctx* o, const double* x, const double* c, const double
 #endif
     double* Ft = Fb + (size_t)N3 * t;
     double* ht = adj ? hb + (size_t)N3 * t : NULL;
+    {
+      /* each thread zeroes its own buffer (and any of a team smaller than requested) */
+      int T = 1;
+#ifdef _OPENMP
+      T = omp_get_num_threads();
+#endif
+      for (int w = t; w < nth; w += T) {
+        memset(Fb + (size_t)N3 * w, 0, sizeof(double) * N3);
+        if (adj) memset(hb + (size_t)N3 * w, 0, sizeof(double) * N3);
+      }
+    }
     double* vt = vb + NTH * t;
     double* dUt = db + NTH * t;
     double* dWt = dWb ? dWb + (size_t)MLP_NW * t : NULL;
@@ -840,6 +928,8 @@ static void eval_terms(or_ctx* o, const double* x, const double* c, const double
     }
     eb[t] = et;
   }
+  /* thread buffers summed in thread order (deterministic for a given thread count) */
+#pragma omp parallel for schedule(static)
   for (int i = 0; i < N3; ++i) {
     double s = 0.0;
     for (int t = 0; t < nth; ++t) s += Fb[(size_t)N3 * t + i];
@@ -877,12 +967,9 @@ static void eval_terms(or_ctx* o, const double* x, const double* c, const double
     }
     free(dWb);
   }
-  free(Fb);
-  free(hb);
   free(vb);
   free(eb);
   free(db);
-  free(u);
 }
 
 int or_forces(or_ctx* o, const double* x, double* F, double* energy) {
@@ -1450,10 +1537,15 @@ static int forward_impl(or_ctx* o, const double* x0, const double* v0, uint64_t 
   }
   double loss = o->cfg.w_rdf * lrdf / p.n_blocks + ldh / p.n_blocks;
   free(Fe);
+  o->terms[0] = o->cfg.w_rdf * lrdf / p.n_blocks;
+  o->terms[1] = ldh / p.n_blocks;
+  o->terms[2] = o->terms[3] = 0.0;
   if (p.msd_on) {
-    double lm;
-    or_msd(o, o->frames, p.nf, NULL, &lm, NULL);
+    double lm, Dm;
+    or_msd(o, o->frames, p.nf, &Dm, &lm, NULL);
     loss += lm;
+    o->terms[2] = lm;
+    o->terms[3] = Dm;
   }
   o->rec_steps = baro ? 0 : n_steps; /* a trajectory that moved the box has no reverse pass */
   o->rec_seed = seed;
@@ -1473,7 +1565,8 @@ int or_forward(or_ctx* o, const double* x0, const double* v0, uint64_t seed, int
 }
 
 /* Gradient seed at state t: RDF and enthalpy-of-vaporization block losses (weights w/n_blocks;
-   the latter also adds its direct term -c' dU/dθ into dth) and the MSD frame gradient. */
+   the latter also adds its direct term -c' dU/dθ into dth) and the MSD frame gradient. The two
+   block-loss terms are added to terms[0] and terms[1]. */
 static double seed_grad(or_ctx* o, const plan_t* p, int t, const double* x, const double* dLdf,
                         double* g, int with_rdf, double* dth) {
   const int N3 = o->N3, M = o->M;
@@ -1485,13 +1578,16 @@ static double seed_grad(or_ctx* o, const plan_t* p, int t, const double* x, cons
     double wgt = o->cfg.w_rdf / p->n_blocks;
     for (int i = 0; i < N3; ++i) g[i] += wgt * gr[i];
     l *= wgt;
+    o->terms[0] += l;
     free(gr);
   }
   if (with_rdf && o->cfg.w_dhvap != 0.0) {
     double* Fe = (double*)malloc(sizeof(double) * N3);
     double dU[NTH], cp;
     const double wgt = 1.0 / p->n_blocks;
-    l += wgt * dhvap_block(o, x, Fe, dU, &cp);
+    const double le = wgt * dhvap_block(o, x, Fe, dU, &cp);
+    l += le;
+    o->terms[1] += le;
     for (int i = 0; i < N3; ++i) g[i] += wgt * cp * Fe[i];
     for (int k = 0; k < NTH; ++k) dth[k] -= wgt * cp * dU[k];
     free(Fe);
@@ -1524,11 +1620,14 @@ int or_reverse(or_ctx* o, const double* xN, const double* vN, uint64_t seed, int
   double loss = 0.0;
   memcpy(x, xN, sizeof(double) * N3);
   memcpy(v, vN, sizeof(double) * N3);
+  o->terms[0] = o->terms[1] = o->terms[2] = o->terms[3] = 0.0;
   if (p.msd_on) {
-    double lm;
+    double lm, Dm;
     dLdf = (double*)malloc(sizeof(double) * (size_t)p.nf * 3 * M);
-    or_msd(o, o->frames, p.nf, NULL, &lm, dLdf);
+    or_msd(o, o->frames, p.nf, &Dm, &lm, dLdf);
     loss += lm;
+    o->terms[2] = lm;
+    o->terms[3] = Dm;
   }
   double th[NTH] = {0};
   loss += seed_grad(o, &p, n_steps, x, dLdf, g, 1, th);
@@ -1674,7 +1773,10 @@ int or_value_and_grad_stored(or_ctx* o, const double* x0, const double* v0, uint
     or_msd(o, o->frames, p.nf, NULL, NULL, dLdf);
   }
   double th[NTH] = {0};
+  double terms_fwd[4];
+  memcpy(terms_fwd, o->terms, sizeof(terms_fwd));
   seed_grad(o, &p, n_steps, traj + (size_t)n_steps * N3, dLdf, lx, 1, th);
+  memcpy(o->terms, terms_fwd, sizeof(terms_fwd)); /* the forward pass's terms stand */
   if (o->use_mlp) {
     if (!o->dW) o->dW = (double*)malloc(sizeof(double) * MLP_NW);
     memset(o->dW, 0, sizeof(double) * MLP_NW);
@@ -1733,6 +1835,15 @@ int or_reweight(or_ctx* o, const double* x0, const double* v0, uint64_t seed, in
 }
 
 double or_list_radius(const or_ctx* o) { return o->rlist; }
+
+/* (l_rdf, l_ev, l_msd, D in m^2/s) of the last forward / reverse call, as rs_loss_terms. */
+int or_loss_terms(const or_ctx* o, double* t) {
+  t[0] = o->terms[0];
+  t[1] = o->terms[1];
+  t[2] = o->terms[2];
+  t[3] = o->terms[3] * 1e-6;
+  return 0;
+}
 
 int or_get_box(const or_ctx* o, double* box) {
   for (int k = 0; k < 3; ++k) box[k] = o->L[k];

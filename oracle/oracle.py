@@ -41,6 +41,7 @@ _SIGS = [
     ("or_value_and_grad", ctypes.c_int, [_vp, _dp, _dp, _u64, ctypes.c_int32, _dp, _dp]),
     ("or_value_and_grad_stored", ctypes.c_int, [_vp, _dp, _dp, _u64, ctypes.c_int32, _dp, _dp]),
     ("or_list_radius", ctypes.c_double, [_vp]),
+    ("or_loss_terms", ctypes.c_int, [_vp, _dp]),
     ("or_set_mlp", ctypes.c_int, [_vp, _P(ctypes.c_float), ctypes.c_int64]),
     ("or_set_mlp_f64", ctypes.c_int, [_vp, _dp, ctypes.c_int64]),
     ("or_mlp_size", ctypes.c_int, []),
@@ -217,6 +218,11 @@ class Oracle:
 
     def list_radius(self):
         return self.L.or_list_radius(self.h)
+
+    def loss_terms(self):
+        t = np.zeros(4)
+        self._chk(self.L.or_loss_terms(self.h, _d(t)))
+        return {"l_rdf": float(t[0]), "l_ev": float(t[1]), "l_msd": float(t[2]), "D": float(t[3])}
 
     # -- radial-basis MLP on O-O pairs --------------------------------------------------------
     def set_mlp(self, w, exact=False):

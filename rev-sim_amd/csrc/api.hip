@@ -50,11 +50,17 @@ struct rs_ctx {
   std::vector<int> snap_steps;
   std::vector<double*> snap_x, snap_v;
   int frames_cap = 0, msd_del_cap = 0;
-  double msd_loss = 0.0;
+  double msd_loss = 0.0, msd_D = 0.0;  // MSD term and D (nm^2/ps) of the last forward trajectory
   int ext_steps = 0;
   int since_build = 0;
   double hext_extra = 0.0;  // added to the O-H extent after an extent violation
   int cell_cap = 0;         // allocated length of the per-cell arrays (>= ncell + 1)
+  int cell_refine = 0;      // cells refined below r_list/2 after a home cell overflowed a build block
+  // list-driven reruns (capacity, stale list, O-H extent, cell overflow) and list builds of the
+  // last trajectory or evaluation call, and reruns since the context was created
+  int64_t reruns_call = 0, builds_call = 0, reruns_total = 0;
+  // loss terms of the last trajectory call: RDF, enthalpy of vaporization, MSD, and D (nm^2/ps)
+  double terms[4] = {0.0, 0.0, 0.0, 0.0};
   int64_t baro_attempted = 0, baro_accepted = 0;  // barostat moves of the last forward trajectory
   bool mlp_set = false;     // weights uploaded and tabulated
   std::string err;
@@ -144,7 +150,7 @@ void set_params(rs_ctx* c) {
   // else all pairs
   p.brute = 0;
   p.ncell = 1;
-  for (p.cs = 2; p.cs >= 1; --p.cs) {
+  for (p.cs = std::min(2 + c->cell_refine, 3); p.cs >= 1; --p.cs) {
     bool ok = true;
     for (int k = 0; k < 3; ++k) {
       p.nc[k] = (int)std::floor(g.box[k] * p.cs / c->rlist);
@@ -437,10 +443,21 @@ void collect_timing(rs_ctx* c) {
 }
 
 void rebuild(rs_ctx* c, bool with_u, bool filed) {
+  // the per-cell arrays must cover the cell grid; every change of the grid reallocates them
+  // (ensure_cells), and cells_fit() refuses a call that would launch past them
+  if (c->L.p.ncell + 1 > c->cell_cap) return;
+  ++c->builds_call;
   tbegin(c, RS_PHASE_LIST);
   launch_list_build(c->L, with_u, filed);
   tend(c);
   c->since_build = 0;
+}
+
+// Host-side check before any launch that files molecules into cells or builds a list.
+int cells_fit(rs_ctx* c) {
+  if (c->L.p.ncell + 1 > c->cell_cap)
+    return fail(c, RS_E_STATE, "internal: the per-cell arrays are shorter than the cell grid");
+  return 0;
 }
 
 // Start of a call: clear the status flags, and the cell counts (an earlier call that ended
@@ -460,22 +477,39 @@ int finish(rs_ctx* c, int* retry) {
   HIPCHK(c, hipMemcpy(f, c->L.b.flags, sizeof(f), hipMemcpyDeviceToHost));
   if (f[FLAG_NONFINITE]) return fail(c, RS_E_NONFINITE, "non-finite positions, velocities or forces");
   if (f[FLAG_MLP_RANGE]) return fail(c, RS_E_NONFINITE, "an O-O pair came closer than the MLP table's range");
-  *retry = (f[FLAG_LIST_OVERFLOW] ? 1 : 0) | (f[FLAG_STALE] ? 2 : 0) | (f[FLAG_HEXTENT] ? 4 : 0);
+  *retry = (f[FLAG_LIST_OVERFLOW] ? 1 : 0) | (f[FLAG_STALE] ? 2 : 0) | (f[FLAG_HEXTENT] ? 4 : 0) |
+           (f[FLAG_CELL_OVERFLOW] ? 8 : 0);
   return 0;
 }
 
 // After an overflow the per-lane capacity doubles; after a stale list the skin grows 1.5x (and the
-// O-H extent grows by 0.01 nm after an extent violation). The list radius and cell grid follow;
-// the set of interacting pairs does not depend on them.
+// O-H extent grows by 0.01 nm after an extent violation); after a home cell held more molecules
+// than a build block stages, the cells shrink (r_list/3, a 7^3 stencil), and where they cannot
+// the call fails at once. The list radius and cell grid follow; the set of interacting pairs does
+// not depend on them. Every rerun is counted (rs_stats).
 int grow_or_widen(rs_ctx* c, int why, int attempt) {
+  ++c->reruns_call;
+  ++c->reruns_total;
   if (attempt >= 6)
     return fail(c, (why & 1) ? RS_E_OVERFLOW : RS_E_STALE,
                 (why & 1) ? "neighbour list capacity exceeded after retries"
                           : "neighbour list still stale after widening the skin");
   const int capc = c->L.p.capc;
+  if (why & 8) {
+    const int cs = c->L.p.cs;
+    if (c->L.p.brute || cs >= 3)
+      return fail(c, RS_E_OVERFLOW, "a cell of the list grid holds more than 256 molecules");
+    c->cell_refine = cs + 1 - 2;
+    set_params(c);
+    if (c->L.p.cs <= cs)
+      return fail(c, RS_E_OVERFLOW,
+                  "a cell of the list grid holds more than 256 molecules and the box is too small for finer cells");
+  }
   if (why & 2) c->cfg.skin = 1.5 * c->cfg.skin + 0.005;
   if (why & 4) c->hext_extra += 0.01;
   if (why & 6) set_params(c);  // a list radius beyond a third of the box falls back to all pairs
+  int r = ensure_cells(c);
+  if (r) return r;
   c->L.p.capc = std::max(capc, default_capc(c));
   if (why & 1) c->L.p.capc = 2 * capc;
   return alloc_list(c);
@@ -495,6 +529,7 @@ int forward_impl(rs_ctx* c, const double* x0, const double* v0, uint64_t seed, i
   if (c->cfg.use_mlp && !c->mlp_set) return fail(c, RS_E_STATE, "use_mlp is set but rs_set_mlp was not called");
   if (c->ext_steps && c->ext_steps < n_steps)
     return fail(c, RS_E_CONFIG, "external noise holds fewer steps than requested");
+  if ((r = cells_fit(c))) return r;
   DevBufs& b = c->L.b;
   const rs_config& g = c->cfg;
   const double dt = g.dt, e = std::exp(-g.gamma * dt), s = std::sqrt(1.0 - e * e), kT = kT_of(c);
@@ -585,9 +620,14 @@ int forward_impl(rs_ctx* c, const double* x0, const double* v0, uint64_t seed, i
     }
     break;
   }
-  double sc[4];
+  double sc[16];
   HIPCHK(c, hipMemcpy(sc, b.scal, sizeof(sc), hipMemcpyDeviceToHost));
   c->msd_loss = P.msd_on ? sc[2] : 0.0;
+  c->msd_D = P.msd_on ? sc[3] : 0.0;
+  c->terms[0] = sc[8];
+  c->terms[1] = sc[9];
+  c->terms[2] = c->msd_loss;
+  c->terms[3] = c->msd_D;
   if (x_out) HIPCHK(c, hipMemcpy(x_out, b.x, sizeof(double) * c->N3, hipMemcpyDeviceToHost));
   if (v_out) HIPCHK(c, hipMemcpy(v_out, b.v, sizeof(double) * c->N3, hipMemcpyDeviceToHost));
   if (loss_out) *loss_out = sc[0];
@@ -637,6 +677,7 @@ int reverse_impl(rs_ctx* c, const double* xN, const double* vN, uint64_t seed, i
   Plan P;
   int r = make_plan(c, n_steps, &P);
   if (r) return r;
+  if ((r = cells_fit(c))) return r;
   DevBufs& b = c->L.b;
   const rs_config& g = c->cfg;
   const double dt = g.dt, e = std::exp(-g.gamma * dt), s = std::sqrt(1.0 - e * e), kT = kT_of(c);
@@ -709,10 +750,16 @@ int reverse_impl(rs_ctx* c, const double* xN, const double* vN, uint64_t seed, i
     }
     break;
   }
-  double sc[4], th[8];
+  double sc[16], th[8];
   HIPCHK(c, hipMemcpy(sc, b.scal, sizeof(sc), hipMemcpyDeviceToHost));
   HIPCHK(c, hipMemcpy(th, b.dth, sizeof(th), hipMemcpyDeviceToHost));
   if (loss_out) *loss_out = sc[0] + c->msd_loss;
+  // the reverse pass's own block losses (water/water.jl:1533-1546); the MSD term and D come from
+  // the forward pass's frames
+  c->terms[0] = sc[8];
+  c->terms[1] = sc[9];
+  c->terms[2] = c->msd_loss;
+  c->terms[3] = c->msd_D;
   if (dth_out) std::memcpy(dth_out, th, sizeof(double) * 6);
   if (x0_out) HIPCHK(c, hipMemcpy(x0_out, b.x, sizeof(double) * c->N3, hipMemcpyDeviceToHost));
   if (v0_out) HIPCHK(c, hipMemcpy(v0_out, b.v, sizeof(double) * c->N3, hipMemcpyDeviceToHost));
@@ -723,7 +770,8 @@ int reverse_impl(rs_ctx* c, const double* xN, const double* vN, uint64_t seed, i
 template <class Body>
 int single_eval(rs_ctx* c, const double* x, bool with_u, Body body) {
   DevBufs& b = c->L.b;
-  int r;
+  int r = cells_fit(c);
+  if (r) return r;
   for (int attempt = 0;; ++attempt) {
     HIPCHK(c, hipMemcpyAsync(b.x, x, sizeof(double) * c->N3, hipMemcpyHostToDevice, c->s));
     if ((r = reset_flags(c))) return r;
@@ -734,8 +782,9 @@ int single_eval(rs_ctx* c, const double* x, bool with_u, Body body) {
     int f[FLAG_COUNT];
     HIPCHK(c, hipMemcpy(f, b.flags, sizeof(f), hipMemcpyDeviceToHost));
     if (f[FLAG_NONFINITE]) return fail(c, RS_E_NONFINITE, "non-finite forces");
-    // a single evaluation builds its list at x: only capacity and the O-H extent can fail
-    const int why = (f[FLAG_LIST_OVERFLOW] ? 1 : 0) | (f[FLAG_HEXTENT] ? 4 : 0);
+    // a single evaluation builds its list at x: only capacity, the O-H extent and a cell's
+    // occupancy can fail
+    const int why = (f[FLAG_LIST_OVERFLOW] ? 1 : 0) | (f[FLAG_HEXTENT] ? 4 : 0) | (f[FLAG_CELL_OVERFLOW] ? 8 : 0);
     if (why) {
       if ((r = grow_or_widen(c, why, attempt))) return r;
       continue;
@@ -753,6 +802,15 @@ __global__ void k_copy16(const uint4* __restrict__ a, uint4* __restrict__ b, siz
 int check_ctx(rs_ctx* c) {
   if (!c) return fail(nullptr, RS_E_CONFIG, "null context");
   HIPCHK(c, hipSetDevice(c->device));
+  return 0;
+}
+
+// check_ctx for the calls that run device work: also clears the per-call counters of rs_stats.
+int begin_call(rs_ctx* c) {
+  int r = check_ctx(c);
+  if (r) return r;
+  c->reruns_call = 0;
+  c->builds_call = 0;
   return 0;
 }
 
@@ -895,11 +953,22 @@ int rs_set_theta(rs_ctx* c, const double* th) {
     return fail(c, RS_E_CONFIG, "sigma_O and r0 must be > 0 and q_H != 0");
   std::memcpy(c->theta, th, sizeof(double) * 6);
   const int capc = c->L.p.capc;
-  set_params(c);
+  set_params(c);  // r0 sets the O-H extent, hence the list radius and the cell grid
   c->L.p.capc = capc;
+  c->rec_valid = false;
+  if ((r = ensure_cells(c))) return r;
   if (c->cfg.rdf_rmax + c->cfg.rdf_window > c->rlist)
     return fail(c, RS_E_CONFIG, "rdf_rmax + rdf_window must not exceed the list radius");
   c->rec_valid = false;
+  return 0;
+}
+
+int rs_set_reverse_options(rs_ctx* c, int32_t n_trunc, double clip) {
+  int r = check_ctx(c);
+  if (r) return r;
+  if (n_trunc < 0 || std::isnan(clip)) return fail(c, RS_E_CONFIG, "n_steps_trunc must be >= 0 and clip_norm not NaN");
+  c->cfg.n_steps_trunc = n_trunc;
+  c->cfg.clip_norm = clip;
   return 0;
 }
 
@@ -929,7 +998,7 @@ int rs_set_external_noise(rs_ctx* c, const double* xi, int32_t n_steps) {
 
 int rs_forward(rs_ctx* c, const double* x0, const double* v0, uint64_t seed, int32_t n_steps,
                double* x_out, double* v_out, double* loss_out) {
-  int r = check_ctx(c);
+  int r = begin_call(c);
   if (r) return r;
   if (!x0 || !v0) return fail(c, RS_E_CONFIG, "null x0/v0");
   return forward_impl(c, x0, v0, seed, n_steps, x_out, v_out, loss_out);
@@ -937,7 +1006,7 @@ int rs_forward(rs_ctx* c, const double* x0, const double* v0, uint64_t seed, int
 
 int rs_reverse(rs_ctx* c, const double* xN, const double* vN, uint64_t seed, int32_t n_steps,
                double* loss_out, double* dth, double* x0_out, double* v0_out) {
-  int r = check_ctx(c);
+  int r = begin_call(c);
   if (r) return r;
   if (!xN || !vN) return fail(c, RS_E_CONFIG, "null xN/vN");
   return reverse_impl(c, xN, vN, seed, n_steps, loss_out, dth, x0_out, v0_out);
@@ -945,7 +1014,7 @@ int rs_reverse(rs_ctx* c, const double* xN, const double* vN, uint64_t seed, int
 
 int rs_value_and_grad(rs_ctx* c, const double* x0, const double* v0, uint64_t seed,
                       int32_t n_steps, double* loss_out, double* dth, float* dL_dmlp) {
-  int r = check_ctx(c);
+  int r = begin_call(c);
   if (r) return r;
   if (!x0 || !v0) return fail(c, RS_E_CONFIG, "null x0/v0");
   std::vector<double> xN(c->N3), vN(c->N3);
@@ -962,7 +1031,7 @@ int rs_value_and_grad(rs_ctx* c, const double* x0, const double* v0, uint64_t se
 
 int rs_reweight(rs_ctx* c, const double* x0, const double* v0, uint64_t seed, int32_t n_steps,
                 double* loss_out, double* dth_out, double* x_out, double* v_out) {
-  int r = check_ctx(c);
+  int r = begin_call(c);
   if (r) return r;
   if (!x0 || !v0) return fail(c, RS_E_CONFIG, "null argument");
   if ((r = forward_impl(c, x0, v0, seed, n_steps, x_out, v_out, nullptr, true))) return r;
@@ -981,7 +1050,7 @@ int rs_reweight(rs_ctx* c, const double* x0, const double* v0, uint64_t seed, in
 }
 
 int rs_energy(rs_ctx* c, const double* x, double* U, double* dU, double* F) {
-  int r = check_ctx(c);
+  int r = begin_call(c);
   if (r) return r;
   if (!x) return fail(c, RS_E_CONFIG, "null argument");
   if (c->cfg.use_mlp) return fail(c, RS_E_CONFIG, "rs_energy covers the classical force field only (use_mlp = 0)");
@@ -1065,7 +1134,7 @@ int rs_mlp_table(rs_ctx* c, double* f, double* df, int32_t capacity, int32_t* n_
 }
 
 int rs_forces(rs_ctx* c, const double* x, double* F) {
-  int r = check_ctx(c);
+  int r = begin_call(c);
   if (r) return r;
   if (!x || !F) return fail(c, RS_E_CONFIG, "null argument");
   if (c->cfg.use_mlp && !c->mlp_set) return fail(c, RS_E_STATE, "use_mlp is set but rs_set_mlp was not called");
@@ -1080,7 +1149,7 @@ int rs_forces(rs_ctx* c, const double* x, double* F) {
 
 int rs_adjoint_terms(rs_ctx* c, const double* x, const double* cc, const double* w, double* F,
                      double* h, double* vjp6) {
-  int r = check_ctx(c);
+  int r = begin_call(c);
   if (r) return r;
   if (!x || !cc || !w) return fail(c, RS_E_CONFIG, "null argument");
   DevBufs& b = c->L.b;
@@ -1125,7 +1194,7 @@ int rs_noise(rs_ctx* c, uint64_t seed, int32_t step, double* xi) {
 }
 
 int rs_rdf(rs_ctx* c, const double* x, double* g, int64_t* hist, double* loss, double* dLdx) {
-  int r = check_ctx(c);
+  int r = begin_call(c);
   if (r) return r;
   if (!x) return fail(c, RS_E_CONFIG, "null x");
   const rs_config& cf = c->cfg;
@@ -1165,7 +1234,7 @@ int rs_msd(rs_ctx* c, const double* frames, int32_t nf, double* D, double* loss,
 }
 
 int rs_molecule_pairs(rs_ctx* c, const double* x, int32_t* pairs, int64_t capacity, int64_t* n_pairs) {
-  int r = check_ctx(c);
+  int r = begin_call(c);
   if (r) return r;
   if (!x || !n_pairs || (capacity > 0 && !pairs)) return fail(c, RS_E_CONFIG, "null argument");
   if ((r = single_eval(c, x, false, [] {}))) return r;
@@ -1200,6 +1269,23 @@ int rs_molecule_pairs(rs_ctx* c, const double* x, int32_t* pairs, int64_t capaci
 }
 
 double rs_list_radius(const rs_ctx* c) { return c ? c->rlist : 0.0; }
+
+int rs_loss_terms(const rs_ctx* c, double* t) {
+  if (!c || !t) return fail(nullptr, RS_E_CONFIG, "null argument");
+  t[0] = c->terms[0];
+  t[1] = c->terms[1];
+  t[2] = c->terms[2];
+  t[3] = c->terms[3] * 1e-6;  // nm^2/ps -> m^2/s
+  return 0;
+}
+
+int rs_stats(const rs_ctx* c, int64_t* st, int32_t n) {
+  if (!c || (!st && n > 0) || n < 0 || n > RS_STAT_COUNT)
+    return fail(nullptr, RS_E_CONFIG, "rs_stats: null context or n outside [0, RS_STAT_COUNT]");
+  const int64_t v[RS_STAT_COUNT] = {c->reruns_call, c->builds_call, c->reruns_total};
+  for (int k = 0; k < n; ++k) st[k] = v[k];
+  return 0;
+}
 
 int rs_get_box(const rs_ctx* c, double* box) {
   if (!c || !box) return fail(nullptr, RS_E_CONFIG, "null argument");

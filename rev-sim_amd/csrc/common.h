@@ -16,6 +16,7 @@ enum {
   FLAG_NONFINITE = 2,      // NaN/Inf in positions, velocities or forces
   FLAG_HEXTENT = 3,        // an O-H distance exceeded the extent assumed by the list radius
   FLAG_MLP_RANGE = 4,      // an O-O pair closer than the MLP table's lower end
+  FLAG_CELL_OVERFLOW = 5,  // a home cell of the list grid held more molecules than a build block stages
   FLAG_COUNT = 8
 };
 

@@ -141,8 +141,9 @@ __global__ __launch_bounds__(RS_BLOCK) void k_energy_reduce(const double* __rest
 
 // Enthalpy of vaporization of the block (water/water.jl:738-754) from e = (U, dU/dθ):
 // dH = u_gas - U/M + RT, l = w (dH - target)^2, c' = dl/d(-U) = 2 w (dH - target)/M. scal[6] = l,
-// scal[7] = c'. With wgt != 0 (a block of the averaged trajectory loss): scal[0] += wgt l and,
-// if direct, dth[k] -= wgt c' dU/dθ_k (the loss's own θ-dependence through U).
+// scal[7] = c'. With wgt != 0 (a block of the averaged trajectory loss): scal[0] and scal[9] (the
+// trajectory's enthalpy term) += wgt l and, if direct, dth[k] -= wgt c' dU/dθ_k (the loss's own
+// θ-dependence through U).
 __global__ void k_dhvap(int M, const double* e, double w, double target, double ugas, double RT,
                         double wgt, int direct, double* scal, double* dth) {
   if (threadIdx.x != 0) return;
@@ -153,6 +154,7 @@ __global__ void k_dhvap(int M, const double* e, double w, double target, double 
   scal[7] = cp;
   if (wgt != 0.0) {
     scal[0] += wgt * l;
+    scal[9] += wgt * l;
     if (direct)
       for (int k = 0; k < 6; ++k) dth[k] -= wgt * cp * e[1 + k];
   }

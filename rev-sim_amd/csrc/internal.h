@@ -31,7 +31,9 @@ struct DevBufs {
   double* slots;   // per-block θ-VJP partials [nblocks * 8]
   double* vjb;     // bonded θ-VJP per molecule [3M]: (k_bond, r0, k_angle)
   double* dth;     // dL/dθ accumulator [8]
-  double* scal;    // device scalars: [0] loss, [1] rdf loss (last), [2] msd loss, [3] D, [4] clip scale
+  double* scal;    // device scalars: [0] loss, [1] rdf loss (last), [2] msd loss, [3] D, [4] clip scale,
+                   // [5] msd gradient factor, [6] enthalpy loss (last), [7] its c', [8] trajectory RDF
+                   // term, [9] trajectory enthalpy term
   double* wmax;    // MLP: max|w_O| per step parity, spread over 64 slots (uint64 bits of doubles)
   long long* hist; // RDF bins (fixed point 2^32)
   double *rdfG, *target, *rdf_g;

@@ -281,7 +281,7 @@ __global__ __launch_bounds__(RS_BLOCK) void k_list_cells(SysParams p, const int*
   const int h0 = start[h], nh = start[h + 1] - h0;
   if (nh == 0) return;  // uniform for the block
   if (nh > LC_MAXH) {
-    if (threadIdx.x == 0) atomicOr(&flags[FLAG_LIST_OVERFLOW], 2);
+    if (threadIdx.x == 0) atomicOr(&flags[FLAG_CELL_OVERFLOW], 1);
     return;
   }
   if (threadIdx.x < RS_WAVE) {

@@ -107,7 +107,11 @@ __global__ void k_rdf_final(RdfArgs r, const long long* hist, const double* targ
       L += d * d;
     }
     scal[1] = L;
-    if (loss_dst) *loss_dst += r.wgt * L;
+    if (loss_dst) {
+      // loss_dst is the scalar block: [0] the trajectory loss, [8] its RDF term
+      loss_dst[0] += r.wgt * L;
+      loss_dst[8] += r.wgt * L;
+    }
   }
 }
 

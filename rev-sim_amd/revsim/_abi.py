@@ -157,6 +157,7 @@ SIGNATURES = [
     ("rs_config_size", ctypes.c_int32, []),
     ("rs_set_theta", ctypes.c_int, [_vp, _dp]),
     ("rs_set_target_rdf", ctypes.c_int, [_vp, _dp, ctypes.c_int32]),
+    ("rs_set_reverse_options", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_double]),
     ("rs_set_external_noise", ctypes.c_int, [_vp, _dp, ctypes.c_int32]),
     ("rs_forward", ctypes.c_int, [_vp, _dp, _dp, ctypes.c_uint64, ctypes.c_int32, _dp, _dp, _dp]),
     ("rs_reverse", ctypes.c_int,
@@ -176,6 +177,8 @@ SIGNATURES = [
     ("rs_msd", ctypes.c_int, [_vp, _dp, ctypes.c_int32, _dp, _dp, _dp]),
     ("rs_molecule_pairs", ctypes.c_int, [_vp, _dp, _ip, ctypes.c_int64, _lp]),
     ("rs_list_radius", ctypes.c_double, [_vp]),
+    ("rs_loss_terms", ctypes.c_int, [_vp, _dp]),
+    ("rs_stats", ctypes.c_int, [_vp, _lp, ctypes.c_int32]),
     ("rs_get_box", ctypes.c_int, [_vp, _dp]),
     ("rs_set_box", ctypes.c_int, [_vp, _dp]),
     ("rs_timings", ctypes.c_int, [_vp, _dp, ctypes.c_int32]),

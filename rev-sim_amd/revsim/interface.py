@@ -38,6 +38,17 @@ BAROSTAT_PRESSURE = 1.0  # bar (water/water.jl:1136)
 
 
 @dataclasses.dataclass
+class ForwardRecords:
+    """What forward_sim leaves for reverse_sim (the reference's fwd_loggers): the context holding
+    the trajectory's snapshots and MSD frames on the device, and the trajectory's identity."""
+    sim: Simulator
+    seed: int
+    n_steps: int
+    theta: np.ndarray
+    loss_type: str
+
+
+@dataclasses.dataclass
 class System:
     coords: np.ndarray
     velocities: np.ndarray
@@ -153,7 +164,9 @@ def forward_sim(coords, velocities, boundary, noise_seeds, gamma, n_steps, n_ste
                              np.asarray(velocities, dtype=np.float64).reshape(-1), _seed(noise_seeds), n_steps)
     box = sim.get_box()
     context.box = box.copy()
-    sys = System(x.reshape(-1, 3), v.reshape(-1, 3), box, loggers=(sim, _seed(noise_seeds), n_steps))
+    theta = params_to_theta(params)
+    sys = System(x.reshape(-1, 3), v.reshape(-1, 3), box,
+                 loggers=ForwardRecords(sim, _seed(noise_seeds), n_steps, theta, str(loss_type).lstrip(":")))
     return sys, loss, None
 
 
@@ -162,7 +175,7 @@ def reweighting_sim(coords, velocities, boundary, noise_seeds, gamma, n_steps, n
                     context: Context, target_rdf=None):
     """water/water.jl:1107-1210: one forward trajectory with the ensemble-reweighting gradient
     estimator over its loss blocks; returns (System, loss, dl_dp, l_rdf, l_ev) with dl_dp in the
-    reference's 7-parameter layout (l_rdf, l_ev are not separated: None)."""
+    reference's 7-parameter layout and l_rdf, l_ev the block-averaged terms (water/water.jl:1204-1209)."""
     _check_types(inter_type, loss_type)
     if str(loss_type).lstrip(":") in ("msd", "rdf_msd"):
         raise ValueError("loss_type not recognised")  # reweighting_sim has the water losses only
@@ -179,31 +192,50 @@ def reweighting_sim(coords, velocities, boundary, noise_seeds, gamma, n_steps, n
                                  np.asarray(velocities, dtype=np.float64).reshape(-1), _seed(noise_seeds), n_steps)
     box = sim.get_box()
     context.box = box.copy()
-    return System(x.reshape(-1, 3), v.reshape(-1, 3), box), loss, theta_grad_to_params(g), None, None
+    t = sim.loss_terms()
+    return System(x.reshape(-1, 3), v.reshape(-1, 3), box), loss, theta_grad_to_params(g), t["l_rdf"], t["l_ev"]
 
 
 def reverse_sim(coords, velocities, boundary, noise_seeds, gamma, n_steps, n_steps_loss, n_steps_trunc, params,
                 dt, n_threads, fwd_loggers, clip_norm_threshold, inter_type="lj", loss_type="rdf"):
-    """water/water.jl:1435-1558: returns (System, loss, dl_dp, l_rdf, l_msd) with dl_dp in the
-    reference's 7-parameter layout."""
+    """water/water.jl:1435-1558: returns (System, loss, dl_dp, l_rdf, l_ev) with dl_dp in the
+    reference's 7-parameter layout; for the MSD loss types the diffusion variant's tuple
+    (System, loss, dl_dp, D, l_diff, l_rdf, l_ev), diffusion/diffusion.jl:1616, D in m^2/s.
+    `fwd_loggers` is the forward System's `loggers` (ForwardRecords). The truncation and the
+    clip threshold are reverse_sim's own arguments: they are passed to the context here and do not
+    disturb the forward records."""
     _check_types(inter_type, loss_type)
     n_loss = n_steps_loss or n_steps
     if n_steps_trunc > n_loss:
         raise ValueError("n_steps_trunc must not be greater than n_steps_loss")  # water/water.jl:1437
-    if fwd_loggers is None:
+    if not isinstance(fwd_loggers, ForwardRecords):
         raise ValueError("fwd_loggers from forward_sim are required")
-    sim, seed, fwd_steps = fwd_loggers
-    if fwd_steps != n_steps or seed != _seed(noise_seeds):
+    rec = fwd_loggers
+    if rec.n_steps != n_steps or rec.seed != _seed(noise_seeds):
         raise ValueError("fwd_loggers belong to a different trajectory")
-    sim.set_theta(params_to_theta(params))
-    # the forward call fixed the loss configuration; truncation and clipping apply to the reverse
-    if (sim.cfg.n_steps_trunc != (0 if n_steps_trunc == n_loss else n_steps_trunc)
-            or sim.cfg.clip_norm != float(clip_norm_threshold)):
-        raise ValueError("n_steps_trunc and clip_norm_threshold must match the context's configuration")
+    if not np.array_equal(rec.theta, params_to_theta(params)):
+        raise ValueError("params differ from those of the forward trajectory")
+    lt = str(loss_type).lstrip(":")
+    if lt != rec.loss_type:
+        raise ValueError("loss_type differs from that of the forward trajectory")
+    sim = rec.sim
+    trunc = 0 if n_steps_trunc in (None, n_loss) else int(n_steps_trunc)
+    sim.set_reverse_options(trunc, float(clip_norm_threshold))
     loss, g, x0, v0 = sim.reverse(np.asarray(coords, dtype=np.float64).reshape(-1),
-                                  np.asarray(velocities, dtype=np.float64).reshape(-1), seed, n_steps)
-    sys = System(x0.reshape(-1, 3), v0.reshape(-1, 3), _box(boundary, sim.cfg.box[:]))
-    return sys, loss, theta_grad_to_params(g), None, None
+                                  np.asarray(velocities, dtype=np.float64).reshape(-1), rec.seed, n_steps)
+    t = sim.loss_terms()
+    sys = System(x0.reshape(-1, 3), v0.reshape(-1, 3), _box(boundary, sim.get_box()))
+    if lt in ("msd", "rdf_msd"):
+        return sys, loss, theta_grad_to_params(g), t["D"], t["l_msd"], t["l_rdf"], t["l_ev"]
+    return sys, loss, theta_grad_to_params(g), t["l_rdf"], t["l_ev"]
+
+
+def train_log_line(epoch, loss, terms, params, grads, msd=False) -> str:
+    """One line of train_<run_n>.log: "epoch loss l_rdf l_ev params... grads..." as written by
+    water/water.jl:1673-1680, or with the MSD loss "epoch loss D l_diff l_rdf l_ev ..." as written
+    by diffusion/diffusion.jl:1727-1733."""
+    head = [epoch, loss] + ([terms["D"], terms["l_msd"]] if msd else []) + [terms["l_rdf"], terms["l_ev"]]
+    return " ".join(map(str, head)) + " " + " ".join(map(str, params)) + " " + " ".join(map(str, grads))
 
 
 class Adam:
@@ -297,7 +329,8 @@ def train(sim: Simulator, x0, v0, params_start, n_epochs: int, n_steps: int, n_s
             rec.append(eval_fn(params, W if joint else None))
         history.append(tuple(rec))
         if log is not None:
-            log(f"{epoch} {loss} " + " ".join(map(str, params)) + " " + " ".join(map(str, grads)))
+            log(train_log_line(epoch, loss, sim.loss_terms(), params, grads,
+                               msd=bool(sim.cfg.w_msd and sim.cfg.msd_window_steps)))
         gp = grads * p0 if scale_params else grads
         gall = np.clip(np.concatenate([gp, gW]) if joint else gp, -grad_clamp_val, grad_clamp_val)
         pall = np.concatenate([scaled if scale_params else params, W]) if joint else (
@@ -313,5 +346,6 @@ def train(sim: Simulator, x0, v0, params_start, n_epochs: int, n_steps: int, n_s
     return (params, W, history) if joint else (params, history)
 
 
-__all__ = ["System", "Context", "forward_sim", "reverse_sim", "reweighting_sim", "train", "Adam", "params_to_theta",
+__all__ = ["System", "ForwardRecords", "Context", "forward_sim", "reverse_sim", "reweighting_sim", "train",
+           "train_log_line", "Adam", "params_to_theta",
            "theta_grad_to_params", "DEFAULT_THETA", "THETA0"]

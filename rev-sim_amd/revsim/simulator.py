@@ -63,6 +63,12 @@ class Simulator:
     def set_theta(self, theta):
         self._chk(self.lib.rs_set_theta(self.h, dptr(self._vec(theta, 6))))
 
+    def set_reverse_options(self, n_steps_trunc: int, clip_norm: float):
+        """reverse_sim's own arguments (truncation, HVP clip threshold); the forward records stay."""
+        self._chk(self.lib.rs_set_reverse_options(self.h, int(n_steps_trunc), float(clip_norm)))
+        self.cfg.n_steps_trunc = int(n_steps_trunc)
+        self.cfg.clip_norm = float(clip_norm)
+
     def set_target_rdf(self, g):
         g = self._vec(g, self.cfg.rdf_bins)
         self._chk(self.lib.rs_set_target_rdf(self.h, dptr(g), g.size))
@@ -187,6 +193,20 @@ class Simulator:
 
     def list_radius(self) -> float:
         return float(self.lib.rs_list_radius(self.h))
+
+    def loss_terms(self) -> dict:
+        """Loss terms of the last trajectory call: l_rdf, l_ev (enthalpy of vaporization), l_msd
+        (averaged over loss blocks, weights applied; they sum to the loss) and D (m^2/s)."""
+        t = np.zeros(4)
+        self._chk(self.lib.rs_loss_terms(self.h, dptr(t)))
+        return {"l_rdf": float(t[0]), "l_ev": float(t[1]), "l_msd": float(t[2]), "D": float(t[3])}
+
+    def stats(self) -> dict:
+        """List-driven reruns and list builds of the last call that ran device work, and reruns
+        since the context was created."""
+        st = np.zeros(3, dtype=np.int64)
+        self._chk(self.lib.rs_stats(self.h, lptr(st), st.size))
+        return {"reruns_call": int(st[0]), "list_builds_call": int(st[1]), "reruns_total": int(st[2])}
 
     def copy_calibration(self, nbytes: int, reps: int = 5):
         """Launch a plain copy kernel of known traffic (nbytes read + nbytes written per launch)."""

@@ -348,20 +348,67 @@ def test_gradient_matches_trajectory_finite_differences(box216):
         assert abs(g[k] - fd) < 1e-3 * abs(fd), (k, g[k], fd)
 
 
-def test_truncated_blocks_and_snapshots_run(box216):
-    """Reference training regime (loss blocks, truncation, snapshot restore): the reversible
-    estimator equals the stored-trajectory adjoint restricted the same way on each block."""
+def _single_block(o1, x0, v0, seed, t0, n):
+    """Loss and gradient of an n-step single-block trajectory started from the forward state at
+    step t0 of the seed's trajectory, with that trajectory's noise of steps t0+1..t0+n replayed."""
+    xs, vs, _ = o1.forward(x0, v0, seed, t0) if t0 > 0 else (x0, v0, None)
+    xi = np.concatenate([o1.noise(seed, t0 + k) for k in range(1, n + 1)])
+    o1.set_external_noise(xi, n)
+    L, g = o1.value_and_grad(xs, vs, seed, n)
+    o1.set_external_noise(None, 0)
+    return L, g
+
+
+@pytest.mark.parametrize("n_log", [20, 0])
+def test_truncated_blocks_equal_per_block_gradients(box216, n_log):
+    """Reference training regime (water/water.jl:1491-1547): loss blocks of 20 steps, each block's
+    reverse pass truncated after 10 steps, forward snapshots restored every n_log steps. Block b's
+    gradient flows only through the last 10 steps before its end t_b, from the state at t_b - 10
+    (restored from the forward pass), so the result must equal the mean over blocks of the
+    single-block 10-step gradients started from the forward state at t_b - 10 with the same noise
+    replayed; the loss is the mean of the block-end RDF losses. Snapshots every 20 steps and none
+    (block boundaries are restored in either case) agree."""
     import revsim
 
-    o = _orc(n_steps_loss=20, n_steps_trunc=10, n_steps_log=20)
-    g0, _, _, _ = o.rdf(box216.coords)
-    o.set_target_rdf(revsim.perturbed_target(g0))
-    L, g = o.value_and_grad(box216.coords, box216.velocities, 13, 60)
-    assert np.all(np.isfinite(g)) and np.isfinite(L)
-    # each block's gradient flows only through its last n_steps_trunc steps: a single-block run
-    # of 10 steps started from the state at step 50 reproduces the last block's contribution
-    xs, vs, _ = o.forward(box216.coords, box216.velocities, 13, 60)
-    assert np.all(np.isfinite(xs))
+    n_loss, n_trunc, T = 20, 10, 60
+    o = _orc(n_steps_loss=n_loss, n_steps_trunc=n_trunc, n_steps_log=n_log)
+    tgt = revsim.perturbed_target(o.rdf(box216.coords)[0])
+    o.set_target_rdf(tgt)
+    L, g = o.value_and_grad(box216.coords, box216.velocities, 13, T)
+    o1 = _orc()
+    o1.set_target_rdf(tgt)
+    blocks = [_single_block(o1, box216.coords, box216.velocities, 13, tb - n_trunc, n_trunc)
+              for tb in range(n_loss, T + 1, n_loss)]
+    Lb = np.mean([b[0] for b in blocks])
+    gb = np.mean([b[1] for b in blocks], axis=0)
+    assert L == pytest.approx(Lb, rel=1e-12)
+    assert rel_err(g, gb) < 1e-10, (g, gb)
+    # a plausible misreading (truncation ignored: the full 20-step block adjoint) is far off
+    gfull = np.mean([_single_block(o1, box216.coords, box216.velocities, 13, tb - n_loss, n_loss)[1]
+                     for tb in range(n_loss, T + 1, n_loss)], axis=0)
+    assert rel_err(gfull, gb) > 1e-3
+
+
+def test_snapshot_restore_without_truncation_is_exact(box216):
+    """Snapshots only reset reconstruction drift: with no truncation, restoring the forward state
+    every 20 steps gives the same FP64 gradient as reconstructing all 60 steps (<= 1e-10), and both
+    equal the mean of the three blocks' full-length gradients."""
+    import revsim
+
+    res = {}
+    for n_log in (20, 0):
+        o = _orc(n_steps_loss=20, n_steps_log=n_log)
+        tgt = revsim.perturbed_target(o.rdf(box216.coords)[0])
+        o.set_target_rdf(tgt)
+        res[n_log] = o.value_and_grad(box216.coords, box216.velocities, 13, 60)
+    assert res[20][0] == pytest.approx(res[0][0], rel=1e-12)
+    assert rel_err(res[20][1], res[0][1]) < 1e-10
+    # the block boundary resets the adjoint: each block's loss sees only its own 20 steps
+    o1 = _orc()
+    o1.set_target_rdf(tgt)
+    gb = np.mean([_single_block(o1, box216.coords, box216.velocities, 13, tb - 20, 20)[1]
+                  for tb in (20, 40, 60)], axis=0)
+    assert rel_err(res[20][1], gb) < 1e-10
 
 
 def _free_config(n, box, rc, **kw):

@@ -195,3 +195,38 @@ def test_reweighting_estimator_from_block_states(box216):
     assert L == pytest.approx(Ls.mean(), rel=1e-12)
     assert rel_err(g, ref) < 1e-10
     assert np.array_equal(xN, o.forward(box216.coords, box216.velocities, 13, 18)[0])
+
+
+def test_loss_terms_from_block_states(box216):
+    """The per-term losses the reference's reverse_sim returns beside the total (l_rdf, l_ev,
+    water/water.jl:1549-1557; l_diff and D, diffusion/diffusion.jl:897-911,1605-1616): rebuilt
+    from the block-end states with the single-state RDF and energy, summing to the loss, and equal
+    after the forward and after the reverse pass (FP64 reconstruction)."""
+    import revsim
+
+    w_rdf, w_dh, w_msd = 0.7, 0.5, 1e-3
+    o = _orc(w_rdf=w_rdf, w_dhvap=w_dh, n_steps_loss=20, msd_window_steps=20, msd_every=5, w_msd=w_msd)
+    o.set_target_rdf(revsim.perturbed_target(o.rdf(box216.coords)[0]))
+    xN, vN, Lf = o.forward(box216.coords, box216.velocities, 13, 40)
+    tf = o.loss_terms()
+    op = _orc(w_rdf=0.0)  # intermediate states of the same trajectory (the loss does not steer it)
+    lr, le = [], []
+    for tb in (20, 40):
+        x = op.forward(box216.coords, box216.velocities, 13, tb)[0]
+        lr.append(w_rdf * o.rdf(x)[2])
+        U = o.energy(x)[0]
+        le.append(w_dh * (3.69 - U / 216 + RT300 - 46.92) ** 2)
+    frames = np.concatenate([op.forward(box216.coords, box216.velocities, 13, t)[0].reshape(-1, 9)[:, :3].reshape(-1)
+                             for t in (25, 30, 35, 40)])
+    D, lm, _ = o.msd(frames)
+    assert tf["l_rdf"] == pytest.approx(np.mean(lr), rel=1e-12)
+    assert tf["l_ev"] == pytest.approx(np.mean(le), rel=1e-12)
+    assert tf["l_msd"] == pytest.approx(lm, rel=1e-12)
+    assert tf["D"] == pytest.approx(D * 1e-6, rel=1e-12)
+    assert Lf == pytest.approx(tf["l_rdf"] + tf["l_ev"] + tf["l_msd"], rel=1e-12)
+    o.forward(box216.coords, box216.velocities, 13, 40)
+    Lr, _, _, _ = o.reverse(xN, vN, 13, 40)
+    tr = o.loss_terms()
+    for k in ("l_rdf", "l_ev", "l_msd", "D"):
+        assert tr[k] == pytest.approx(tf[k], rel=1e-9), k
+    assert Lr == pytest.approx(tr["l_rdf"] + tr["l_ev"] + tr["l_msd"], rel=1e-12)

@@ -39,6 +39,9 @@ class OracleSim:
             return L, g, self.o.mlp_grad().astype(np.float32)
         return L, g
 
+    def loss_terms(self):
+        return self.o.loss_terms()
+
 
 def _setup(use_mlp, n_steps_loss=0):
     import revsim
@@ -153,3 +156,36 @@ def test_loss_types_map_to_weights():
         assert (kw["w_rdf"], kw["w_dhvap"], kw.get("w_msd", 0.0)) == (wr, wd, wm), lt
     with pytest.raises(ValueError):
         I._check_types("lj", "energy")
+
+
+@pytest.mark.parametrize("msd", [False, True])
+def test_train_log_line_format(msd):
+    """The per-epoch log line (water/water.jl:1673-1680; diffusion/diffusion.jl:1727-1733 with the
+    MSD loss): epoch, loss, [D, l_diff,] l_rdf, l_ev, the 7 parameters, the 7 gradients; the terms
+    are the oracle's for the epoch's trajectory and sum to the loss."""
+    import revsim
+    from revsim import interface as I
+
+    wb = revsim.water_box(216, 1.862)
+    kw = dict(msd_window_steps=10, msd_every=5, w_msd=1e-3) if msd else {}
+    cfg = revsim.make_config(216, 1.862, 0.8, n_steps_loss=10, w_dhvap=0.5, **kw)
+    sim = OracleSim(cfg)
+    sim.set_theta(revsim.DEFAULT_THETA)
+    sim.set_target_rdf(revsim.perturbed_target(sim.o.rdf(wb.coords)[0]))
+    p0 = np.array([0.315, 0.636, -0.834, 462750.0, 0.09572, 836.8, 1.82421813418])
+    lines = []
+    _, hist = I.train(sim, wb.coords, wb.velocities, p0, 1, 20, log=lines.append)
+    f = [float(v) for v in lines[0].split()]
+    nh = 6 if msd else 4
+    assert len(f) == nh + 14 and f[0] == 1 and f[1] == hist[0][0]
+    rng = np.random.Generator(np.random.Philox(1))
+    _, s_run = (int(k) for k in rng.integers(0, 2 ** 63, size=2))
+    sim.set_theta(I.params_to_theta(p0))
+    L, g = sim.value_and_grad(wb.coords, wb.velocities, s_run, 20)
+    t = sim.loss_terms()
+    want = ([t["D"], t["l_msd"]] if msd else []) + [t["l_rdf"], t["l_ev"]]
+    assert f[2:nh] == pytest.approx(want, rel=1e-15)
+    assert t["l_rdf"] > 0 and t["l_ev"] > 0 and (t["l_msd"] > 0) == msd
+    assert L == pytest.approx(t["l_rdf"] + t["l_ev"] + t["l_msd"], rel=1e-12)
+    assert f[nh:nh + 7] == pytest.approx(list(p0), rel=1e-15)
+    assert f[nh + 7:] == pytest.approx(list(I.theta_grad_to_params(g)), rel=1e-15)
