@@ -119,6 +119,11 @@ typedef struct or_ctx {
 #define MLP_B2 (MLP_W2 + MLP_NH * MLP_NH)
 #define MLP_W3 (MLP_B2 + MLP_NH)
 #define MLP_B3 (MLP_W3 + MLP_NH)
+/* oracle-internal copies after the packed weights: W1 and W2 transposed, so the layer loops run
+   over output neurons innermost (vectorizable, each sum still taken in input order) */
+#define MLP_W1T MLP_NW
+#define MLP_W2T (MLP_W1T + MLP_NF * MLP_NH)
+#define MLP_NWX (MLP_W2T + MLP_NH * MLP_NH)
 
 static void set_err(or_ctx* o, const char* m) {
   if (o) snprintf(o->err, sizeof(o->err), "%s", m);
@@ -561,10 +566,18 @@ static double bf16_round(float f) {
   return (double)r;
 }
 
+static void mlp_transpose(double* W) {
+  for (int n = 0; n < MLP_NH; ++n)
+    for (int f = 0; f < MLP_NF; ++f) W[MLP_W1T + f * MLP_NH + n] = W[MLP_W1 + n * MLP_NF + f];
+  for (int m = 0; m < MLP_NH; ++m)
+    for (int n = 0; n < MLP_NH; ++n) W[MLP_W2T + n * MLP_NH + m] = W[MLP_W2 + m * MLP_NH + n];
+}
+
 int or_set_mlp(or_ctx* o, const float* w, int64_t n) {
   if (n != MLP_NW) return RS_E_CONFIG;
-  if (!o->mlp) o->mlp = (double*)malloc(sizeof(double) * MLP_NW);
+  if (!o->mlp) o->mlp = (double*)malloc(sizeof(double) * MLP_NWX);
   for (int i = 0; i < MLP_NW; ++i) o->mlp[i] = bf16_round(w[i]);
+  mlp_transpose(o->mlp);
   o->use_mlp = 1;
   return 0;
 }
@@ -574,8 +587,9 @@ int or_mlp_size(void) { return MLP_NW; }
 /* Unrounded weights (finite-difference tests of the weight gradient). */
 int or_set_mlp_f64(or_ctx* o, const double* w, int64_t n) {
   if (n != MLP_NW) return RS_E_CONFIG;
-  if (!o->mlp) o->mlp = (double*)malloc(sizeof(double) * MLP_NW);
+  if (!o->mlp) o->mlp = (double*)malloc(sizeof(double) * MLP_NWX);
   memcpy(o->mlp, w, sizeof(double) * MLP_NW);
+  mlp_transpose(o->mlp);
   o->use_mlp = 1;
   return 0;
 }
@@ -621,16 +635,20 @@ typedef struct {
 static void mlp_forward(const double* W, double r, double rc, mlp_jet* J) {
   rbf(r, rc, J->p, J->p1, J->p2);
   for (int n = 0; n < MLP_NH; ++n) {
-    double z = W[MLP_B1 + n], za = 0.0, zb = 0.0;
-    for (int f = 0; f < MLP_NF; ++f) {
-      double wv = W[MLP_W1 + n * MLP_NF + f];
-      z += wv * J->p[f];
-      za += wv * J->p1[f];
-      zb += wv * J->p2[f];
+    J->z1[n] = W[MLP_B1 + n];
+    J->z1a[n] = J->z1b[n] = 0.0;
+  }
+  for (int f = 0; f < MLP_NF; ++f) {
+    const double* wt = W + MLP_W1T + f * MLP_NH;
+    const double p0 = J->p[f], p1 = J->p1[f], p2 = J->p2[f];
+    for (int n = 0; n < MLP_NH; ++n) {
+      J->z1[n] += wt[n] * p0;
+      J->z1a[n] += wt[n] * p1;
+      J->z1b[n] += wt[n] * p2;
     }
-    J->z1[n] = z;
-    J->z1a[n] = za;
-    J->z1b[n] = zb;
+  }
+  for (int n = 0; n < MLP_NH; ++n) {
+    const double z = J->z1[n], za = J->z1a[n], zb = J->z1b[n];
     silu4(z, &J->s1[0][n], &J->s1[1][n], &J->s1[2][n], &J->s1[3][n]);
     J->a1[n] = J->s1[0][n];
     J->a1a[n] = J->s1[1][n] * za;
@@ -640,16 +658,20 @@ static void mlp_forward(const double* W, double r, double rc, mlp_jet* J) {
   J->u1 = 0.0;
   J->u2 = 0.0;
   for (int m = 0; m < MLP_NH; ++m) {
-    double z = W[MLP_B2 + m], za = 0.0, zb = 0.0;
-    for (int n = 0; n < MLP_NH; ++n) {
-      double wv = W[MLP_W2 + m * MLP_NH + n];
-      z += wv * J->a1[n];
-      za += wv * J->a1a[n];
-      zb += wv * J->a1b[n];
+    J->z2[m] = W[MLP_B2 + m];
+    J->z2a[m] = J->z2b[m] = 0.0;
+  }
+  for (int n = 0; n < MLP_NH; ++n) {
+    const double* wt = W + MLP_W2T + n * MLP_NH;
+    const double a0 = J->a1[n], aa = J->a1a[n], ab = J->a1b[n];
+    for (int m = 0; m < MLP_NH; ++m) {
+      J->z2[m] += wt[m] * a0;
+      J->z2a[m] += wt[m] * aa;
+      J->z2b[m] += wt[m] * ab;
     }
-    J->z2[m] = z;
-    J->z2a[m] = za;
-    J->z2b[m] = zb;
+  }
+  for (int m = 0; m < MLP_NH; ++m) {
+    const double z = J->z2[m], za = J->z2a[m], zb = J->z2b[m];
     silu4(z, &J->s2[0][m], &J->s2[1][m], &J->s2[2][m], &J->s2[3][m]);
     J->a2[m] = J->s2[0][m];
     J->a2a[m] = J->s2[1][m] * za;

@@ -1,5 +1,6 @@
 // mlp.hip — the radial-basis MLP correction on O-O pairs: tabulation of f = -u' and f' = -u'' by
-// forward-mode jets through the network (BF16 MFMA with FP32 accumulation in the FP32 path, FP64
+// forward-mode jets through the network (BF16 MFMA with FP32 accumulation in the FP32 path, the
+// activations split into three BF16 terms so that only FP32 rounding remains; FP64
 // in the parity path), and dL/dW from the table gradient the adjoint pair kernels accumulate.
 //
 // Network (features of one O-O distance r < r_c):
@@ -86,39 +87,59 @@ __global__ void k_mlp_tab64(const double* __restrict__ W, double rc, double rlo,
 
 // BF16 MFMA tabulation: one wave per 16 nodes. v_mfma_f32_16x16x16_bf16 operand layout: lane l
 // holds A[row l%16][k 4(l/16)+v], B[k 4(l/16)+v][col l%16] and D[row 4(l/16)+v][col l%16], v=0..3.
-// The three jet streams (value, d/dr, d²/dr²) are three GEMMs sharing the weight operand.
+// The three jet streams (value, d/dr, d²/dr²) are three GEMMs sharing the weight operand. The
+// weights are BF16 values, so only the activation operand needs care: each FP32 activation is
+// split into three BF16 terms x = hi + mid + lo (24 significant bits, the split itself exact to
+// FP32 rounding) and the three partial GEMMs accumulate in FP32 on the MFMA, so the table carries
+// FP32 rounding only, not BF16 rounding of the features and activations.
+__device__ __forceinline__ void split3(float x, short& hi, short& mid, short& lo) {
+  hi = f2bf(x);
+  const float r1 = x - __uint_as_float((unsigned)(unsigned short)hi << 16);
+  mid = f2bf(r1);
+  lo = f2bf(r1 - __uint_as_float((unsigned)(unsigned short)mid << 16));
+}
+
+// D = A B + C over the three BF16 terms of the activation operand (FP32 accumulation).
+__device__ __forceinline__ f32x4 mfma3(const float x[4], const bf16x4& b, f32x4 acc) {
+  bf16x4 xh, xm, xl;
+  for (int v = 0; v < 4; ++v) {
+    short h, m, lo;
+    split3(x[v], h, m, lo);
+    xh[v] = h;
+    xm[v] = m;
+    xl[v] = lo;
+  }
+  acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(xl, b, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(xm, b, acc, 0, 0, 0);
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(xh, b, acc, 0, 0, 0);
+}
+
 __global__ __launch_bounds__(64) void k_mlp_tab_bf16(const unsigned short* __restrict__ Wb,
                                                      const double* __restrict__ W, double rc,
                                                      double rlo, double h, float2* tab) {
-  __shared__ short A1[3][16][MLP_NH];
+  __shared__ float A1[3][16][MLP_NH + 1];
   const int l = threadIdx.x, lr = l & 15, lq = l >> 4;
   const int node0 = blockIdx.x * 16;
   // layer 1: A = features of node (node0 + lr), k = 4 lq + v
   const float r = (float)(rlo + (node0 + lr) * h);
-  bf16x4 a[3];
-  for (int v = 0; v < 4; ++v) {
-    float p0, p1, p2;
-    rbf1<float>(r, (float)rc, 4 * lq + v, p0, p1, p2);
-    a[0][v] = f2bf(p0);
-    a[1][v] = f2bf(p1);
-    a[2][v] = f2bf(p2);
-  }
+  float a[3][4];
+  for (int v = 0; v < 4; ++v) rbf1<float>(r, (float)rc, 4 * lq + v, a[0][v], a[1][v], a[2][v]);
   for (int t = 0; t < 4; ++t) {
     const int n = 16 * t + lr;
     bf16x4 b;
     for (int v = 0; v < 4; ++v) b[v] = (short)Wb[MLP_W1 + n * MLP_NF + 4 * lq + v];
-    f32x4 zero = {0.f, 0.f, 0.f, 0.f};
-    f32x4 d0 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[0], b, zero, 0, 0, 0);
-    f32x4 d1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[1], b, zero, 0, 0, 0);
-    f32x4 d2 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[2], b, zero, 0, 0, 0);
+    const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
+    const f32x4 d0 = mfma3(a[0], b, zero);
+    const f32x4 d1 = mfma3(a[1], b, zero);
+    const f32x4 d2 = mfma3(a[2], b, zero);
     const float b1 = (float)W[MLP_B1 + n];
     for (int v = 0; v < 4; ++v) {
       const int row = 4 * lq + v;
       float s0, s1, s2, s3;
       silu4<float>(d0[v] + b1, s0, s1, s2, s3);
-      A1[0][row][n] = f2bf(s0);
-      A1[1][row][n] = f2bf(s1 * d1[v]);
-      A1[2][row][n] = f2bf(s2 * d1[v] * d1[v] + s1 * d2[v]);
+      A1[0][row][n] = s0;
+      A1[1][row][n] = s1 * d1[v];
+      A1[2][row][n] = s2 * d1[v] * d1[v] + s1 * d2[v];
     }
   }
   __syncthreads();
@@ -127,7 +148,8 @@ __global__ __launch_bounds__(64) void k_mlp_tab_bf16(const unsigned short* __res
     const int out = 16 * t2 + lr;
     f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = acc0, acc2 = acc0;
     for (int ks = 0; ks < 4; ++ks) {
-      bf16x4 b, x0, x1, x2;
+      bf16x4 b;
+      float x0[4], x1[4], x2[4];
       for (int v = 0; v < 4; ++v) {
         const int kin = 16 * ks + 4 * lq + v;
         b[v] = (short)Wb[MLP_W2 + out * MLP_NH + kin];
@@ -135,9 +157,9 @@ __global__ __launch_bounds__(64) void k_mlp_tab_bf16(const unsigned short* __res
         x1[v] = A1[1][lr][kin];
         x2[v] = A1[2][lr][kin];
       }
-      acc0 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(x0, b, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(x1, b, acc1, 0, 0, 0);
-      acc2 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(x2, b, acc2, 0, 0, 0);
+      acc0 = mfma3(x0, b, acc0);
+      acc1 = mfma3(x1, b, acc1);
+      acc2 = mfma3(x2, b, acc2);
     }
     const float b2 = (float)W[MLP_B2 + out], w3 = (float)W[MLP_W3 + out];
     for (int v = 0; v < 4; ++v) {

@@ -58,8 +58,8 @@ def test_extent_rerun(gpu_lib, box3000, prec, tol):
     assert per_atom_rel(F, Fo) < tol
 
 
-@pytest.mark.parametrize("prec,tol", [(FP64, 1e-11), (FP32, 1e-5)])
-def test_ragged_molecule_count(gpu_lib, prec, tol):
+@pytest.mark.parametrize("prec,tol,gtol", [(FP64, 1e-11, 1e-10), (FP32, 1e-5, 1e-5)])
+def test_ragged_molecule_count(gpu_lib, prec, tol, gtol):
     """2,999 waters (not a multiple of the 4 molecules per pair wave, the 16 per pair block or the
     21 per integrator wave) on the cell-list path: pair set, forces and a 10-step gradient."""
     import revsim
@@ -74,8 +74,8 @@ def test_ragged_molecule_count(gpu_lib, prec, tol):
     orc.set_target_rdf(tgt)
     L, g = sim.value_and_grad(x, wb.velocities, 13, 10)
     Lo, go = orc.value_and_grad(x, wb.velocities, 13, 10)
-    assert abs(L - Lo) <= 100 * tol * abs(Lo)
-    assert rel_err(g, go) < 100 * tol, (g, go)
+    assert abs(L - Lo) <= gtol * abs(Lo)
+    assert rel_err(g, go) < gtol, (g, go)
 
 
 def test_invalid_step_counts(gpu_lib, box216):

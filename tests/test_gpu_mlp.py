@@ -3,8 +3,9 @@
 FP64 path: the library's FP64 table + cubic Hermite interpolation differs from the exact network
 by the interpolation error only (< 1e-8 relative in f, < 1e-6 in f'), so forces agree to 1e-9
 relative to the typical force and gradients to 1e-6. FP32 path: the table comes from BF16 MFMA
-GEMMs (BF16 operands, FP32 accumulation), so the MLP part carries BF16-level error (~1e-2 of the
-MLP force, a small part of the total force)."""
+GEMMs with FP32 accumulation whose activation operand is split into three BF16 terms (the weights
+are BF16 values), so the table carries FP32 rounding (~1e-6) and the FP32 gradients and dL/dW meet
+1e-4."""
 import numpy as np
 import pytest
 
@@ -30,7 +31,7 @@ def _pair(prec, n=216, box=1.862, rc=0.8, **kw):
     return sim, orc
 
 
-@pytest.mark.parametrize("prec,tol", [(FP64, 1e-12), (FP32, 3e-2)])
+@pytest.mark.parametrize("prec,tol", [(FP64, 1e-12), (FP32, 1e-5)])
 def test_mlp_table(gpu_lib, prec, tol):
     sim, orc = _pair(prec)
     r, f, df = sim.mlp_table()
@@ -39,14 +40,14 @@ def test_mlp_table(gpu_lib, prec, tol):
     assert np.max(np.abs(df + u2)) <= tol * np.max(np.abs(u2))
 
 
-@pytest.mark.parametrize("prec,tol", [(FP64, 1e-9), (FP32, 1e-4)])
+@pytest.mark.parametrize("prec,tol", [(FP64, 1e-9), (FP32, 1e-5)])
 def test_mlp_forces(gpu_lib, prec, tol, jittered216):
     sim, orc = _pair(prec)
     F, Fo = sim.forces(jittered216), orc.forces(jittered216)
     assert per_atom_rel(F, Fo) < tol, per_atom_rel(F, Fo)
 
 
-@pytest.mark.parametrize("prec,tol,tol_w", [(FP64, 1e-7, 1e-6), (FP32, 1e-4, 3e-2)])
+@pytest.mark.parametrize("prec,tol,tol_w", [(FP64, 1e-7, 1e-6), (FP32, 1e-5, 1e-4)])
 def test_mlp_adjoint_terms_and_weight_vjp(gpu_lib, prec, tol, tol_w, jittered216):
     sim, orc = _pair(prec)
     rng = np.random.default_rng(5)
@@ -62,7 +63,7 @@ def test_mlp_adjoint_terms_and_weight_vjp(gpu_lib, prec, tol, tol_w, jittered216
     assert rel_err(dW, dWo) < tol_w, rel_err(dW, dWo)
 
 
-@pytest.mark.parametrize("prec,tol,tol_w", [(FP64, 1e-6, 1e-5), (FP32, 1e-2, 5e-2)])
+@pytest.mark.parametrize("prec,tol,tol_w", [(FP64, 1e-6, 1e-5), (FP32, 1e-4, 1e-4)])
 def test_mlp_value_and_grad(gpu_lib, prec, tol, tol_w, box216):
     import revsim
 
