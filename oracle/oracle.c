@@ -104,11 +104,18 @@ typedef struct or_ctx {
   pairbuf_t* qbuf; /* per-thread list-build buffers, kept between builds */
   int qbuf_threads;
   double* ubuf;
+  int64_t* qbufF; /* per-thread fixed-point pair-force sums */
+  int float_sums;  /* 1: pair forces summed in FP64 instead (finite-difference tests only) */
+  double* fbond;  /* bonded forces of the last evaluation */
   /* loss terms of the last trajectory call: RDF, enthalpy of vaporization, MSD (averaged over
      blocks, weights applied) and D (nm^2/ps) */
   double terms[4];
   char err[256];
 } or_ctx;
+
+/* fixed-point unit of the pair-force sums: 2^-33 kJ/mol/nm */
+#define FX_SCALE 8589934592.0
+#define FX_INV (1.0 / 8589934592.0)
 
 #define MLP_NF 16
 #define MLP_NH 64
@@ -214,6 +221,8 @@ int or_destroy(or_ctx* o) {
   free(o->cnt);
   free(o->tbuf);
   free(o->ubuf);
+  free(o->qbufF);
+  free(o->fbond);
   for (int t = 0; t < o->qbuf_threads; ++t) {
     free(o->qbuf[t].a);
     free(o->qbuf[t].b);
@@ -744,6 +753,182 @@ int or_mlp_vjp(or_ctx* o, const double* r, const double* g1, const double* g2, i
   return 0;
 }
 
+/* ---------------------------------------------------------------- deterministic log, sincos, acos */
+
+/* log, sin/cos of 2πu and acos from IEEE operations only (+, -, *, /, sqrt, fma, rint), the same
+   fixed polynomials and evaluation order as the GPU library's (the file is built with
+   -ffp-contract=off, so only the explicit fma calls fuse): the Box-Muller noise and the angle force
+   then agree bit for bit, and so does the FP64 trajectory.
+     log(x): x = m 2^k, m in (1/√2, √2], log m = 2 atanh((m-1)/(m+1)) to s^25, k ln2 (fdlibm split)
+     sincos(2πu): 4u = q + r, q = rint(4u), θ = r π/2, Taylor to θ^19 / θ^20, quadrant q
+     acos(x): π/2 - asin x (|x| <= 1/2), 2 asin √((1-x)/2), π - 2 asin √((1+x)/2); asin series to
+              n = 25 */
+static inline double rs_log(double x) {
+  uint64_t bits;
+  memcpy(&bits, &x, 8);
+  int k = (int)((bits >> 52) & 0x7ffULL) - 1023;
+  const uint64_t mb = (bits & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL;
+  double m;
+  memcpy(&m, &mb, 8);
+  if (m > 1.4142135623730951) {
+    m = m * 0.5;
+    k += 1;
+  }
+  const double s = (m - 1.0) / (m + 1.0);
+  const double z = s * s;
+  double p = 0.04;
+  p = fma(p, z, 0.043478260869565216);
+  p = fma(p, z, 0.047619047619047616);
+  p = fma(p, z, 0.05263157894736842);
+  p = fma(p, z, 0.058823529411764705);
+  p = fma(p, z, 0.06666666666666667);
+  p = fma(p, z, 0.07692307692307693);
+  p = fma(p, z, 0.09090909090909091);
+  p = fma(p, z, 0.1111111111111111);
+  p = fma(p, z, 0.14285714285714285);
+  p = fma(p, z, 0.2);
+  p = fma(p, z, 0.3333333333333333);
+  p = fma(p, z, 1.0);
+  const double lm = (2.0 * s) * p;
+  const double kd = (double)k;
+  return fma(kd, 6.93147180369123816490e-01, fma(kd, 1.90821492927058770002e-10, lm));
+}
+
+static inline void rs_sincos2pi(double u, double* sn, double* cs) {
+  const double t = 4.0 * u;
+  const double q = rint(t);
+  const double th = (t - q) * 1.5707963267948966;
+  const double z = th * th;
+  double ps = -8.22063524662433e-18;
+  ps = fma(ps, z, 2.8114572543455206e-15);
+  ps = fma(ps, z, -7.647163731819816e-13);
+  ps = fma(ps, z, 1.6059043836821613e-10);
+  ps = fma(ps, z, -2.505210838544172e-08);
+  ps = fma(ps, z, 2.7557319223985893e-06);
+  ps = fma(ps, z, -0.0001984126984126984);
+  ps = fma(ps, z, 0.008333333333333333);
+  ps = fma(ps, z, -0.16666666666666666);
+  ps = fma(ps, z, 1.0);
+  double pc = 4.110317623312165e-19;
+  pc = fma(pc, z, -1.5619206968586225e-16);
+  pc = fma(pc, z, 4.779477332387385e-14);
+  pc = fma(pc, z, -1.1470745597729725e-11);
+  pc = fma(pc, z, 2.08767569878681e-09);
+  pc = fma(pc, z, -2.755731922398589e-07);
+  pc = fma(pc, z, 2.48015873015873e-05);
+  pc = fma(pc, z, -0.001388888888888889);
+  pc = fma(pc, z, 0.041666666666666664);
+  pc = fma(pc, z, -0.5);
+  pc = fma(pc, z, 1.0);
+  const double sv = th * ps;
+  switch (((int)q) & 3) {
+    case 0: *sn = sv; *cs = pc; break;
+    case 1: *sn = pc; *cs = -sv; break;
+    case 2: *sn = -sv; *cs = -pc; break;
+    default: *sn = -pc; *cs = sv; break;
+  }
+}
+
+static inline double rs_asin_series(double y) {
+  const double z = y * y;
+  double p = 0.0022014739737101384;
+  p = fma(p, z, 0.002338091892111975);
+  p = fma(p, z, 0.0024894486782468836);
+  p = fma(p, z, 0.00265787063820729);
+  p = fma(p, z, 0.002846178401108942);
+  p = fma(p, z, 0.0030578216492580306);
+  p = fma(p, z, 0.003297059503473485);
+  p = fma(p, z, 0.0035692053938259347);
+  p = fma(p, z, 0.003880964558837669);
+  p = fma(p, z, 0.004240907093679363);
+  p = fma(p, z, 0.004660143486915096);
+  p = fma(p, z, 0.005153309682319905);
+  p = fma(p, z, 0.005740037670841924);
+  p = fma(p, z, 0.006447210311889649);
+  p = fma(p, z, 0.0073125258735988454);
+  p = fma(p, z, 0.008390335809616815);
+  p = fma(p, z, 0.009761609529194078);
+  p = fma(p, z, 0.011551800896139705);
+  p = fma(p, z, 0.01396484375);
+  p = fma(p, z, 0.017352764423076924);
+  p = fma(p, z, 0.022372159090909092);
+  p = fma(p, z, 0.030381944444444444);
+  p = fma(p, z, 0.044642857142857144);
+  p = fma(p, z, 0.075);
+  p = fma(p, z, 0.16666666666666666);
+  p = fma(p, z, 1.0);
+  return y * p;
+}
+
+static inline double rs_acos(double x) {
+  const double pio2_hi = 1.57079632679489655800e+00, pio2_lo = 6.12323399573676603587e-17;
+  if (x > 0.5) return 2.0 * rs_asin_series(sqrt(0.5 * (1.0 - x)));
+  if (x < -0.5) return 2.0 * pio2_hi - (2.0 * rs_asin_series(sqrt(0.5 * (1.0 + x))) - 2.0 * pio2_lo);
+  return pio2_hi - (rs_asin_series(x) - pio2_lo);
+}
+
+
+/* ---------------------------------------------------------------- bonded forces, library order */
+
+static inline void cross3c(const double a[3], const double b[3], double c[3]) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+static inline double dot3c(const double a[3], const double b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+/* Bond and angle forces of one water (calc_F!, water/water.jl:915-968) in the GPU library's form
+   and operation order, so that the FP64 forces agree bit for bit: n = d (1/r) for the bonds; for
+   the angle (a = H1 - O, b = H2 - O, c = a x b) F_H1 = k_a (θ0 - θ) (a x c)/(|a|^2 |c|),
+   F_H2 = k_a (θ0 - θ) (c x b)/(|b|^2 |c|), F_O = -(F_H1 + F_H2), with θ = acos(a.b / (|a||b|))
+   from rs_acos. (The HVP and the θ-VJP, which do not steer the trajectory, come from the
+   dual-number form in eval_terms.) */
+static void bonded_force_mol(const or_ctx* o, const double* xm, double F[9]) {
+  const double r0 = o->theta[4], kb = o->theta[3], ka = o->theta[5];
+  for (int t = 0; t < 9; ++t) F[t] = 0.0;
+  double a[3], b[3];
+  for (int k = 0; k < 3; ++k) {
+    a[k] = mic1(xm[3 + k] - xm[k], o->L[k], o->invL[k]);
+    b[k] = mic1(xm[6 + k] - xm[k], o->L[k], o->invL[k]);
+  }
+  for (int hh = 1; hh <= 2; ++hh) {
+    const double* d = hh == 1 ? a : b;
+    const double r = sqrt(r2_of(d));
+    const double ir = 1.0 / r;
+    const double n[3] = {d[0] * ir, d[1] * ir, d[2] * ir};
+    const double disp = r - r0;
+    const double Fv = -kb * disp;
+    for (int k = 0; k < 3; ++k) {
+      F[k] -= Fv * n[k];
+      F[3 * hh + k] += Fv * n[k];
+    }
+  }
+  double c[3];
+  cross3c(a, b, c);
+  if (c[0] == 0.0 && c[1] == 0.0 && c[2] == 0.0) return;
+  const double aa = dot3c(a, a), bb = dot3c(b, b), ab = dot3c(a, b), cc = dot3c(c, c);
+  const double P = aa * bb;
+  const double iP = 1.0 / P;
+  const double rab = sqrt(P);
+  const double irab = rab * iP;
+  const double rc = sqrt(cc), irc = 1.0 / rc;
+  const double cs = ab * irab;
+  const double th = rs_acos(cs);
+  const double disp = o->cfg.theta0 - th;
+  const double Fa = ka * disp;
+  const double s1 = bb * iP * irc, s3 = aa * iP * irc;
+  double p1u[3], p3u[3];
+  cross3c(a, c, p1u);
+  cross3c(c, b, p3u);
+  const double c1 = Fa * s1, c3 = Fa * s3;
+  for (int k = 0; k < 3; ++k) {
+    const double gi = c1 * p1u[k], gk = c3 * p3u[k];
+    F[3 + k] += gi;
+    F[6 + k] += gk;
+    F[k] -= gi + gk;
+  }
+}
+
 /* ---------------------------------------------------------------- forces and adjoint terms */
 
 /* One pass over all interactions. With c == NULL only forces (and energy) are produced
@@ -765,10 +950,17 @@ static void eval_terms(or_ctx* o, const double* x, const double* c, const double
 #endif
   if (o->tbuf_threads < nth) {
     free(o->tbuf);
+    free(o->qbufF);
     o->tbuf = (double*)malloc(sizeof(double) * 2 * (size_t)N3 * nth);
+    o->qbufF = (int64_t*)malloc(sizeof(int64_t) * (size_t)N3 * nth);
     o->tbuf_threads = nth;
   }
   if (!o->ubuf) o->ubuf = (double*)malloc(sizeof(double) * N3);
+  if (!o->fbond) o->fbond = (double*)malloc(sizeof(double) * N3);
+  /* the pair-force constants exactly as the library forms them (per pair type O-O, O-H, H-H) */
+  const double kqq[3] = {kc * qO * qO, kc * qO * qH, kc * qH * qH};
+  const double kBq[3] = {kqq[0] * 2.0 * krf, kqq[1] * 2.0 * krf, kqq[2] * 2.0 * krf};
+  const double sig2c = sig * sig, eps24c = 24.0 * eps;
   double* Fb = o->tbuf;
   double* hb = adj ? o->tbuf + (size_t)N3 * nth : NULL;
   double* vb = (double*)calloc((size_t)NTH * nth, sizeof(double));
@@ -791,6 +983,7 @@ static void eval_terms(or_ctx* o, const double* x, const double* c, const double
 #endif
     double* Ft = Fb + (size_t)N3 * t;
     double* ht = adj ? hb + (size_t)N3 * t : NULL;
+    int64_t* Fqt = o->qbufF + (size_t)N3 * t;
     {
       /* each thread zeroes its own buffer (and any of a team smaller than requested) */
       int T = 1;
@@ -799,6 +992,7 @@ static void eval_terms(or_ctx* o, const double* x, const double* c, const double
 #endif
       for (int w = t; w < nth; w += T) {
         memset(Fb + (size_t)N3 * w, 0, sizeof(double) * N3);
+        memset(o->qbufF + (size_t)N3 * w, 0, sizeof(int64_t) * N3);
         if (adj) memset(hb + (size_t)N3 * w, 0, sizeof(double) * N3);
       }
     }
@@ -855,9 +1049,30 @@ static void eval_terms(or_ctx* o, const double* x, const double* c, const double
           mlp_backward(o->mlp, &J, -wd, 0.0, dWt); /* dF/dW . w = -(du'/dW) r̂.(w_b - w_a) */
         }
       }
-      for (int k = 0; k < 3; ++k) {
-        Ft[3 * a + k] -= Fv * ndr[k];
-        Ft[3 * b + k] += Fv * ndr[k];
+      {
+        /* the force in the library's form and operation order: f/r from powers of 1/r,
+           F_a -= (f/r) d, one term round((f/r) d_t 2^33) per component summed as 64-bit integers,
+           so the sum is exact and independent of pair and thread order */
+        const int ty = (!aO) + (!bO);
+        const double ri = 1.0 / sqrt(r2), ri2 = ri * ri, ri3 = ri2 * ri;
+        double ftot = fma(kqq[ty], ri3, -kBq[ty]);
+        if (aO && bO) {
+          const double s2 = sig2c * ri2, s6 = s2 * s2 * s2, s12 = s6 * s6;
+          ftot = ftot + (eps24c * ri2) * fma(2.0, s12, -s6);
+        }
+        if (aO && bO && mlp_on) ftot = ftot + (-J.u1) * ri;
+        if (o->float_sums) {
+          for (int k = 0; k < 3; ++k) {
+            Ft[3 * a + k] -= ftot * dr[k];
+            Ft[3 * b + k] += ftot * dr[k];
+          }
+        } else {
+          for (int k = 0; k < 3; ++k) {
+            const int64_t q = (int64_t)rint((ftot * dr[k]) * FX_SCALE);
+            Fqt[3 * a + k] += q;
+            Fqt[3 * b + k] -= q;
+          }
+        }
       }
       if (adj) {
         /* K = F' r̂r̂ᵀ + (F/r)(I - r̂r̂ᵀ); h_a += K (u_a - u_b); h_b -= K (u_a - u_b) */
@@ -888,6 +1103,7 @@ static void eval_terms(or_ctx* o, const double* x, const double* c, const double
 #pragma omp for schedule(static)
     for (int m = 0; m < o->M; ++m) {
       int O = 3 * m;
+      bonded_force_mol(o, x + 9 * (size_t)m, o->fbond + 9 * (size_t)m);
       for (int hh = 1; hh <= 2; ++hh) {
         int i = O, j = O + hh;
         double dr[3];
@@ -902,10 +1118,6 @@ static void eval_terms(or_ctx* o, const double* x, const double* c, const double
         if (dU) {
           dUt[3] += 0.5 * disp * disp;
           dUt[4] += -kb * disp;
-        }
-        for (int k = 0; k < 3; ++k) {
-          Ft[3 * i + k] -= Fv * ndr[k];
-          Ft[3 * j + k] += Fv * ndr[k];
         }
         if (adj) {
           double du[3], dot = 0.0;
@@ -935,9 +1147,6 @@ static void eval_terms(or_ctx* o, const double* x, const double* c, const double
         et += ea;
         if (dU) dUt[5] += 0.5 * disp * disp;
         for (int k = 0; k < 3; ++k) {
-          Ft[3 * (O + 1) + k] += fi[k];
-          Ft[3 * O + k] += fj[k];
-          Ft[3 * (O + 2) + k] += fk[k];
           if (adj) {
             ht[3 * (O + 1) + k] += hi[k];
             ht[3 * O + k] += hj[k];
@@ -950,12 +1159,17 @@ static void eval_terms(or_ctx* o, const double* x, const double* c, const double
     }
     eb[t] = et;
   }
-  /* thread buffers summed in thread order (deterministic for a given thread count) */
+  /* forces: bonded + the exact integer pair sum; the HVP's thread buffers summed in thread order
+     (deterministic for a given thread count) */
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < N3; ++i) {
-    double s = 0.0;
-    for (int t = 0; t < nth; ++t) s += Fb[(size_t)N3 * t + i];
-    F[i] = s;
+    int64_t q = 0;
+    double fs = 0.0;
+    for (int t = 0; t < nth; ++t) {
+      q += o->qbufF[(size_t)N3 * t + i];
+      fs += Fb[(size_t)N3 * t + i];
+    }
+    F[i] = o->fbond[i] + (o->float_sums ? fs : -((double)q * FX_INV));
     if (adj) {
       double sh = 0.0;
       for (int t = 0; t < nth; ++t) sh += hb[(size_t)N3 * t + i];
@@ -1069,11 +1283,13 @@ static void normals3(uint64_t seed, int32_t step, int a, double n[3]) {
   const double two53 = 1.0 / 9007199254740992.0;
   double u1 = (double)((r[0] >> 11) + 1) * two53, u2 = (double)(r[1] >> 11) * two53;
   double u3 = (double)((r[2] >> 11) + 1) * two53, u4 = (double)(r[3] >> 11) * two53;
-  double rad1 = sqrt(-2.0 * log(u1)), rad2 = sqrt(-2.0 * log(u3));
-  const double twopi = 6.283185307179586;
-  n[0] = rad1 * cos(twopi * u2);
-  n[1] = rad1 * sin(twopi * u2);
-  n[2] = rad2 * cos(twopi * u4);
+  const double rad1 = sqrt(-2.0 * rs_log(u1)), rad2 = sqrt(-2.0 * rs_log(u3));
+  double s2, c2, s4, c4;
+  rs_sincos2pi(u2, &s2, &c2);
+  rs_sincos2pi(u4, &s4, &c4);
+  n[0] = rad1 * c2;
+  n[1] = rad1 * s2;
+  n[2] = rad2 * c4;
 }
 
 /* generate_noise! (water/water.jl:867-873): ξ_a = sqrt(kB T / m_a) · N(0, I3). */
@@ -1268,7 +1484,9 @@ int or_msd(or_ctx* o, const double* frames, int32_t nf, double* Dout, double* lo
   double* del = (double*)malloc(sizeof(double) * (size_t)K * M * 3);
   double Dsum = 0.0;
   for (int s = 0; s < K; ++s) {
-    double msd = 0.0;
+    /* Σ_i |Δ_i|^2 in the library's order: index order within chunks of 256 molecules, then the
+       chunk sums in order */
+    double msd = 0.0, part = 0.0;
     for (int i = 0; i < M; ++i) {
       double acc[3] = {0, 0, 0};
       for (int f = s; f < s + K; ++f) {
@@ -1277,7 +1495,11 @@ int or_msd(or_ctx* o, const double* frames, int32_t nf, double* Dout, double* lo
         for (int k = 0; k < 3; ++k) acc[k] += d[k];
       }
       for (int k = 0; k < 3; ++k) del[((size_t)s * M + i) * 3 + k] = acc[k];
-      msd += acc[0] * acc[0] + acc[1] * acc[1] + acc[2] * acc[2];
+      part += acc[0] * acc[0] + acc[1] * acc[1] + acc[2] * acc[2];
+      if ((i + 1) % 256 == 0 || i == M - 1) {
+        msd += part;
+        part = 0.0;
+      }
     }
     msd /= M;
     Dsum += msd / lag;
@@ -1643,12 +1865,12 @@ int or_reverse(or_ctx* o, const double* xN, const double* vN, uint64_t seed, int
   memcpy(x, xN, sizeof(double) * N3);
   memcpy(v, vN, sizeof(double) * N3);
   o->terms[0] = o->terms[1] = o->terms[2] = o->terms[3] = 0.0;
+  double lmsd = 0.0; /* added after the block losses, in the library's order */
   if (p.msd_on) {
-    double lm, Dm;
+    double Dm;
     dLdf = (double*)malloc(sizeof(double) * (size_t)p.nf * 3 * M);
-    or_msd(o, o->frames, p.nf, &Dm, &lm, dLdf);
-    loss += lm;
-    o->terms[2] = lm;
+    or_msd(o, o->frames, p.nf, &Dm, &lmsd, dLdf);
+    o->terms[2] = lmsd;
     o->terms[3] = Dm;
   }
   double th[NTH] = {0};
@@ -1737,6 +1959,7 @@ int or_reverse(or_ctx* o, const double* xN, const double* vN, uint64_t seed, int
       }
     }
   }
+  loss += lmsd;
   if (loss_out) *loss_out = loss;
   if (dLdth) memcpy(dLdth, th, sizeof(th));
   if (x0_out) memcpy(x0_out, x, sizeof(double) * N3);
@@ -1857,6 +2080,20 @@ int or_reweight(or_ctx* o, const double* x0, const double* v0, uint64_t seed, in
 }
 
 double or_list_radius(const or_ctx* o) { return o->rlist; }
+
+/* Pair forces summed in FP64 (1) or as the library's exact fixed-point integers (0, the
+   default). Finite differences of forces with tiny steps resolve below the fixed-point unit
+   (2^-33 kJ/mol/nm per term), so those tests use FP64 sums; a separate test bounds the difference
+   between the two. */
+int or_set_float_sums(or_ctx* o, int on) {
+  o->float_sums = on != 0;
+  return 0;
+}
+
+/* the deterministic elementary functions, for their accuracy tests */
+double or_log(double x) { return rs_log(x); }
+void or_sincos2pi(double u, double* sn, double* cs) { rs_sincos2pi(u, sn, cs); }
+double or_acos(double x) { return rs_acos(x); }
 
 /* (l_rdf, l_ev, l_msd, D in m^2/s) of the last forward / reverse call, as rs_loss_terms. */
 int or_loss_terms(const or_ctx* o, double* t) {

@@ -42,6 +42,7 @@ _SIGS = [
     ("or_value_and_grad_stored", ctypes.c_int, [_vp, _dp, _dp, _u64, ctypes.c_int32, _dp, _dp]),
     ("or_list_radius", ctypes.c_double, [_vp]),
     ("or_loss_terms", ctypes.c_int, [_vp, _dp]),
+    ("or_set_float_sums", ctypes.c_int, [_vp, ctypes.c_int]),
     ("or_set_mlp", ctypes.c_int, [_vp, _P(ctypes.c_float), ctypes.c_int64]),
     ("or_set_mlp_f64", ctypes.c_int, [_vp, _dp, ctypes.c_int64]),
     ("or_mlp_size", ctypes.c_int, []),
@@ -54,6 +55,9 @@ _SIGS = [
     ("or_get_box", ctypes.c_int, [_vp, _dp]),
     ("or_reweight", ctypes.c_int, [_vp, _dp, _dp, _u64, ctypes.c_int32, _dp, _dp, _dp, _dp]),
     ("or_exp", ctypes.c_double, [ctypes.c_double]),
+    ("or_log", ctypes.c_double, [ctypes.c_double]),
+    ("or_sincos2pi", None, [ctypes.c_double, _dp, _dp]),
+    ("or_acos", ctypes.c_double, [ctypes.c_double]),
     ("or_philox4x64", None, [_P(ctypes.c_uint64), _P(ctypes.c_uint64), _P(ctypes.c_uint64)]),
 ]
 
@@ -218,6 +222,10 @@ class Oracle:
 
     def list_radius(self):
         return self.L.or_list_radius(self.h)
+
+    def set_float_sums(self, on=True):
+        """Pair forces summed in FP64 instead of the library's exact fixed-point integers."""
+        self._chk(self.L.or_set_float_sums(self.h, int(on)))
 
     def loss_terms(self):
         t = np.zeros(4)

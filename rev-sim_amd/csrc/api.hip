@@ -222,7 +222,6 @@ int alloc_all(rs_ctx* c) {
   if ((r = dalloc(c, &b.hist, nb)) || (r = dalloc(c, &b.rdfG, nb)) || (r = dalloc(c, &b.target, nb)) ||
       (r = dalloc(c, &b.rdf_g, nb)))
     return r;
-  if ((r = dalloc(c, &b.msd_acc, 64))) return r;
   if ((r = dalloc(c, &b.Fe, N3)) || (r = dalloc(c, &b.eslots, (size_t)pair_blocks(p) * 8)) ||
       (r = dalloc(c, &b.eout, 8)) || (r = dalloc(c, &b.rwacc, 32)))
     return r;
@@ -401,10 +400,14 @@ int ensure_frames(rs_ctx* c, int nf) {
   if (b.frames) hipFree(b.frames);
   if (b.msd_grad) hipFree(b.msd_grad);
   if (b.msd_del) hipFree(b.msd_del);
+  if (b.msd_sq) hipFree(b.msd_sq);
+  if (b.msd_part) hipFree(b.msd_part);
   int r;
   if ((r = dalloc(c, &b.frames, (size_t)nf * 3 * c->M)) ||
       (r = dalloc(c, &b.msd_grad, (size_t)nf * 3 * c->M)) ||
-      (r = dalloc(c, &b.msd_del, (size_t)(nf / 2 + 1) * 3 * c->M)))
+      (r = dalloc(c, &b.msd_del, (size_t)(nf / 2 + 1) * 3 * c->M)) ||
+      (r = dalloc(c, &b.msd_sq, (size_t)(nf / 2 + 1) * c->M)) ||
+      (r = dalloc(c, &b.msd_part, (size_t)(nf / 2 + 1) * ((c->M + 255) / 256))))
     return r;
   c->frames_cap = nf;
   return 0;
@@ -477,6 +480,8 @@ int finish(rs_ctx* c, int* retry) {
   HIPCHK(c, hipMemcpy(f, c->L.b.flags, sizeof(f), hipMemcpyDeviceToHost));
   if (f[FLAG_NONFINITE]) return fail(c, RS_E_NONFINITE, "non-finite positions, velocities or forces");
   if (f[FLAG_MLP_RANGE]) return fail(c, RS_E_NONFINITE, "an O-O pair came closer than the MLP table's range");
+  if (f[FLAG_FIXED_RANGE])
+    return fail(c, RS_E_OVERFLOW, "an FP64 pair-force term exceeded the fixed-point sums' range (2^18 kJ/mol/nm)");
   *retry = (f[FLAG_LIST_OVERFLOW] ? 1 : 0) | (f[FLAG_STALE] ? 2 : 0) | (f[FLAG_HEXTENT] ? 4 : 0) |
            (f[FLAG_CELL_OVERFLOW] ? 8 : 0);
   return 0;
@@ -782,6 +787,8 @@ int single_eval(rs_ctx* c, const double* x, bool with_u, Body body) {
     int f[FLAG_COUNT];
     HIPCHK(c, hipMemcpy(f, b.flags, sizeof(f), hipMemcpyDeviceToHost));
     if (f[FLAG_NONFINITE]) return fail(c, RS_E_NONFINITE, "non-finite forces");
+    if (f[FLAG_FIXED_RANGE])
+      return fail(c, RS_E_OVERFLOW, "an FP64 pair-force term exceeded the fixed-point sums' range (2^18 kJ/mol/nm)");
     // a single evaluation builds its list at x: only capacity, the O-H extent and a cell's
     // occupancy can fail
     const int why = (f[FLAG_LIST_OVERFLOW] ? 1 : 0) | (f[FLAG_HEXTENT] ? 4 : 0) | (f[FLAG_CELL_OVERFLOW] ? 8 : 0);
@@ -932,7 +939,7 @@ int rs_destroy(rs_ctx* c) {
   void* ptrs[] = {b.x, b.v, b.F, b.h, b.A, b.B, b.C, b.D, b.grad, b.cell, b.cell_count,
                   b.cell_start, b.cell_fill, b.tmp, b.order, b.rank, b.posq, b.pos64, b.posq_alt, b.pos64_alt, b.pu,
                   b.ud, b.O64, b.Oq, b.nbr, b.nbr_tmp, b.ncnt, b.flags, b.slots, b.vjb, b.dth, b.scal, b.wmax, b.hist, b.rdfG,
-                  b.target, b.rdf_g, b.frames, b.msd_del, b.msd_acc, b.msd_grad, b.ext_noise,
+                  b.target, b.rdf_g, b.frames, b.msd_del, b.msd_sq, b.msd_part, b.msd_grad, b.ext_noise,
                   b.hpart, b.mlp_w, b.mlp_wb, b.mlp_tab, b.mlp_slot, b.mlp_G, b.mlp_part, b.mlp_dW,
                   b.Fe, b.eslots, b.eout, b.rwacc, b.xbak};
   for (void* q : ptrs)

@@ -10,11 +10,13 @@
 #include "common.h"
 
 __device__ __forceinline__ void cross3(const double a[3], const double b[3], double c[3]) {
+#pragma clang fp contract(off)
   c[0] = a[1] * b[2] - a[2] * b[1];
   c[1] = a[2] * b[0] - a[0] * b[2];
   c[2] = a[0] * b[1] - a[1] * b[0];
 }
 __device__ __forceinline__ double dot3(const double a[3], const double b[3]) {
+#pragma clang fp contract(off)
   return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
 }
 
@@ -28,9 +30,13 @@ __device__ __forceinline__ double dot3(const double a[3], const double b[3]) {
 // dθ/d(cos θ) = -1/sin θ = -|a||b|/|c|. Written so, the value needs two divisions (1/(|a|^2|b|^2)
 // and 1/|c|) and two square roots, and its tangent along (u_H1 - u_O, u_H2 - u_O) needs none
 // more: FP64 divisions are long dependent sequences, and these kernels run few waves per SIMD.
+//
+// Evaluated without contraction, as written, with the library's own acos (common.h): the CPU
+// oracle evaluates the forces by the same sequence, so they agree bit for bit.
 template <bool ADJ>
 __device__ void bonded_mol(const SysParams& p, const double* xm, const double* um, const double* wm,
                            double F[9], double h[9], double vj[3], int* flags, double4 o_build) {
+#pragma clang fp contract(off)
 #pragma unroll
   for (int t = 0; t < 9; ++t) {
     F[t] = 0.0;
@@ -97,7 +103,7 @@ __device__ void bonded_mol(const SysParams& p, const double* xm, const double* u
   const double irab = rab * iP;        // 1/(|a||b|)
   const double rc = sqrt(cc), irc = 1.0 / rc;
   const double cs = ab * irab;
-  const double th = acos(cs);
+  const double th = rs_acos(cs);
   const double disp = p.theta0 - th;
   const double Fa = p.ka * disp;
   const double s1 = bb * iP * irc, s3 = aa * iP * irc;  // 1/(|a|^2|c|), 1/(|b|^2|c|)

@@ -17,6 +17,7 @@ enum {
   FLAG_HEXTENT = 3,        // an O-H distance exceeded the extent assumed by the list radius
   FLAG_MLP_RANGE = 4,      // an O-O pair closer than the MLP table's lower end
   FLAG_CELL_OVERFLOW = 5,  // a home cell of the list grid held more molecules than a build block stages
+  FLAG_FIXED_RANGE = 6,    // an FP64 pair-force term beyond the fixed-point sums' exact range (2^18)
   FLAG_COUNT = 8
 };
 
@@ -64,6 +65,14 @@ struct SysParams {
 __device__ __forceinline__ double mic1(double d, double L, double invL) {
 #pragma clang fp contract(off)
   return d - L * rint(d * invL);
+}
+
+// a + b evaluated as written: instructions made in this function carry no contraction flag, so the
+// compiler cannot fuse the add with a multiply that feeds it (a plain a + b in a kernel compiled
+// with contraction may become an fma, which rounds differently).
+__device__ __forceinline__ double add_nc(double a, double b) {
+#pragma clang fp contract(off)
+  return a + b;
 }
 
 __device__ __forceinline__ double r2_of(double dx, double dy, double dz) {
@@ -142,9 +151,127 @@ __device__ __forceinline__ double rs_exp(double t) {
 }
 
 // ----------------------------------------------------------------------------------------------
+// log, sin/cos of 2πu and acos from IEEE operations only (+, -, *, /, sqrt, fma, rint): fixed
+// polynomials evaluated by fused multiply-adds in a fixed order, so the CPU oracle, which
+// evaluates the same sequences, gets the same bits. They feed the Langevin noise (Box-Muller)
+// and the angle force, i.e. the trajectory itself.
+//   log(x), x > 0 normal: x = m 2^k with m in (1/√2, √2], log m = 2 atanh(s), s = (m-1)/(m+1),
+//     |s| <= 0.1716: the atanh series to s^25, k ln2 with fdlibm's two-part ln2.
+//   sincos(2πu): 4u = q + r with q = rint(4u), |r| <= 1/2 (both exact), θ = r π/2: Taylor series
+//     of sin θ to θ^19 and cos θ to θ^20 (|θ| <= π/4), then the quadrant q.
+//   acos(x): π/2 - asin x for |x| <= 1/2, 2 asin √((1-x)/2) above, π - 2 asin √((1+x)/2) below;
+//     asin y = y Σ_n C(2n,n)/(4^n (2n+1)) y^(2n), to n = 25 (|y| <= 1/2).
+// Accuracy: a few units in the last place (tests compare them with the C library's).
+// ----------------------------------------------------------------------------------------------
+__device__ __forceinline__ double rs_log(double x) {
+#pragma clang fp contract(off)
+  const unsigned long long bits = (unsigned long long)__double_as_longlong(x);
+  int k = (int)((bits >> 52) & 0x7ffULL) - 1023;
+  double m = __longlong_as_double((long long)((bits & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL));
+  if (m > 1.4142135623730951) {
+    m = m * 0.5;
+    k += 1;
+  }
+  const double s = (m - 1.0) / (m + 1.0);
+  const double z = s * s;
+  double p = 0.04;
+  p = fma(p, z, 0.043478260869565216);
+  p = fma(p, z, 0.047619047619047616);
+  p = fma(p, z, 0.05263157894736842);
+  p = fma(p, z, 0.058823529411764705);
+  p = fma(p, z, 0.06666666666666667);
+  p = fma(p, z, 0.07692307692307693);
+  p = fma(p, z, 0.09090909090909091);
+  p = fma(p, z, 0.1111111111111111);
+  p = fma(p, z, 0.14285714285714285);
+  p = fma(p, z, 0.2);
+  p = fma(p, z, 0.3333333333333333);
+  p = fma(p, z, 1.0);
+  const double lm = (2.0 * s) * p;
+  const double kd = (double)k;
+  return fma(kd, 6.93147180369123816490e-01, fma(kd, 1.90821492927058770002e-10, lm));
+}
+
+__device__ __forceinline__ void rs_sincos2pi(double u, double* sn, double* cs) {
+#pragma clang fp contract(off)
+  const double t = 4.0 * u;
+  const double q = rint(t);
+  const double th = (t - q) * 1.5707963267948966;
+  const double z = th * th;
+  double ps = -8.22063524662433e-18;
+  ps = fma(ps, z, 2.8114572543455206e-15);
+  ps = fma(ps, z, -7.647163731819816e-13);
+  ps = fma(ps, z, 1.6059043836821613e-10);
+  ps = fma(ps, z, -2.505210838544172e-08);
+  ps = fma(ps, z, 2.7557319223985893e-06);
+  ps = fma(ps, z, -0.0001984126984126984);
+  ps = fma(ps, z, 0.008333333333333333);
+  ps = fma(ps, z, -0.16666666666666666);
+  ps = fma(ps, z, 1.0);
+  double pc = 4.110317623312165e-19;
+  pc = fma(pc, z, -1.5619206968586225e-16);
+  pc = fma(pc, z, 4.779477332387385e-14);
+  pc = fma(pc, z, -1.1470745597729725e-11);
+  pc = fma(pc, z, 2.08767569878681e-09);
+  pc = fma(pc, z, -2.755731922398589e-07);
+  pc = fma(pc, z, 2.48015873015873e-05);
+  pc = fma(pc, z, -0.001388888888888889);
+  pc = fma(pc, z, 0.041666666666666664);
+  pc = fma(pc, z, -0.5);
+  pc = fma(pc, z, 1.0);
+  const double sv = th * ps;
+  switch (((int)q) & 3) {
+    case 0: *sn = sv; *cs = pc; break;
+    case 1: *sn = pc; *cs = -sv; break;
+    case 2: *sn = -sv; *cs = -pc; break;
+    default: *sn = -pc; *cs = sv; break;
+  }
+}
+
+__device__ __forceinline__ double rs_asin_series(double y) {
+#pragma clang fp contract(off)
+  const double z = y * y;
+  double p = 0.0022014739737101384;
+  p = fma(p, z, 0.002338091892111975);
+  p = fma(p, z, 0.0024894486782468836);
+  p = fma(p, z, 0.00265787063820729);
+  p = fma(p, z, 0.002846178401108942);
+  p = fma(p, z, 0.0030578216492580306);
+  p = fma(p, z, 0.003297059503473485);
+  p = fma(p, z, 0.0035692053938259347);
+  p = fma(p, z, 0.003880964558837669);
+  p = fma(p, z, 0.004240907093679363);
+  p = fma(p, z, 0.004660143486915096);
+  p = fma(p, z, 0.005153309682319905);
+  p = fma(p, z, 0.005740037670841924);
+  p = fma(p, z, 0.006447210311889649);
+  p = fma(p, z, 0.0073125258735988454);
+  p = fma(p, z, 0.008390335809616815);
+  p = fma(p, z, 0.009761609529194078);
+  p = fma(p, z, 0.011551800896139705);
+  p = fma(p, z, 0.01396484375);
+  p = fma(p, z, 0.017352764423076924);
+  p = fma(p, z, 0.022372159090909092);
+  p = fma(p, z, 0.030381944444444444);
+  p = fma(p, z, 0.044642857142857144);
+  p = fma(p, z, 0.075);
+  p = fma(p, z, 0.16666666666666666);
+  p = fma(p, z, 1.0);
+  return y * p;
+}
+
+__device__ __forceinline__ double rs_acos(double x) {
+#pragma clang fp contract(off)
+  const double pio2_hi = 1.57079632679489655800e+00, pio2_lo = 6.12323399573676603587e-17;
+  if (x > 0.5) return 2.0 * rs_asin_series(sqrt(0.5 * (1.0 - x)));
+  if (x < -0.5) return 2.0 * pio2_hi - (2.0 * rs_asin_series(sqrt(0.5 * (1.0 + x))) - 2.0 * pio2_lo);
+  return pio2_hi - (rs_asin_series(x) - pio2_lo);
+}
+
+// ----------------------------------------------------------------------------------------------
 // Philox4x64-10 (Salmon, Moraes, Dror, Shaw, SC'11), the generator numpy exposes as
 // np.random.Philox. Counter (step, atom, 0, 0), key (seed, 0); four 64-bit outputs give the three
-// normals of one atom by Box-Muller on 53-bit uniforms.
+// normals of one atom by Box-Muller on 53-bit uniforms (rs_log, rs_sincos2pi above).
 // ----------------------------------------------------------------------------------------------
 __device__ __forceinline__ void philox4x64(uint64_t c[4], uint64_t k0, uint64_t k1) {
   const uint64_t M0 = 0xD2E7470EE14C6C93ULL, M1 = 0xCA5A826395121157ULL;
@@ -169,11 +296,13 @@ __device__ __forceinline__ void normals3(uint64_t seed, int step, int atom, doub
   const double two53 = 1.0 / 9007199254740992.0;
   double u1 = (double)((c[0] >> 11) + 1) * two53, u2 = (double)(c[1] >> 11) * two53;
   double u3 = (double)((c[2] >> 11) + 1) * two53, u4 = (double)(c[3] >> 11) * two53;
-  double rad1 = sqrt(-2.0 * log(u1)), rad2 = sqrt(-2.0 * log(u3));
-  const double twopi = 6.283185307179586;
-  n[0] = rad1 * cos(twopi * u2);
-  n[1] = rad1 * sin(twopi * u2);
-  n[2] = rad2 * cos(twopi * u4);
+  const double rad1 = sqrt(-2.0 * rs_log(u1)), rad2 = sqrt(-2.0 * rs_log(u3));
+  double s2, c2, s4, c4;
+  rs_sincos2pi(u2, &s2, &c2);
+  rs_sincos2pi(u4, &s4, &c4);
+  n[0] = rad1 * c2;
+  n[1] = rad1 * s2;
+  n[2] = rad2 * c4;
 }
 
 // Word offset of atom a (0 = O) in a molecule's 20-word adjoint pair record: H1 at 0, H2 at 6,
@@ -220,6 +349,12 @@ __device__ __forceinline__ float chunk_sum(float v) {
 }
 template <int CH>
 __device__ __forceinline__ double chunk_sum(double v) {
+#pragma unroll
+  for (int o = CH / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, RS_WAVE);
+  return v;
+}
+template <int CH>
+__device__ __forceinline__ unsigned long long chunk_sum(unsigned long long v) {
 #pragma unroll
   for (int o = CH / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, RS_WAVE);
   return v;

@@ -39,7 +39,8 @@ struct DevBufs {
   double *rdfG, *target, *rdf_g;
   double* frames;  // MSD frames [nf * 3M]
   double* msd_del; // [K * M * 3]
-  long long* msd_acc; // [K] fixed point 2^40
+  double* msd_sq;     // |Δ|^2 per origin and molecule [K * M]
+  double* msd_part;   // their sums over chunks of 256 molecules [K * ceil(M/256)]
   double* msd_grad;   // dL/dframes [nf * 3M]
   double* ext_noise;  // [steps * 3N] or null
   double* hpart;      // h-norm partials

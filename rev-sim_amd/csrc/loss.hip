@@ -186,7 +186,12 @@ __global__ void k_frame_copy(SysParams p, const double* x, double* frames, int f
   for (int k = 0; k < 3; ++k) frames[(size_t)f * 3 * p.M + 3 * m + k] = x[9 * (size_t)m + k];
 }
 
-__global__ void k_msd_del(SysParams p, int K, const double* frames, double* del, long long* acc) {
+// Per origin s and molecule m, the unwrapped displacement Δ over K frame intervals and |Δ|^2 (no
+// contraction). The |Δ|^2 are then summed in a fixed order that the oracle follows too, so D agrees
+// bit for bit: in index order within chunks of MSD_CHUNK molecules (k_msd_chunks, one thread per
+// chunk and origin), then the chunk sums in order (k_msd_final).
+__global__ void k_msd_del(SysParams p, int K, const double* frames, double* del, double* sq_out) {
+#pragma clang fp contract(off)
   int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= p.M) return;
   const size_t F = 3 * (size_t)p.M;
@@ -200,18 +205,42 @@ __global__ void k_msd_del(SysParams p, int K, const double* frames, double* del,
       del[((size_t)s * p.M + m) * 3 + k] = a[k];
       sq += a[k] * a[k];
     }
-    atomicAdd((unsigned long long*)&acc[s], (unsigned long long)__double2ll_rn(sq * 1099511627776.0));
+    sq_out[(size_t)s * p.M + m] = sq;
   }
 }
 
+constexpr int MSD_CHUNK = 256;
+
+__global__ void k_msd_chunks(SysParams p, int K, const double* __restrict__ sq, double* part) {
+  const int nch = (p.M + MSD_CHUNK - 1) / MSD_CHUNK;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;  // (origin, chunk)
+  if (t >= K * nch) return;
+  const int s = t / nch, c = t % nch;
+  const double* q = sq + (size_t)s * p.M;
+  const int m1 = min(p.M, (c + 1) * MSD_CHUNK);
+  double a = 0.0;
+#pragma unroll 8
+  for (int m = c * MSD_CHUNK; m < m1; ++m) a += q[m];
+  part[t] = a;
+}
+
+// D = Σ_s (Σ_chunks (Σ_m |Δ_sm|^2) / M) / lag / (6 K): one thread per origin sums its chunks in
+// order (once per trajectory), thread 0 the origins in order.
 __global__ void k_msd_final(SysParams p, int K, double lag, double w, double target,
-                            const long long* acc, double* scal, double* loss_dst) {
+                            const double* part, double* scal, double* loss_dst) {
+#pragma clang fp contract(off)
+  __shared__ double msd_s[64];
+  const int nch = (p.M + MSD_CHUNK - 1) / MSD_CHUNK;
+  if ((int)threadIdx.x < K) {
+    double msd = 0.0;
+    const double* q = part + (size_t)threadIdx.x * nch;
+    for (int c = 0; c < nch; ++c) msd += q[c];
+    msd_s[threadIdx.x] = msd / p.M;
+  }
+  __syncthreads();
   if (threadIdx.x != 0) return;
   double Dsum = 0.0;
-  for (int s = 0; s < K; ++s) {
-    double msd = ((double)acc[s] / 1099511627776.0) / p.M;
-    Dsum += msd / lag;
-  }
+  for (int s = 0; s < K; ++s) Dsum += msd_s[s] / lag;
   double D = Dsum / (6.0 * K);
   double diff = 1e-6 * D - target;
   double L = w * 1e18 * diff * diff;
@@ -277,9 +306,10 @@ void launch_frame_copy(const Launch& L, int f) {
 void launch_msd(const Launch& L, int nf, double lag, double w_msd, double target, double* loss_dst) {
   const SysParams& p = L.p;
   const int K = nf / 2;
-  k_zero_ll<<<1, 64, 0, L.s>>>(L.b.msd_acc, K);
-  k_msd_del<<<(p.M + RS_BLOCK - 1) / RS_BLOCK, RS_BLOCK, 0, L.s>>>(p, K, L.b.frames, L.b.msd_del, L.b.msd_acc);
-  k_msd_final<<<1, 64, 0, L.s>>>(p, K, lag, w_msd, target, L.b.msd_acc, L.b.scal, loss_dst);
+  k_msd_del<<<(p.M + RS_BLOCK - 1) / RS_BLOCK, RS_BLOCK, 0, L.s>>>(p, K, L.b.frames, L.b.msd_del, L.b.msd_sq);
+  const int nch = (p.M + MSD_CHUNK - 1) / MSD_CHUNK;
+  k_msd_chunks<<<(K * nch + RS_BLOCK - 1) / RS_BLOCK, RS_BLOCK, 0, L.s>>>(p, K, L.b.msd_sq, L.b.msd_part);
+  k_msd_final<<<1, 64, 0, L.s>>>(p, K, lag, w_msd, target, L.b.msd_part, L.b.scal, loss_dst);
   size_t tot = (size_t)nf * 3 * p.M;
   k_msd_grad<<<(unsigned)((tot + RS_BLOCK - 1) / RS_BLOCK), RS_BLOCK, 0, L.s>>>(p, K, L.b.msd_del, L.b.scal, L.b.msd_grad);
 }

@@ -87,6 +87,21 @@ __device__ __forceinline__ unsigned long long fix_rn(double v) {
   return (unsigned long long)(__double_as_longlong(v + 6755399441055744.0) - 0x4338000000000000LL);
 }
 
+// FP64 pair forces are summed as 64-bit integers in units of 2^-33 kJ/mol/nm, so a molecule's
+// force does not depend on the order its pair terms arrive in (lanes, list order, list layout) and
+// equals, bit for bit, what the oracle's sum in any order gives. A term c = (f/r) d_t (one IEEE
+// product) enters as round(c 2^33): fma(c, 2^33, 1.5 2^52) leaves that integer plus the bias
+// 0x4338000000000000 in its bit pattern (exact for |c| < 2^18; the kernel flags larger terms), so
+// a lane adds the bit patterns and removes the bias once per term at the end.
+constexpr double FX_SCALE = 8589934592.0;        // 2^33
+constexpr double FX_INV = 1.0 / 8589934592.0;
+constexpr double FX_CMAX = 262144.0;             // 2^18: |c| at which the bias trick stops being exact
+constexpr unsigned long long FX_BIAS = 0x4338000000000000ULL;
+__device__ __forceinline__ unsigned long long fx_term(double ftot, double d) {
+  const double c = ftot * d;  // rounded on its own: it is an operand of the fma, not an addend
+  return (unsigned long long)__double_as_longlong(__builtin_fma(c, FX_SCALE, 6755399441055744.0));
+}
+
 struct P32 {
   typedef int4 Pos;
   typedef float4 U;
@@ -224,6 +239,9 @@ void k_pair(PairArgs A, StepArgs St) {
   Pos own[3];
   R uo[3][3];
   R F[3][3], hh[3][3], Flj[3], Fs[3], Fml[3];
+  // FP64 path: the pair forces as fixed-point integer sums (fx_term), F unused
+  constexpr bool FX = sizeof(R) == 8;
+  unsigned long long Fq[3][3];
   gather(act ? i : 0, 0, own, uo);
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
@@ -231,6 +249,7 @@ void k_pair(PairArgs A, StepArgs St) {
     for (int k = 0; k < 3; ++k) {
       F[a][k] = 0;
       hh[a][k] = 0;
+      Fq[a][k] = 0;
     }
     Flj[a] = 0;
     Fs[a] = 0;
@@ -316,8 +335,11 @@ void k_pair(PairArgs A, StepArgs St) {
           const R ri2 = ri * ri, ri3 = ri2 * ri;
           const int ty = (a > 0) + (b > 0);
           const R fr = P::fma_(kA[ty], ri3, in ? -kB[ty] : (R)0);
+          R ftot = fr;  // f/r of all the pair's terms (FP64 path)
+          if constexpr (!FX) {
 #pragma unroll
-          for (int t = 0; t < 3; ++t) F[a][t] -= fr * d[t];
+            for (int t = 0; t < 3; ++t) F[a][t] -= fr * d[t];
+          }
           R du[3], dot = 0;
           if (ADJ) {
 #pragma unroll
@@ -330,9 +352,13 @@ void k_pair(PairArgs A, StepArgs St) {
           }
           if (S == 0 && a == 0 && b == 0) {
             const R s2 = sig2 * ri2, s6 = s2 * s2 * s2, s12 = s6 * s6;
-            const R frl = eps24 * ri2 * ((R)2 * s12 - s6);
+            const R frl = (eps24 * ri2) * P::fma_((R)2, s12, -s6);
+            if constexpr (FX) {
+              ftot = add_nc(ftot, frl);
+            } else {
 #pragma unroll
-            for (int t = 0; t < 3; ++t) F[0][t] -= frl * d[t];
+              for (int t = 0; t < 3; ++t) F[0][t] -= frl * d[t];
+            }
             if (ADJ) {
               const R gdl = -eps24 * ri2 * ri2 * ((R)28 * s12 - (R)8 * s6) * dot;
               const R fs = epss * ri2 * ((R)4 * s12 - s6);
@@ -367,8 +393,12 @@ void k_pair(PairArgs A, StepArgs St) {
               const R f = hb[0] * fk0 + hb[1] * mh * dk0 + hb[2] * fk1 + hb[3] * mh * dk1;
               const R fp = (hdb[0] * fk0 + hdb[1] * mh * dk0 + hdb[2] * fk1 + hdb[3] * mh * dk1) * minvh;
               const R frm = f * ri;
+              if constexpr (FX) {
+                ftot = add_nc(ftot, frm);
+              } else {
 #pragma unroll
-              for (int t = 0; t < 3; ++t) F[0][t] -= frm * d[t];
+                for (int t = 0; t < 3; ++t) F[0][t] -= frm * d[t];
+              }
               if (ADJ) {
                 const R g2m = (fp * lu - f * ri) * ri2;
 #pragma unroll
@@ -387,6 +417,12 @@ void k_pair(PairArgs A, StepArgs St) {
               }
             }
           }
+          if constexpr (FX) {
+#pragma unroll
+            for (int t = 0; t < 3; ++t) Fq[a][t] += fx_term(ftot, d[t]);
+            // (flagged where it happens: a flag carried through the loop costs registers)
+            if (fabs(ftot) * (r2 * ri) >= FX_CMAX) atomicOr(&A.flags[FLAG_FIXED_RANGE], 1);
+          }
         }
       }
     }
@@ -395,13 +431,22 @@ void k_pair(PairArgs A, StepArgs St) {
   shell(std::integral_constant<int, 1>());
   shell(std::integral_constant<int, 2>());
 
-  // combine the CH lanes of the molecule (fixed xor tree, FP64)
+  // combine the CH lanes of the molecule (fixed xor tree, FP64; integer sums in the FP64 path,
+  // each lane's bias removed first: one per term it added, i.e. per evaluated atom pair)
   double Fd[9], hd[9], Fljd[3], Fsd[3], Fmld[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
+    // terms per own atom: shell 0 all three atoms of j, shell 1 no O-O, shell 2 H-H only
+    const unsigned long long nterm =
+        3ull * (unsigned)ncnt.x + (a == 0 ? 2ull : 3ull) * (unsigned)ncnt.y + (a == 0 ? 0ull : 2ull) * (unsigned)ncnt.z;
 #pragma unroll
     for (int t = 0; t < 3; ++t) {
-      Fd[3 * a + t] = chunk_sum<CH>((double)F[a][t]);
+      if constexpr (FX) {
+        const long long q = (long long)chunk_sum<CH>(Fq[a][t] - nterm * FX_BIAS);
+        Fd[3 * a + t] = -((double)q * FX_INV);
+      } else {
+        Fd[3 * a + t] = chunk_sum<CH>((double)F[a][t]);
+      }
       if (ADJ) hd[3 * a + t] = chunk_sum<CH>((double)hh[a][t]);
     }
     if (ADJ) {

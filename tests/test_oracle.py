@@ -154,6 +154,7 @@ def test_theta_vjp_matches_finite_differences(jittered216):
     import revsim
 
     o = _orc()
+    o.set_float_sums()  # steps of 1e-6 resolve below the fixed-point unit of the force sums
     rng = np.random.default_rng(5)
     w = rng.standard_normal(jittered216.size)
     _, _, vj = o.adjoint_terms(jittered216, np.zeros_like(w), w)
@@ -448,3 +449,76 @@ def test_free_particle_closed_form(oracle_mod, box216):
     d -= 1.862 * np.round(d / 1.862)
     assert np.abs(d).max() < 1e-12
     assert np.abs(v - vc).max() < 1e-13
+
+
+def test_deterministic_elementary_functions_accuracy(oracle_mod):
+    """rs_log, rs_sincos2pi and rs_acos (IEEE operations and fixed polynomials, shared bit for bit
+    with the GPU library) against the C library over their ranges: within a few units in the last
+    place, and exact at the special points (log 1 = 0, sincos at quarter turns, acos(±1), acos 0)."""
+    import ctypes
+
+    L = oracle_mod.lib()
+    rng = np.random.default_rng(9)
+    x = np.concatenate([rng.random(20000) + 1e-300, 2.0 ** rng.uniform(-60, 10, 20000)])
+    lg = np.array([L.or_log(float(v)) for v in x])
+    ref = np.log(x.astype(np.longdouble))
+    err = np.abs(lg - ref).astype(np.float64)
+    assert np.all(err <= 3.0 * np.spacing(np.abs(ref.astype(np.float64))) + 1e-300)
+    assert L.or_log(1.0) == 0.0
+    sn, cs = ctypes.c_double(), ctypes.c_double()
+    u = rng.random(20000)
+    got = []
+    for v in u:
+        L.or_sincos2pi(float(v), ctypes.byref(sn), ctypes.byref(cs))
+        got.append((sn.value, cs.value))
+    got = np.array(got)
+    # reference in extended precision (2πu itself rounds in double)
+    ang = 2 * np.pi * u.astype(np.longdouble)
+    ang += 2 * (np.longdouble("3.14159265358979323846264338327950288") - np.longdouble(np.pi)) * u
+    assert np.max(np.abs(got[:, 0] - np.sin(ang).astype(np.float64))) < 3.5e-16
+    assert np.max(np.abs(got[:, 1] - np.cos(ang).astype(np.float64))) < 3.5e-16
+    for v, (s0, c0) in ((0.0, (0.0, 1.0)), (0.25, (1.0, 0.0)), (0.5, (0.0, -1.0)), (0.75, (-1.0, 0.0))):
+        L.or_sincos2pi(v, ctypes.byref(sn), ctypes.byref(cs))
+        assert (sn.value, cs.value) == (s0, c0)
+    xa = np.concatenate([rng.uniform(-1, 1, 20000), [-1.0, -0.5, 0.0, 0.5, 1.0]])
+    ac = np.array([L.or_acos(float(v)) for v in xa])
+    acr = np.arccos(xa.astype(np.longdouble))
+    assert np.all(np.abs(ac - acr).astype(np.float64) <= 3.0 * np.spacing(acr.astype(np.float64)))
+    assert L.or_acos(1.0) == 0.0 and L.or_acos(-1.0) == np.pi and L.or_acos(0.0) == np.pi / 2
+
+
+@pytest.mark.parametrize("n,box,rc", [(216, 1.862, 0.8), (3000, 4.48, 1.0)])
+def test_fixed_point_force_sums(n, box, rc):
+    """The pair forces are summed as 64-bit integers of 2^-33 kJ/mol/nm: each term rounds by at
+    most 2^-34, so the exact-integer forces differ from the same terms summed in FP64 by no more
+    than (terms per atom) 2^-34 plus FP64 rounding, and they do not depend on the thread count (the
+    integer sum is exact in any order), while FP64 sums do."""
+    import subprocess
+    import sys
+
+    import revsim
+
+    wb = revsim.water_box(n, box)
+    rng = np.random.default_rng(3)
+    x = np.ascontiguousarray(wb.coords + 0.005 * rng.standard_normal(wb.coords.size))
+    o = _orc(n, box, rc)
+    F = o.forces(x)
+    o.set_float_sums()
+    Ff = o.forces(x)
+    nterms = 2 * 3 * 4.0 / 3.0 * np.pi * (rc + 0.2) ** 3 * (3 * n / box ** 3)
+    assert np.abs(F - Ff).max() <= nterms * 2.0 ** -34 + 1e-12 * np.abs(Ff).max()
+    assert np.abs(F - Ff).max() > 0  # the two sums do differ
+    code = (
+        "import sys, numpy as np; sys.path[:0] = %r\n"
+        "import revsim, oracle as O\n"
+        "o = O.Oracle(revsim.make_config(%d, %r, %r)); o.set_theta(revsim.DEFAULT_THETA)\n"
+        "x = np.frombuffer(sys.stdin.buffer.read(), dtype=np.float64).copy()\n"
+        "sys.stdout.buffer.write(o.forces(x).tobytes())\n" % (sys.path[:4], n, box, rc))
+    import os
+
+    out = {}
+    for th in ("1", "3"):
+        r = subprocess.run([sys.executable, "-c", code], input=x.tobytes(), capture_output=True,
+                           env=dict(os.environ, OMP_NUM_THREADS=th), check=True)
+        out[th] = np.frombuffer(r.stdout, dtype=np.float64)
+    assert np.array_equal(out["1"], out["3"]) and np.array_equal(out["1"], F)

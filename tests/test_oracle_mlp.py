@@ -71,6 +71,7 @@ def test_bf16_weights():
 
 def test_mlp_system_forces_hvp_and_weight_vjp(jittered216, box216):
     o, W = _mlp_only()
+    o.set_float_sums()  # steps of 1e-6 resolve below the fixed-point unit of the force sums
     x = jittered216
     F, _ = o.forces(x, with_energy=True)
     rng = np.random.default_rng(1)
