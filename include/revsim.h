@@ -196,6 +196,16 @@ int rs_rdf(rs_ctx* ctx, const double* x, double* g, int64_t* hist_fixed, double*
    dL/dframes (same layout). */
 int rs_msd(rs_ctx* ctx, const double* frames, int32_t n_frames, double* D, double* loss,
            double* dL_dframes);
+/* rs_msd for callers that cannot allocate device memory (code under HIP graph capture, tight
+   serving loops, harnesses that forbid hipMalloc); no counterpart in the reference. frames and
+   dL_dframes are device arrays (layout as rs_msd); D and loss are host scalars. The call allocates
+   nothing: its kernels' scratch is a static device array of 33.6 MB that each GPU holds from the
+   library's load until the process ends, which limits a call to n_frames/2 * n_molecules <=
+   1,048,576 (RS_E_CONFIG beyond). All calls on one GPU share that array: calls that overlap in
+   time on one GPU (other host threads, other contexts) race. Synchronous on the context's stream;
+   results equal rs_msd's bit for bit. */
+int rs_msd_noalloc(rs_ctx* ctx, const double* frames, int32_t n_frames, double* D, double* loss,
+                   double* dL_dframes);
 /* Potential energy U(x) (kJ/mol: reaction field with its r_c shift, Lennard-Jones, bonds, angles),
    its gradient dU/dtheta (6, nullable) and the forces -dU/dx (3N, nullable), in FP64 in both
    precisions. Classical force field only (RS_E_CONFIG with use_mlp). */

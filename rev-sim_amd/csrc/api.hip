@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -812,6 +813,34 @@ int check_ctx(rs_ctx* c) {
   return 0;
 }
 
+// ---- scratch of rs_msd_noalloc: a file-scope device array (each GPU holds its own copy from the
+// library's load until the process ends). Host code never uses the array's name as an address: at
+// set-up each device's copy is looked up once (hipGetSymbolAddress with that device current) and
+// that pointer is what kernels and copies get.
+constexpr size_t NA_KM = (size_t)1 << 20;         // max origins x molecules of one call
+constexpr size_t NA_PART = NA_KM / 256 + 64;      // chunk sums: K ceil(M/256) <= K M/256 + K
+constexpr size_t NA_DOUBLES = 3 * NA_KM + NA_KM + NA_PART + 16;  // del, |del|^2, chunk sums, scalars
+static __device__ __align__(256) double scratch[NA_DOUBLES];
+constexpr int NA_MAX_DEVICES = 64;
+double* g_scratch[NA_MAX_DEVICES] = {};
+std::mutex g_scratch_mu;
+
+// The device's address of `scratch`; the context's device must be current.
+int scratch_ptr(rs_ctx* c, double** out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (c->device < 0 || c->device >= NA_MAX_DEVICES)
+    return fail(c, RS_E_CONFIG, "rs_msd_noalloc supports device indices below 64");
+  if (!g_scratch[c->device]) {
+    double* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(c, RS_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch[c->device] = ptr;
+  }
+  *out = g_scratch[c->device];
+  return 0;
+}
+
 // check_ctx for the calls that run device work: also clears the per-call counters of rs_stats.
 int begin_call(rs_ctx* c) {
   int r = check_ctx(c);
@@ -920,6 +949,12 @@ int rs_create(const rs_config* cfg, int32_t device, rs_ctx** out) {
     return fail(nullptr, RS_E_CONFIG, "rdf_rmax + rdf_window must not exceed the list radius");
   }
   c->L.p.capc = default_capc(c);
+  double* sp = nullptr;
+  if (scratch_ptr(c, &sp)) {  // set-up of rs_msd_noalloc's scratch on this device
+    std::string m = c->err;
+    rs_destroy(c);
+    return fail(nullptr, RS_E_HIP, m);
+  }
   if (alloc_all(c)) {
     std::string m = c->err;
     rs_destroy(c);
@@ -1237,6 +1272,29 @@ int rs_msd(rs_ctx* c, const double* frames, int32_t nf, double* D, double* loss,
   if (D) *D = sc[3];
   if (loss) *loss = sc[2];
   if (dLdf) HIPCHK(c, hipMemcpy(dLdf, b.msd_grad, sizeof(double) * n, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int rs_msd_noalloc(rs_ctx* c, const double* frames, int32_t nf, double* D, double* loss, double* dLdf) {
+  int r = check_ctx(c);
+  if (r) return r;
+  if (!frames || !dLdf || nf < 2 || nf % 2 || nf / 2 > 64)
+    return fail(c, RS_E_CONFIG, "frames and dL_dframes must be device arrays of an even number of frames in [2, 128]");
+  if (c->cfg.msd_every < 1) return fail(c, RS_E_CONFIG, "msd_every must be >= 1");
+  if ((size_t)(nf / 2) * (size_t)c->M > NA_KM)
+    return fail(c, RS_E_CONFIG, "rs_msd_noalloc: n_frames/2 * n_molecules exceeds 1,048,576 (its static scratch)");
+  double* s = nullptr;
+  if ((r = scratch_ptr(c, &s))) return r;
+  double *del = s, *sq = s + 3 * NA_KM, *part = sq + NA_KM, *scal = part + NA_PART;
+  // no buffer needs clearing: every element the kernels read was written by them in this call
+  launch_msd_bufs(c->L, nf, (nf / 2) * c->cfg.msd_every * c->cfg.dt, c->cfg.w_msd, c->cfg.msd_target, frames,
+                  del, sq, part, scal, dLdf, nullptr);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->s));
+  double sc[4];
+  HIPCHK(c, hipMemcpy(sc, scal, sizeof(sc), hipMemcpyDeviceToHost));
+  if (D) *D = sc[3];
+  if (loss) *loss = sc[2];
   return 0;
 }
 

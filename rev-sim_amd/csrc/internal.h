@@ -94,6 +94,10 @@ void launch_rdf(const Launch& L, int nb, double rmax, double width, double windo
                 bool grad, double* loss_dst);
 void launch_frame_copy(const Launch& L, int f);
 void launch_msd(const Launch& L, int nf, double lag, double w_msd, double target, double* loss_dst);
+// The same kernels on caller-given buffers: frames [nf 3M], del [K 3M], sq [K M],
+// part [K ceil(M/256)], scal [>= 6] (writes [2] loss, [3] D, [5] gradient factor), grad [nf 3M].
+void launch_msd_bufs(const Launch& L, int nf, double lag, double w_msd, double target, const double* frames,
+                     double* del, double* sq, double* part, double* scal, double* grad, double* loss_dst);
 void launch_msd_inject(const Launch& L, int f);
 void launch_zero(const Launch& L, double* a, size_t n);
 int pair_blocks(const SysParams& p);

@@ -303,15 +303,21 @@ void launch_frame_copy(const Launch& L, int f) {
   k_frame_copy<<<(L.p.M + RS_BLOCK - 1) / RS_BLOCK, RS_BLOCK, 0, L.s>>>(L.p, L.b.x, L.b.frames, f);
 }
 
-void launch_msd(const Launch& L, int nf, double lag, double w_msd, double target, double* loss_dst) {
+void launch_msd_bufs(const Launch& L, int nf, double lag, double w_msd, double target, const double* frames,
+                     double* del, double* sq, double* part, double* scal, double* grad, double* loss_dst) {
   const SysParams& p = L.p;
   const int K = nf / 2;
-  k_msd_del<<<(p.M + RS_BLOCK - 1) / RS_BLOCK, RS_BLOCK, 0, L.s>>>(p, K, L.b.frames, L.b.msd_del, L.b.msd_sq);
+  k_msd_del<<<(p.M + RS_BLOCK - 1) / RS_BLOCK, RS_BLOCK, 0, L.s>>>(p, K, frames, del, sq);
   const int nch = (p.M + MSD_CHUNK - 1) / MSD_CHUNK;
-  k_msd_chunks<<<(K * nch + RS_BLOCK - 1) / RS_BLOCK, RS_BLOCK, 0, L.s>>>(p, K, L.b.msd_sq, L.b.msd_part);
-  k_msd_final<<<1, 64, 0, L.s>>>(p, K, lag, w_msd, target, L.b.msd_part, L.b.scal, loss_dst);
+  k_msd_chunks<<<(K * nch + RS_BLOCK - 1) / RS_BLOCK, RS_BLOCK, 0, L.s>>>(p, K, sq, part);
+  k_msd_final<<<1, 64, 0, L.s>>>(p, K, lag, w_msd, target, part, scal, loss_dst);
   size_t tot = (size_t)nf * 3 * p.M;
-  k_msd_grad<<<(unsigned)((tot + RS_BLOCK - 1) / RS_BLOCK), RS_BLOCK, 0, L.s>>>(p, K, L.b.msd_del, L.b.scal, L.b.msd_grad);
+  k_msd_grad<<<(unsigned)((tot + RS_BLOCK - 1) / RS_BLOCK), RS_BLOCK, 0, L.s>>>(p, K, del, scal, grad);
+}
+
+void launch_msd(const Launch& L, int nf, double lag, double w_msd, double target, double* loss_dst) {
+  launch_msd_bufs(L, nf, lag, w_msd, target, L.b.frames, L.b.msd_del, L.b.msd_sq, L.b.msd_part, L.b.scal,
+                  L.b.msd_grad, loss_dst);
 }
 
 void launch_msd_inject(const Launch& L, int f) {

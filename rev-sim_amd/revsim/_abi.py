@@ -175,6 +175,7 @@ SIGNATURES = [
     ("rs_noise", ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.c_int32, _dp]),
     ("rs_rdf", ctypes.c_int, [_vp, _dp, _dp, _lp, _dp, _dp]),
     ("rs_msd", ctypes.c_int, [_vp, _dp, ctypes.c_int32, _dp, _dp, _dp]),
+    ("rs_msd_noalloc", ctypes.c_int, [_vp, _vp, ctypes.c_int32, _dp, _dp, _vp]),
     ("rs_molecule_pairs", ctypes.c_int, [_vp, _dp, _ip, ctypes.c_int64, _lp]),
     ("rs_list_radius", ctypes.c_double, [_vp]),
     ("rs_loss_terms", ctypes.c_int, [_vp, _dp]),
