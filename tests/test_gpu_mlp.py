@@ -93,3 +93,40 @@ def test_mlp_gradient_is_deterministic(gpu_lib, box216):
     L2, g2 = sim.value_and_grad(box216.coords, box216.velocities, 13, 20)
     d2 = sim.mlp_grad()
     assert L1 == L2 and np.array_equal(g1, g2) and np.array_equal(d1, d2)
+
+
+_ORACLE_CACHE = {}
+
+
+@pytest.mark.timeout(1200)
+@pytest.mark.parametrize("prec,tol", [(FP64, 1e-8), (FP32, 5e-5)])
+def test_config4_98k_mlp_first_100_steps(gpu_lib, prec, tol):
+    """Learned correction at scale: the 98,304-atom box with the MLP term and the RDF + MSD loss,
+    against the FP64 oracle (exact per-pair network) on the first 100 steps. The FP64 path's table
+    is within ~1e-11 of the network, which chaos grows ~100x over 100 steps; the FP32 path's
+    rounding grows the same way (1.3e-5 at 100 steps without the MLP, 7e-8 at 20), hence the
+    bounds. The MLP changes dL/dθ against the plain force field by far more than either."""
+    import oracle as O
+    import revsim
+    from revsim.system import mlp_init
+
+    wb = revsim.water_box(32768, 9.95)
+    kw = dict(msd_window_steps=50, msd_every=25, w_msd=1.0)
+    sim, orc = _pair(prec, n=32768, box=9.95, rc=1.0, **kw)
+    tgt = revsim.perturbed_target(orc.rdf(wb.coords)[0])
+    sim.set_target_rdf(tgt)
+    if "c4" not in _ORACLE_CACHE:  # one oracle run (~2 minutes) serves both precisions
+        orc_f = O.Oracle(revsim.make_config(32768, 9.95, 1.0, precision=FP64, use_mlp=1, **kw))
+        orc_f.set_theta(revsim.DEFAULT_THETA)
+        orc_f.set_mlp(mlp_init())
+        orc_f.set_target_rdf(tgt)
+        _ORACLE_CACHE["c4"] = orc_f.value_and_grad(wb.coords, wb.velocities, 13, 100)
+    Lo, go = _ORACLE_CACHE["c4"]
+    L, g = sim.value_and_grad(wb.coords, wb.velocities, 13, 100)
+    assert abs(L - Lo) <= tol * abs(Lo)
+    assert rel_err(g, go) < tol, (rel_err(g, go), g, go)
+    plain = revsim.Simulator(revsim.make_config(32768, 9.95, 1.0, precision=prec, **kw), device=0)
+    plain.set_theta(revsim.DEFAULT_THETA)
+    plain.set_target_rdf(tgt)
+    _, gp = plain.value_and_grad(wb.coords, wb.velocities, 13, 100)
+    assert rel_err(gp, go) > 100 * tol
