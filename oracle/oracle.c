@@ -867,6 +867,20 @@ static inline double rs_acos(double x) {
   return pio2_hi - (rs_asin_series(x) - pio2_lo);
 }
 
+/* 1/√x (x > 0 normal) as the GPU library's FP64 pair terms compute it: the seed
+   0x5FE6EB50C7B537A9 - (bits >> 1) (Lomont's / Robertson's 64-bit constant; relative error below
+   3.5 %), then four Newton steps y ← y (3/2 - (x/2) y²); within 2 units in the last place. */
+static inline double rs_rsqrt(double x) {
+  int64_t b;
+  memcpy(&b, &x, 8);
+  b = 0x5FE6EB50C7B537A9LL - (b >> 1);
+  double y;
+  memcpy(&y, &b, 8);
+  const double hx = 0.5 * x;
+  for (int k = 0; k < 4; ++k) y = y * fma(-hx, y * y, 1.5);
+  return y;
+}
+
 
 /* ---------------------------------------------------------------- bonded forces, library order */
 
@@ -1054,7 +1068,7 @@ static void eval_terms(or_ctx* o, const double* x, const double* c, const double
            F_a -= (f/r) d, one term round((f/r) d_t 2^33) per component summed as 64-bit integers,
            so the sum is exact and independent of pair and thread order */
         const int ty = (!aO) + (!bO);
-        const double ri = 1.0 / sqrt(r2), ri2 = ri * ri, ri3 = ri2 * ri;
+        const double ri = rs_rsqrt(r2), ri2 = ri * ri, ri3 = ri2 * ri;
         double ftot = fma(kqq[ty], ri3, -kBq[ty]);
         if (aO && bO) {
           const double s2 = sig2c * ri2, s6 = s2 * s2 * s2, s12 = s6 * s6;
@@ -2094,6 +2108,7 @@ int or_set_float_sums(or_ctx* o, int on) {
 double or_log(double x) { return rs_log(x); }
 void or_sincos2pi(double u, double* sn, double* cs) { rs_sincos2pi(u, sn, cs); }
 double or_acos(double x) { return rs_acos(x); }
+double or_rsqrt(double x) { return rs_rsqrt(x); }
 
 /* (l_rdf, l_ev, l_msd, D in m^2/s) of the last forward / reverse call, as rs_loss_terms. */
 int or_loss_terms(const or_ctx* o, double* t) {

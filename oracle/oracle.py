@@ -58,6 +58,7 @@ _SIGS = [
     ("or_log", ctypes.c_double, [ctypes.c_double]),
     ("or_sincos2pi", None, [ctypes.c_double, _dp, _dp]),
     ("or_acos", ctypes.c_double, [ctypes.c_double]),
+    ("or_rsqrt", ctypes.c_double, [ctypes.c_double]),
     ("or_philox4x64", None, [_P(ctypes.c_uint64), _P(ctypes.c_uint64), _P(ctypes.c_uint64)]),
 ]
 

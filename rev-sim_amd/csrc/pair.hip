@@ -90,16 +90,17 @@ __device__ __forceinline__ unsigned long long fix_rn(double v) {
 // FP64 pair forces are summed as 64-bit integers in units of 2^-33 kJ/mol/nm, so a molecule's
 // force does not depend on the order its pair terms arrive in (lanes, list order, list layout) and
 // equals, bit for bit, what the oracle's sum in any order gives. A term c = (f/r) d_t (one IEEE
-// product) enters as round(c 2^33): fma(c, 2^33, 1.5 2^52) leaves that integer plus the bias
-// 0x4338000000000000 in its bit pattern (exact for |c| < 2^18; the kernel flags larger terms), so
-// a lane adds the bit patterns and removes the bias once per term at the end.
+// product) enters as round(c 2^33). The kernel scales f/r by 2^33 once per pair (exact), so
+// (f/r 2^33) d_t is c 2^33 with c's rounding; adding 1.5 2^52 (not fused) rounds it to an integer
+// and leaves that integer plus the bias 0x4338000000000000 in the bit pattern (exact for
+// |c 2^33| < 2^51; the kernel flags larger terms), so a lane adds the bit patterns and removes the
+// bias once per term at the end.
 constexpr double FX_SCALE = 8589934592.0;        // 2^33
 constexpr double FX_INV = 1.0 / 8589934592.0;
-constexpr double FX_CMAX = 262144.0;             // 2^18: |c| at which the bias trick stops being exact
+constexpr double FX_CMAX2 = 2251799813685248.0;  // 2^51: |c 2^33| at which the bias trick stops being exact
 constexpr unsigned long long FX_BIAS = 0x4338000000000000ULL;
-__device__ __forceinline__ unsigned long long fx_term(double ftot, double d) {
-  const double c = ftot * d;  // rounded on its own: it is an operand of the fma, not an addend
-  return (unsigned long long)__double_as_longlong(__builtin_fma(c, FX_SCALE, 6755399441055744.0));
+__device__ __forceinline__ unsigned long long fx_term(double ftot2, double d) {
+  return (unsigned long long)__double_as_longlong(add_nc(ftot2 * d, 6755399441055744.0));
 }
 
 struct P32 {
@@ -126,27 +127,34 @@ struct P64 {
   typedef double R;
   static __device__ __forceinline__ Pos ld_pos(const void* p, int slot) { return ((const double4*)p)[slot]; }
   static __device__ __forceinline__ U ld_u(const void* p, int slot) { return ((const double4*)p)[slot]; }
+  // mic1 with the multiply fused: the sorted positions lie in [0, L], so rint(d/L) is -1, 0 or 1,
+  // L k is exact and fma(-L, k, d) rounds d - L k once, as mic1's separate multiply and subtract do
+  static __device__ __forceinline__ double mic_f(double d, double L, double invL) {
+    return __builtin_fma(-L, rint(d * invL), d);
+  }
   static __device__ __forceinline__ void diff(const SysParams& p, const double4& a, const double4& b,
                                               double& dx, double& dy, double& dz) {
-    dx = mic1(b.x - a.x, p.L[0], p.invL[0]);
-    dy = mic1(b.y - a.y, p.L[1], p.invL[1]);
-    dz = mic1(b.z - a.z, p.L[2], p.invL[2]);
+    dx = mic_f(add_nc(b.x, -a.x), p.L[0], p.invL[0]);
+    dy = mic_f(add_nc(b.y, -a.y), p.L[1], p.invL[1]);
+    dz = mic_f(add_nc(b.z, -a.z), p.L[2], p.invL[2]);
   }
   static __device__ __forceinline__ double r2(double dx, double dy, double dz) { return r2_of(dx, dy, dz); }
-  static __device__ __forceinline__ double rinv(double r2) { return 1.0 / sqrt(r2); }
+  static __device__ __forceinline__ double rinv(double r2) { return rs_rsqrt(r2); }
   static __device__ __forceinline__ double fma_(double a, double b, double c) { return __builtin_fma(a, b, c); }
 };
 
 // ------------------------------------------------------------------ the pair kernel
 
-// Occupancy targets of the FP32 variants (the others are left to the compiler). The forward loop
-// gathers one entry ahead: without the MLP term in 64 VGPRs, 8 waves per SIMD, so its 8,192 waves
-// at 98k atoms run in one round (2 % faster than 7 waves); with it in 72 VGPRs, 7 waves (at 8 it
-// spills inside the loops). The adjoint fits 128 VGPRs, 4 waves (prefetching there needs 154
-// VGPRs, 3 waves, and ran 1 % slower).
+// Occupancy targets (waves per SIMD) of the 16-lane variants. FP32: the forward loop gathers one
+// entry ahead: without the MLP term in 64 VGPRs, 8 waves per SIMD, so its 8,192 waves at 98k atoms
+// run in one round (2 % faster than 7 waves); with it in 72 VGPRs, 7 waves (at 8 it spills inside
+// the loops). The adjoint fits 128 VGPRs, 4 waves (prefetching there needs 154 VGPRs, 3 waves, and
+// ran 1 % slower). FP64: the forward at 128 VGPRs, 4 waves (a few spills outside the loops) ran 2 %
+// faster than at 136 VGPRs, 3 waves; the adjoint needs 256 VGPRs, 2 waves (at 3 waves it spills
+// 150 VGPRs).
 template <class P, int CH, bool ADJ, bool FMLP>
 constexpr int pair_waves() {
-  return sizeof(typename P::R) == 4 && CH == 16 ? (ADJ ? 4 : (FMLP ? 7 : 8)) : 1;
+  return CH != 16 ? 1 : sizeof(typename P::R) == 4 ? (ADJ ? 4 : (FMLP ? 7 : 8)) : (ADJ ? 2 : 4);
 }
 
 // Whether the pair loop gathers the next entry's inputs one iteration ahead.
@@ -268,7 +276,8 @@ void k_pair(PairArgs A, StepArgs St) {
   __shared__ float2 stab[sizeof(R) == 4 ? MLP_NT : 1];
   R wO[3] = {0, 0, 0};
   double gscale = 0.0;
-  const bool mlp = ADJ ? (bool)A.mlp : FMLP;
+  // (the FP32 adjoint follows the runtime flag; every other variant has it compiled in)
+  const bool mlp = ADJ && sizeof(R) == 4 ? (bool)A.mlp : FMLP;
   if (mlp) {
     if (sizeof(R) == 4)
       for (int q = threadIdx.x; q < MLP_NT; q += RS_BLOCK) stab[q] = ((const float2*)A.mtab)[q];
@@ -418,10 +427,12 @@ void k_pair(PairArgs A, StepArgs St) {
             }
           }
           if constexpr (FX) {
+            const R ftot2 = ftot * (R)FX_SCALE;
 #pragma unroll
-            for (int t = 0; t < 3; ++t) Fq[a][t] += fx_term(ftot, d[t]);
-            // (flagged where it happens: a flag carried through the loop costs registers)
-            if (fabs(ftot) * (r2 * ri) >= FX_CMAX) atomicOr(&A.flags[FLAG_FIXED_RANGE], 1);
+            for (int t = 0; t < 3; ++t) Fq[a][t] += fx_term(ftot2, d[t]);
+            // |c| <= |f/r| r; NaN (coincident atoms) fails the test too. Flagged where it
+            // happens: a flag carried through the loop costs registers.
+            if (!(fabs(ftot2) * (r2 * ri) < (R)FX_CMAX2)) atomicOr(&A.flags[FLAG_FIXED_RANGE], 1);
           }
         }
       }
@@ -587,7 +598,8 @@ void dispatch_ch(const Launch& L, const PairArgs& a, const StepArgs* st) {
       return;
     }
   }
-  if (!ADJ && a.mlp) k_pair<P, CH, ADJ, !ADJ, false><<<g, RS_BLOCK, 0, L.s>>>(a, none);
+  constexpr bool F32ADJ = ADJ && sizeof(typename P::R) == 4;
+  if (a.mlp && !F32ADJ) k_pair<P, CH, ADJ, !F32ADJ, false><<<g, RS_BLOCK, 0, L.s>>>(a, none);
   else k_pair<P, CH, ADJ, false, false><<<g, RS_BLOCK, 0, L.s>>>(a, none);
 }
 

@@ -452,9 +452,10 @@ def test_free_particle_closed_form(oracle_mod, box216):
 
 
 def test_deterministic_elementary_functions_accuracy(oracle_mod):
-    """rs_log, rs_sincos2pi and rs_acos (IEEE operations and fixed polynomials, shared bit for bit
-    with the GPU library) against the C library over their ranges: within a few units in the last
-    place, and exact at the special points (log 1 = 0, sincos at quarter turns, acos(±1), acos 0)."""
+    """rs_log, rs_sincos2pi, rs_acos and rs_rsqrt (IEEE operations and fixed polynomials or
+    Newton steps, shared bit for bit with the GPU library) against extended-precision references
+    over their ranges: within a few units in the last place (rs_rsqrt: 2; measured 1.7), and exact
+    at the special points (log 1 = 0, sincos at quarter turns, acos(±1), acos 0, 1/√(4^k))."""
     import ctypes
 
     L = oracle_mod.lib()
@@ -485,6 +486,16 @@ def test_deterministic_elementary_functions_accuracy(oracle_mod):
     acr = np.arccos(xa.astype(np.longdouble))
     assert np.all(np.abs(ac - acr).astype(np.float64) <= 3.0 * np.spacing(acr.astype(np.float64)))
     assert L.or_acos(1.0) == 0.0 and L.or_acos(-1.0) == np.pi and L.or_acos(0.0) == np.pi / 2
+    # rs_rsqrt over squared pair distances (1e-4 .. 4 nm^2), a wide range of magnitudes, and the
+    # exact squares whose reciprocal square roots are powers of two
+    xr = np.concatenate([rng.uniform(1e-4, 4.0, 40000), 2.0 ** rng.uniform(-200, 200, 20000),
+                         4.0 ** np.arange(-20, 21)])
+    rr = np.array([L.or_rsqrt(float(v)) for v in xr])
+    rref = 1 / np.sqrt(xr.astype(np.longdouble))
+    ulp = np.spacing(rref.astype(np.float64))
+    assert np.all(np.abs(rr - rref).astype(np.float64) <= 2.0 * ulp)
+    assert np.all(rr[-41:] * np.sqrt(xr[-41:]) == 1.0)
+    assert np.isnan(L.or_rsqrt(0.0))
 
 
 @pytest.mark.parametrize("n,box,rc", [(216, 1.862, 0.8), (3000, 4.48, 1.0)])
