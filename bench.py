@@ -44,6 +44,8 @@ PEAK_HBM = 8000.0  # GB/s
 FLOP_FWD = (36, 15)
 FLOP_REV = (72, 33)
 ADJ_KERNEL = "k_pair<P32, 16, true, false, false>"
+ADJ_KERNEL64 = "k_pair<P64, 16, true, false, false>"
+PEAK_FP64 = 78.6  # TFLOP/s, MI355X vector FP64
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_water32768.json")
 
 
@@ -169,15 +171,15 @@ def cpu_model() -> str:
     return platform.processor() or "unknown"
 
 
-def traffic_from_profile(digest):
-    """Bytes per launch of the adjoint pair kernel from the committed counter run
+def traffic_from_profile(digest, kernel=ADJ_KERNEL):
+    """Bytes per launch of an adjoint pair kernel from the committed counter run
     (rev-sim_amd/measure_traffic.py), used only if it was taken on these kernel sources."""
     try:
         with open(TRAFFIC_FILE) as f:
             rec = json.load(f)
     except (OSError, ValueError):
         return None, {"file": None}
-    k = rec.get("kernels", {}).get(ADJ_KERNEL)
+    k = rec.get("kernels", {}).get(kernel)
     match = rec.get("sources_sha256") == digest
     src = {"file": os.path.relpath(TRAFFIC_FILE, ROOT), "sources_match": match,
            "command": rec.get("command"), "scale": rec.get("scale")}
@@ -337,6 +339,19 @@ def main():
             "phases_ms_one_trajectory": {k: round(vv["ms"], 3) for k, vv in tim64.items()},
             "note": "FP64 pair arithmetic, pair forces summed as exact 64-bit fixed point: the same "
                     "trajectory as the FP64 CPU oracle, bit for bit",
+        }
+        # roofline of its adjoint pair kernel against the FP64 vector peak
+        rp64 = tim64["rev_pair"]
+        avg64 = rp64["ms"] / max(rp64["launches"], 1)
+        ach64 = flop / (avg64 * 1e-3) / 1e12
+        tr64, tr64_src = (traffic_from_profile(digest, ADJ_KERNEL64) if a.workload == "water32768"
+                          else (None, {"file": None}))
+        out["fp64"]["roofline"] = {
+            "kernel": ADJ_KERNEL64, "bound": "valu", "achieved": round(ach64, 3), "peak": PEAK_FP64,
+            "unit": "TFLOP/s", "frac": round(ach64 / PEAK_FP64, 4), "traffic": tr64,
+            "traffic_unit": "bytes per launch", "traffic_source": tr64_src,
+            "avg_launch_ms": round(avg64, 5),
+            "fwd_pair_step_avg_launch_ms": round(tim64["fwd_pair"]["ms"] / max(tim64["fwd_pair"]["launches"], 1), 5),
         }
 
     if not a.no_mlp:

@@ -2,10 +2,11 @@
 """HBM traffic per launch of the library's kernels, from rocprofv3 hardware counters.
 
 Runs a short bench.py workload (20 + 20 steps of the 98,304-atom box after a 300-step
-equilibration, plus five launches of the library's 256 MiB copy kernel) twice under rocprofv3,
-once counting FETCH_SIZE and once WRITE_SIZE (each pass holds one TCC counter group), averages
-the counters per kernel over its dispatches and scales them by what the same counters read for the
-copy kernel, whose traffic is known exactly (256 MiB read and 256 MiB written per launch). Writes
+equilibration, plus five launches of the library's 256 MiB copy kernel) under rocprofv3, once
+counting FETCH_SIZE and once WRITE_SIZE (each pass holds one TCC counter group), for FP32 and for
+FP64 pair arithmetic, averages the counters per kernel over its dispatches and scales them by what
+the same counters read for the copy kernel in that run, whose traffic is known exactly (256 MiB
+read and 256 MiB written per launch). Writes
 the summary (default gpurun_out/traffic_water32768.json; kept as profiles/traffic_water32768.json),
 which bench.py reads for the roofline's `traffic` when the HIP sources still hash to the value
 recorded there.
@@ -57,10 +58,10 @@ def short_name(name: str) -> str:
     return n[:cut].strip()
 
 
-def run_pass(counter: str) -> dict:
-    d = os.path.join(WORK, counter.lower())
+def run_pass(counter: str, precision: str) -> dict:
+    d = os.path.join(WORK, f"{counter.lower()}_{precision}")
     cmd = ["rocprofv3", "--pmc", counter, "--kernel-trace", "-f", "csv", "-d", d, "-o", "run", "--",
-           sys.executable, *BENCH_ARGS]
+           sys.executable, *BENCH_ARGS, "--precision", precision]
     print("[traffic]", " ".join(cmd), flush=True)
     subprocess.run(["timeout", "-s", "KILL", "300", *cmd], cwd=ROOT, check=True,
                    stdout=subprocess.DEVNULL)
@@ -83,21 +84,26 @@ def run_pass(counter: str) -> dict:
 def main():
     out = sys.argv[1] if len(sys.argv) > 1 else OUT
     os.makedirs(WORK, exist_ok=True)
-    fetch = run_pass("FETCH_SIZE")
-    write = run_pass("WRITE_SIZE")
     nb = COPY_MB << 20
     ck = "k_copy16"
-    sf = nb / (fetch[ck][0] * 1024.0)
-    sw = nb / (write[ck][0] * 1024.0)
-    kernels = {}
-    for k in sorted(set(fetch) & set(write)):
-        kernels[k] = {"fetch_bytes": fetch[k][0] * 1024.0 * sf, "write_bytes": write[k][0] * 1024.0 * sw,
-                      "dispatches": fetch[k][1], "fetch_kib_raw": fetch[k][0], "write_kib_raw": write[k][0]}
+    kernels, scale = {}, {}
+    for prec in ("fp32", "fp64"):
+        fetch = run_pass("FETCH_SIZE", prec)
+        write = run_pass("WRITE_SIZE", prec)
+        sf = nb / (fetch[ck][0] * 1024.0)
+        sw = nb / (write[ck][0] * 1024.0)
+        scale[prec] = {"fetch": sf, "write": sw}
+        for k in sorted(set(fetch) & set(write)):
+            if k in kernels and prec == "fp64":
+                continue  # kernels both runs launch: keep the FP32 run's figures
+            kernels[k] = {"fetch_bytes": fetch[k][0] * 1024.0 * sf, "write_bytes": write[k][0] * 1024.0 * sw,
+                          "dispatches": fetch[k][1], "fetch_kib_raw": fetch[k][0], "write_kib_raw": write[k][0],
+                          "run": prec}
     rec = {
         "command": "python3 rev-sim_amd/measure_traffic.py  (rocprofv3 --pmc FETCH_SIZE | WRITE_SIZE "
-                   "--kernel-trace -f csv -- python3 " + " ".join(BENCH_ARGS) + ")",
+                   "--kernel-trace -f csv -- python3 " + " ".join(BENCH_ARGS) + " --precision fp32 | fp64)",
         "sources_sha256": sources_digest(),
-        "scale": {"fetch": sf, "write": sw, "calibration_kernel": ck, "bytes_each_way": nb},
+        "scale": dict(scale, calibration_kernel=ck, bytes_each_way=nb),
         "kernels": kernels,
     }
     with open(out, "w") as f:
