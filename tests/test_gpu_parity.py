@@ -236,3 +236,28 @@ def test_reversibility_fp64_98k_atoms(gpu_lib):
     assert np.max(np.abs(d)) < 1e-9
     assert abs(lr - lf) < 1e-7
     assert np.all(np.isfinite(g))
+
+
+@pytest.mark.timeout(600)
+def test_config2_98k_fp32_rdf_msd(gpu_lib):
+    """configs[2]'s FP32 path: 32,768 waters (98,304 atoms), 9.95 nm, r_c 1.0 nm, list every 10
+    steps, RDF + MSD loss (MSD over the last 20 steps), 20 + 20 steps, against the oracle with the
+    same FP32 pair-membership rule: loss and dL/dθ within 1e-5 (measured 3e-8 and 7e-8), the
+    per-term losses within 1e-5, and no list rerun."""
+    import revsim
+
+    wb = revsim.water_box(32768, 9.95)
+    kw = dict(msd_window_steps=20, msd_every=5, w_msd=1.0)
+    sim, orc = _pair(FP32, n=32768, box=9.95, rc=1.0, **kw)
+    tgt = revsim.perturbed_target(orc.rdf(wb.coords)[0])
+    sim.set_target_rdf(tgt)
+    orc.set_target_rdf(tgt)
+    L, g = sim.value_and_grad(wb.coords, wb.velocities, 13, 20)
+    Lo, go = orc.value_and_grad(wb.coords, wb.velocities, 13, 20)
+    print(f"98k FP32 20+20 RDF+MSD: loss rel {abs(L - Lo) / abs(Lo):.3e}, dL/dθ rel {rel_err(g, go):.3e}")
+    assert abs(L - Lo) <= 1e-5 * abs(Lo)
+    assert rel_err(g, go) < 1e-5, (g, go)
+    t, to = sim.loss_terms(), orc.loss_terms()
+    for k in ("l_rdf", "l_msd", "D"):
+        assert abs(t[k] - to[k]) <= 1e-5 * abs(to[k]) + 1e-30, (k, t[k], to[k])
+    assert sim.stats()["reruns_call"] == 0
