@@ -1,7 +1,8 @@
 """Edge cases of the GPU path against the FP64 oracle: a molecule that outruns the list's skin and
 an O-H bond beyond the assumed extent (the call is rerun with a wider skin or extent and the
 results equal the oracle's, which needs no list), a molecule count that fills no whole wave or
-block on the cell-list path, and invalid step counts."""
+block on the cell-list path, invalid step counts, and a close contact whose FP64 pair force leaves
+the fixed-point sums' range."""
 import numpy as np
 import pytest
 
@@ -91,3 +92,22 @@ def test_invalid_step_counts(gpu_lib, box216):
     # the context stays usable
     x, v, _ = sim.forward(box216.coords, box216.velocities, 1, 3)
     assert np.all(np.isfinite(x)) and np.all(np.isfinite(v))
+
+
+def test_fixed_point_range_is_an_error(gpu_lib, box216):
+    """FP64 pair forces are exact 64-bit fixed-point sums of terms below 2^18 kJ/mol/nm. Two waters
+    whose O atoms sit 0.12 nm apart (O-O Lennard-Jones force ~3e7 kJ/mol/nm) must make the call
+    fail with RS_E_OVERFLOW rather than return a wrapped sum; the context stays usable and then
+    matches the oracle."""
+    import revsim
+
+    sim, orc = _pair(FP64, 216, 1.862, 0.8)
+    x = box216.coords.copy().reshape(-1, 3, 3)
+    d = _mic(x[1, 0] - x[0, 0], 1.862)
+    x[1] += (x[0, 0] + 0.12 * d / np.linalg.norm(d)) - x[1, 0]  # the whole molecule moves
+    with pytest.raises(revsim.RevSimError) as e:
+        sim.forces(x.reshape(-1))
+    assert e.value.code == -3 and "fixed-point" in str(e.value)
+    F = sim.forces(box216.coords)
+    Fo = orc.forces(box216.coords)
+    assert rel_err(F, Fo) < 1e-12
