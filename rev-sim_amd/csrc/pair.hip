@@ -150,8 +150,8 @@ struct P64 {
 // run in one round (2 % faster than 7 waves); with it in 72 VGPRs, 7 waves (at 8 it spills inside
 // the loops). The adjoint fits 128 VGPRs, 4 waves (prefetching there needs 154 VGPRs, 3 waves, and
 // ran 1 % slower). FP64: the forward at 128 VGPRs, 4 waves (a few spills outside the loops) ran 2 %
-// faster than at 136 VGPRs, 3 waves; the adjoint needs 256 VGPRs, 2 waves (at 3 waves it spills
-// 150 VGPRs).
+// faster than at 136 VGPRs, 3 waves; the adjoint needs 238 VGPRs, 2 waves (at 3 waves it spills
+// ~150 VGPRs).
 template <class P, int CH, bool ADJ, bool FMLP>
 constexpr int pair_waves() {
   return CH != 16 ? 1 : sizeof(typename P::R) == 4 ? (ADJ ? 4 : (FMLP ? 7 : 8)) : (ADJ ? 2 : 4);
@@ -296,6 +296,7 @@ void k_pair(PairArgs A, StepArgs St) {
   const R mrlo = (R)A.mrlo, mh = (R)A.mh, minvh = (R)A.minvh;
 
   const int4 ncnt = act ? A.ncnt[gtid] : make_int4(0, 0, 0, 0);
+  bool fxbad = false;  // FP64 adjoint: a fixed-point term out of range (flagged after the loops)
   auto shell = [&](auto tag) {
     constexpr int S = decltype(tag)::value;
     constexpr int b0 = S == 2 ? 1 : 0;  // first atom of j that can interact
@@ -430,9 +431,11 @@ void k_pair(PairArgs A, StepArgs St) {
             const R ftot2 = ftot * (R)FX_SCALE;
 #pragma unroll
             for (int t = 0; t < 3; ++t) Fq[a][t] += fx_term(ftot2, d[t]);
-            // |c| <= |f/r| r; NaN (coincident atoms) fails the test too. Flagged where it
-            // happens: a flag carried through the loop costs registers.
-            if (!(fabs(ftot2) * (r2 * ri) < (R)FX_CMAX2)) atomicOr(&A.flags[FLAG_FIXED_RANGE], 1);
+            // |c| <= |f/r| r; NaN (coincident atoms) fails the test too. The adjoint carries the
+            // outcome through its loops and flags once (3 % faster there, fewer VGPRs); the
+            // forward flags where it happens (carried, it spills at 4 waves per SIMD).
+            if constexpr (ADJ) fxbad |= !(fabs(ftot2) * (r2 * ri) < (R)FX_CMAX2);
+            else if (!(fabs(ftot2) * (r2 * ri) < (R)FX_CMAX2)) atomicOr(&A.flags[FLAG_FIXED_RANGE], 1);
           }
         }
       }
@@ -441,6 +444,7 @@ void k_pair(PairArgs A, StepArgs St) {
   shell(std::integral_constant<int, 0>());
   shell(std::integral_constant<int, 1>());
   shell(std::integral_constant<int, 2>());
+  if (fxbad) atomicOr(&A.flags[FLAG_FIXED_RANGE], 1);
 
   // combine the CH lanes of the molecule (fixed xor tree, FP64; integer sums in the FP64 path,
   // each lane's bias removed first: one per term it added, i.e. per evaluated atom pair)

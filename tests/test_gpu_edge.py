@@ -108,6 +108,11 @@ def test_fixed_point_range_is_an_error(gpu_lib, box216):
     with pytest.raises(revsim.RevSimError) as e:
         sim.forces(x.reshape(-1))
     assert e.value.code == -3 and "fixed-point" in str(e.value)
+    # the adjoint kernel (flags once after its loops) as well
+    rng = np.random.default_rng(3)
+    with pytest.raises(revsim.RevSimError) as e:
+        sim.adjoint_terms(x.reshape(-1), rng.standard_normal(x.size), rng.standard_normal(x.size))
+    assert e.value.code == -3 and "fixed-point" in str(e.value)
     F = sim.forces(box216.coords)
     Fo = orc.forces(box216.coords)
     assert rel_err(F, Fo) < 1e-12
