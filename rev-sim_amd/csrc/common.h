@@ -370,6 +370,24 @@ __device__ __forceinline__ double chunk_sum(double v) {
   for (int o = CH / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, RS_WAVE);
   return v;
 }
+// FP32 sum over the CH = 8 or 16 lanes of a molecule (consecutive, aligned) by DPP row
+// permutations, one VALU instruction per step and no LDS round trip: xor 1 and xor 2 inside
+// quads, then the mirrored half-row (lane i with 7 - i) and, for 16 lanes, the mirrored row
+// (i with 15 - i). Each lane adds its partner's partial to its own; a + b = b + a exactly, so
+// every lane of the group ends with the same sum, in a fixed order (deterministic).
+template <int CTRL>
+__device__ __forceinline__ float dpp_f32(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+template <int CH>
+__device__ __forceinline__ float chunk_sum_dpp(float v) {
+  static_assert(CH == 8 || CH == 16, "DPP lane sums cover 8 or 16 lanes");
+  v += dpp_f32<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_f32<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_f32<0x141>(v);  // row_half_mirror
+  if constexpr (CH == 16) v += dpp_f32<0x140>(v);  // row_mirror
+  return v;
+}
 template <int CH>
 __device__ __forceinline__ unsigned long long chunk_sum(unsigned long long v) {
 #pragma unroll

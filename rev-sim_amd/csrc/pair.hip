@@ -145,21 +145,23 @@ struct P64 {
 
 // ------------------------------------------------------------------ the pair kernel
 
-// Occupancy targets (waves per SIMD) of the 16-lane variants. FP32: the forward loop gathers one
+// Occupancy targets (waves per SIMD), the same for 16 and 8 lanes per molecule. FP32: the forward loop gathers one
 // entry ahead: without the MLP term in 64 VGPRs, 8 waves per SIMD, so its 8,192 waves at 98k atoms
 // run in one round (2 % faster than 7 waves); with it in 72 VGPRs, 7 waves (at 8 it spills inside
 // the loops). The adjoint fits 128 VGPRs, 4 waves (prefetching there needs 154 VGPRs, 3 waves, and
 // ran 1 % slower). FP64: the forward at 128 VGPRs, 4 waves (a few spills outside the loops) ran 2 %
 // faster than at 136 VGPRs, 3 waves; the adjoint needs 238 VGPRs, 2 waves (at 3 waves it spills
-// ~150 VGPRs).
+// ~150 VGPRs). The FP64 path runs 8 lanes per molecule by default: its adjoint's 4,096 waves at
+// 98k atoms then take two rounds of 2 per SIMD instead of four (adjoint -4 %, forward +2.5 %, the
+// step +1 %); the FP32 path keeps 16 (at 8 both of its kernels ran 3-4 % slower).
 template <class P, int CH, bool ADJ, bool FMLP>
 constexpr int pair_waves() {
-  return CH != 16 ? 1 : sizeof(typename P::R) == 4 ? (ADJ ? 4 : (FMLP ? 7 : 8)) : (ADJ ? 2 : 4);
+  return sizeof(typename P::R) == 4 ? (ADJ ? 4 : (FMLP ? 7 : 8)) : (ADJ ? 2 : 4);
 }
 
 // Whether the pair loop gathers the next entry's inputs one iteration ahead.
 template <class P, int CH, bool ADJ>
-constexpr bool pair_prefetch() { return sizeof(typename P::R) == 4 && CH == 16 && !ADJ; }
+constexpr bool pair_prefetch() { return sizeof(typename P::R) == 4 && !ADJ; }
 
 // FMLP: the forward variant's MLP term as a compile-time switch (the plain forward kernel then
 // holds none of its state); the adjoint variants follow the runtime flag.
@@ -446,8 +448,13 @@ void k_pair(PairArgs A, StepArgs St) {
   shell(std::integral_constant<int, 2>());
   if (fxbad) atomicOr(&A.flags[FLAG_FIXED_RANGE], 1);
 
-  // combine the CH lanes of the molecule (fixed xor tree, FP64; integer sums in the FP64 path,
-  // each lane's bias removed first: one per term it added, i.e. per evaluated atom pair)
+  // combine the CH lanes of the molecule in a fixed order: FP32 partials by DPP row steps
+  // (chunk_sum_dpp), FP64 ones by an xor-shuffle tree, the FP64 path's integer force sums likewise
+  // (each lane's bias removed first: one per term it added, i.e. per evaluated atom pair)
+  auto lane_sum = [](R v) -> double {
+    if constexpr (sizeof(R) == 4) return (double)chunk_sum_dpp<CH>(v);
+    else return chunk_sum<CH>((double)v);
+  };
   double Fd[9], hd[9], Fljd[3], Fsd[3], Fmld[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
@@ -460,14 +467,14 @@ void k_pair(PairArgs A, StepArgs St) {
         const long long q = (long long)chunk_sum<CH>(Fq[a][t] - nterm * FX_BIAS);
         Fd[3 * a + t] = -((double)q * FX_INV);
       } else {
-        Fd[3 * a + t] = chunk_sum<CH>((double)F[a][t]);
+        Fd[3 * a + t] = lane_sum(F[a][t]);
       }
-      if (ADJ) hd[3 * a + t] = chunk_sum<CH>((double)hh[a][t]);
+      if (ADJ) hd[3 * a + t] = lane_sum(hh[a][t]);
     }
     if (ADJ) {
-      Fljd[a] = chunk_sum<CH>((double)Flj[a]);
-      Fsd[a] = chunk_sum<CH>((double)Fs[a]);
-      Fmld[a] = mlp ? chunk_sum<CH>((double)Fml[a]) : 0.0;
+      Fljd[a] = lane_sum(Flj[a]);
+      Fsd[a] = lane_sum(Fs[a]);
+      Fmld[a] = mlp ? lane_sum(Fml[a]) : 0.0;
     }
   }
 
