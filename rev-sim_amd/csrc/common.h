@@ -370,22 +370,31 @@ __device__ __forceinline__ double chunk_sum(double v) {
   for (int o = CH / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, RS_WAVE);
   return v;
 }
-// FP32 sum over the CH = 8 or 16 lanes of a molecule (consecutive, aligned) by DPP row
+// Sum over the CH = 8 or 16 lanes of a molecule (consecutive, aligned) by DPP row
 // permutations, one VALU instruction per step and no LDS round trip: xor 1 and xor 2 inside
 // quads, then the mirrored half-row (lane i with 7 - i) and, for 16 lanes, the mirrored row
 // (i with 15 - i). Each lane adds its partner's partial to its own; a + b = b + a exactly, so
 // every lane of the group ends with the same sum, in a fixed order (deterministic).
-template <int CTRL>
-__device__ __forceinline__ float dpp_f32(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
+// 64-bit values (FP64, integer sums) move as two 32-bit words; their adds are exact (integers) or
+// commutative (FP64) as well.
+template <int CTRL, class T>
+__device__ __forceinline__ T dpp_move(T v) {
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, true));
+  } else {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, true);
+    return __builtin_bit_cast(T, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+  }
 }
-template <int CH>
-__device__ __forceinline__ float chunk_sum_dpp(float v) {
+template <int CH, class T>
+__device__ __forceinline__ T chunk_sum_dpp(T v) {
   static_assert(CH == 8 || CH == 16, "DPP lane sums cover 8 or 16 lanes");
-  v += dpp_f32<0xB1>(v);   // quad_perm [1,0,3,2]
-  v += dpp_f32<0x4E>(v);   // quad_perm [2,3,0,1]
-  v += dpp_f32<0x141>(v);  // row_half_mirror
-  if constexpr (CH == 16) v += dpp_f32<0x140>(v);  // row_mirror
+  v += dpp_move<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_move<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_move<0x141>(v);  // row_half_mirror
+  if constexpr (CH == 16) v += dpp_move<0x140>(v);  // row_mirror
   return v;
 }
 template <int CH>

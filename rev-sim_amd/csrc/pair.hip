@@ -448,13 +448,10 @@ void k_pair(PairArgs A, StepArgs St) {
   shell(std::integral_constant<int, 2>());
   if (fxbad) atomicOr(&A.flags[FLAG_FIXED_RANGE], 1);
 
-  // combine the CH lanes of the molecule in a fixed order: FP32 partials by DPP row steps
-  // (chunk_sum_dpp), FP64 ones by an xor-shuffle tree, the FP64 path's integer force sums likewise
-  // (each lane's bias removed first: one per term it added, i.e. per evaluated atom pair)
-  auto lane_sum = [](R v) -> double {
-    if constexpr (sizeof(R) == 4) return (double)chunk_sum_dpp<CH>(v);
-    else return chunk_sum<CH>((double)v);
-  };
+  // combine the CH lanes of the molecule in a fixed order by DPP row steps (chunk_sum_dpp): FP32
+  // or FP64 partials, and the FP64 path's integer force sums (each lane's bias removed first: one
+  // per term it added, i.e. per evaluated atom pair)
+  auto lane_sum = [](R v) -> double { return (double)chunk_sum_dpp<CH>(v); };
   double Fd[9], hd[9], Fljd[3], Fsd[3], Fmld[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
@@ -464,7 +461,7 @@ void k_pair(PairArgs A, StepArgs St) {
 #pragma unroll
     for (int t = 0; t < 3; ++t) {
       if constexpr (FX) {
-        const long long q = (long long)chunk_sum<CH>(Fq[a][t] - nterm * FX_BIAS);
+        const long long q = (long long)chunk_sum_dpp<CH>(Fq[a][t] - nterm * FX_BIAS);
         Fd[3 * a + t] = -((double)q * FX_INV);
       } else {
         Fd[3 * a + t] = lane_sum(F[a][t]);
