@@ -48,6 +48,8 @@ struct SysParams {
   float rlo2f, rhi2f;  // FP32 screen of the list rule: accept below, reject above, else FP64
   float rlo2q, rhi2q;  // the same band in quantized units (cubic boxes)
   int CH, capc;
+  // entries of the current molecule list: 16-bit words (at most 65,536 molecules) or 32-bit ints
+  int list16;
 };
 
 // ----------------------------------------------------------------------------------------------
@@ -344,6 +346,10 @@ __device__ __forceinline__ float4 ld3f(const float* p) { return make_float4(p[0]
 //   nbr[list_base(p, gtid, s) + k*64]
 // and ncnt[gtid] holds the three sub-list lengths (x, y, z; w unused).
 constexpr int LIST_SHELLS = 3;
+// Entry type of the molecule list: sorted molecule indices, 16-bit where they fit (L16), which
+// halves the list stream of the pair kernels and the list build's stores.
+template <bool L16> struct ListEntry { typedef int type; };
+template <> struct ListEntry<true> { typedef unsigned short type; };
 __device__ __host__ __forceinline__ size_t list_base(const SysParams& p, size_t gtid, int s) {
   return (((gtid / RS_WAVE) * LIST_SHELLS + s) * (size_t)p.capc) * RS_WAVE + (gtid & (RS_WAVE - 1));
 }

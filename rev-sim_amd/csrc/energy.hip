@@ -22,8 +22,9 @@ namespace {
 
 constexpr int NE = 7;  // U, dU/dθ[6]
 
-template <int CH>
-__global__ __launch_bounds__(RS_BLOCK) void k_energy(SysParams p, const int* __restrict__ nbr,
+template <int CH, bool L16>
+__global__ __launch_bounds__(RS_BLOCK) void k_energy(SysParams p,
+                                                      const typename ListEntry<L16>::type* __restrict__ nbr,
                                                       const int4* __restrict__ ncnt,
                                                       const int* __restrict__ order,
                                                       const double* __restrict__ x, double* Fe,
@@ -41,7 +42,7 @@ __global__ __launch_bounds__(RS_BLOCK) void k_energy(SysParams p, const int* __r
     const int4 nc4 = ncnt[gtid];
     for (int s = 0; s < LIST_SHELLS; ++s) {
       const int n = s == 0 ? nc4.x : (s == 1 ? nc4.y : nc4.z);
-      const int* L = nbr + list_base(p, gtid, s);
+      const typename ListEntry<L16>::type* L = nbr + list_base(p, gtid, s);
       const double sig2 = p.sig * p.sig;
       for (int k = 0; k < n; ++k) {
         const double* xj = x + 9 * (size_t)order[L[(size_t)k * RS_WAVE]];
@@ -185,10 +186,16 @@ __global__ void k_reweight_acc(const double* scal, const double* e, double w_rdf
 void launch_energy(const Launch& L) {
   const SysParams& p = L.p;
   const int g = pair_blocks(p);
-  if (p.CH == 8)
-    k_energy<8><<<g, RS_BLOCK, 0, L.s>>>(p, L.b.nbr, L.b.ncnt, L.b.order, L.b.x, L.b.Fe, L.b.eslots, L.b.flags);
+  const unsigned short* n16 = (const unsigned short*)L.b.nbr;
+  const int* n32 = (const int*)L.b.nbr;
+  if (p.CH == 8 && p.list16)
+    k_energy<8, true><<<g, RS_BLOCK, 0, L.s>>>(p, n16, L.b.ncnt, L.b.order, L.b.x, L.b.Fe, L.b.eslots, L.b.flags);
+  else if (p.CH == 8)
+    k_energy<8, false><<<g, RS_BLOCK, 0, L.s>>>(p, n32, L.b.ncnt, L.b.order, L.b.x, L.b.Fe, L.b.eslots, L.b.flags);
+  else if (p.list16)
+    k_energy<16, true><<<g, RS_BLOCK, 0, L.s>>>(p, n16, L.b.ncnt, L.b.order, L.b.x, L.b.Fe, L.b.eslots, L.b.flags);
   else
-    k_energy<16><<<g, RS_BLOCK, 0, L.s>>>(p, L.b.nbr, L.b.ncnt, L.b.order, L.b.x, L.b.Fe, L.b.eslots, L.b.flags);
+    k_energy<16, false><<<g, RS_BLOCK, 0, L.s>>>(p, n32, L.b.ncnt, L.b.order, L.b.x, L.b.Fe, L.b.eslots, L.b.flags);
   k_energy_reduce<<<NE, RS_BLOCK, 0, L.s>>>(L.b.eslots, g, L.b.eout);
 }
 

@@ -23,8 +23,9 @@ struct DevBufs {
   double4* ud;     // FP64 mode: accum_C / m
   double4* O64;    // O position at the last list build, sorted
   int4* Oq;        // the same, quantized (2^32 per box edge), for the list build's FP32 screen
-  // molecule list, ELL by wave and distance shell (list_base in common.h); per-lane lengths
-  int* nbr;
+  // molecule list, ELL by wave and distance shell (list_base in common.h), entries of
+  // ListEntry<p.list16>; per-lane lengths
+  void* nbr;
   int4* ncnt;
   int* nbr_tmp;    // all-pairs list-build scratch: [((gtid/64)*capc + k)*64 + gtid%64]
   int* flags;

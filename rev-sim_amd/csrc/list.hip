@@ -169,10 +169,11 @@ __global__ void k_cells_order(SysParams p, int precision, const int* start, cons
   }
 }
 
-template <int CH>
+template <int CH, bool L16>
 __global__ __launch_bounds__(RS_BLOCK) void k_list_build(SysParams p, const double4* __restrict__ O64,
                                                          const int4* __restrict__ Oq, int* tmp,
-                                                         int* nbr, int4* ncnt, int* flags) {
+                                                         typename ListEntry<L16>::type* nbr, int4* ncnt,
+                                                         int* flags) {
   // All-pairs build for boxes too small for cells. Pass 1: lane c tests every CH-th molecule and
   // keeps the accepted ones in discovery order in its slot column of tmp. Everything goes to
   // shell 0 (all nine atom pairs evaluated).
@@ -241,7 +242,8 @@ __global__ __launch_bounds__(RS_BLOCK) void k_list_build(SysParams p, const doub
           d = q + 1;
           before = incl_of[q];
         }
-      nbr[list_base(p, gtid, 0) + (size_t)slot * RS_WAVE] = src0[(size_t)(g - before) * RS_WAVE + d];
+      nbr[list_base(p, gtid, 0) + (size_t)slot * RS_WAVE] =
+          (typename ListEntry<L16>::type)src0[(size_t)(g - before) * RS_WAVE + d];
     }
   }
   if (act) {
@@ -261,11 +263,13 @@ constexpr int LC_CHUNK = 1280;  // staged candidates per pass (a 98k-atom stenci
 constexpr int LC_MAXH = 256;    // molecules per home cell
 constexpr int LC_MAXR = 2 * RS_WAVE;  // stencil ranges: (2 cs + 1)^2 <= 25 rows, two slots each
 
-template <int CH>
+template <int CH, bool L16>
 __global__ __launch_bounds__(RS_BLOCK) void k_list_cells(SysParams p, const int* __restrict__ start,
                                                          const double4* __restrict__ O64,
-                                                         const int4* __restrict__ Oq, int* nbr,
+                                                         const int4* __restrict__ Oq,
+                                                         typename ListEntry<L16>::type* nbr,
                                                          int4* ncnt, int* flags) {
+  typedef typename ListEntry<L16>::type LT;
   // quantized O positions as three ints (24 KB of LDS per block in all: six blocks per CU)
   struct Q3 {
     int x, y, z;
@@ -360,7 +364,7 @@ __global__ __launch_bounds__(RS_BLOCK) void k_list_cells(SysParams p, const int*
       const int4 qi = Oq[i];
       const double4 oi = O64[i];
       const size_t g0 = (size_t)i * CH;
-      int* out0 = nbr + list_base(p, g0, 0);
+      LT* out0 = nbr + list_base(p, g0, 0);
       const unsigned shell_words = (unsigned)p.capc * RS_WAVE;  // one shell of a wave's list
       unsigned T0 = Ts[0][m], T1 = Ts[1][m], T2 = Ts[2][m];
 #pragma unroll 2
@@ -402,7 +406,7 @@ __global__ __launch_bounds__(RS_BLOCK) void k_list_cells(SysParams p, const int*
                              __builtin_amdgcn_mbcnt_hi((unsigned)(ms >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)ms, 0u));
           const unsigned slot = g / CH;
           const unsigned shoff = s0 ? 0u : (s01 ? shell_words : 2u * shell_words);
-          if (slot < (unsigned)p.capc) out0[shoff + slot * RS_WAVE + (g % CH)] = j;
+          if (slot < (unsigned)p.capc) out0[shoff + slot * RS_WAVE + (g % CH)] = (LT)j;
         }
         T0 += __popcll(b0);
         T1 += __popcll(m1);
@@ -444,20 +448,22 @@ void launch_list_build(const Launch& L, bool with_u, bool filed) {
   int gw = (p.ncell * RS_WAVE + RS_BLOCK - 1) / RS_BLOCK;
   k_cells_order<<<gw, RS_BLOCK, 0, L.s>>>(p, L.precision, b.cell_start, b.tmp, b.order, b.rank, b.x,
                                           with_u ? b.C : nullptr, b.posq, b.pos64, b.pu, b.ud, b.O64, b.Oq);
-  if (!p.brute) {
-    switch (p.CH) {
-      case 4: k_list_cells<4><<<(p.ncell + RS_XCDS - 1) / RS_XCDS * RS_XCDS, RS_BLOCK, 0, L.s>>>(p, b.cell_start, b.O64, b.Oq, b.nbr, b.ncnt, b.flags); break;
-      case 8: k_list_cells<8><<<(p.ncell + RS_XCDS - 1) / RS_XCDS * RS_XCDS, RS_BLOCK, 0, L.s>>>(p, b.cell_start, b.O64, b.Oq, b.nbr, b.ncnt, b.flags); break;
-      case 16: k_list_cells<16><<<(p.ncell + RS_XCDS - 1) / RS_XCDS * RS_XCDS, RS_BLOCK, 0, L.s>>>(p, b.cell_start, b.O64, b.Oq, b.nbr, b.ncnt, b.flags); break;
-      default: k_list_cells<32><<<(p.ncell + RS_XCDS - 1) / RS_XCDS * RS_XCDS, RS_BLOCK, 0, L.s>>>(p, b.cell_start, b.O64, b.Oq, b.nbr, b.ncnt, b.flags); break;
-    }
-    return;
-  }
-  int gl = pair_blocks(p);
-  switch (p.CH) {
-    case 4: k_list_build<4><<<gl, RS_BLOCK, 0, L.s>>>(p, b.O64, b.Oq, b.nbr_tmp, b.nbr, b.ncnt, b.flags); break;
-    case 8: k_list_build<8><<<gl, RS_BLOCK, 0, L.s>>>(p, b.O64, b.Oq, b.nbr_tmp, b.nbr, b.ncnt, b.flags); break;
-    case 16: k_list_build<16><<<gl, RS_BLOCK, 0, L.s>>>(p, b.O64, b.Oq, b.nbr_tmp, b.nbr, b.ncnt, b.flags); break;
-    default: k_list_build<32><<<gl, RS_BLOCK, 0, L.s>>>(p, b.O64, b.Oq, b.nbr_tmp, b.nbr, b.ncnt, b.flags); break;
-  }
+  // p.CH is 8 or 16 (rs_create), the entry width follows p.list16
+  auto build = [&](auto ch, auto l16) {
+    constexpr int CH = decltype(ch)::value;
+    constexpr bool L16 = decltype(l16)::value;
+    typedef typename ListEntry<L16>::type LT;
+    if (!p.brute)
+      k_list_cells<CH, L16><<<(p.ncell + RS_XCDS - 1) / RS_XCDS * RS_XCDS, RS_BLOCK, 0, L.s>>>(
+          p, b.cell_start, b.O64, b.Oq, (LT*)b.nbr, b.ncnt, b.flags);
+    else
+      k_list_build<CH, L16><<<pair_blocks(p), RS_BLOCK, 0, L.s>>>(p, b.O64, b.Oq, b.nbr_tmp, (LT*)b.nbr,
+                                                                   b.ncnt, b.flags);
+  };
+  auto by_width = [&](auto ch) {
+    if (p.list16) build(ch, std::true_type());
+    else build(ch, std::false_type());
+  };
+  if (p.CH == 8) by_width(std::integral_constant<int, 8>());
+  else by_width(std::integral_constant<int, 16>());
 }

@@ -54,9 +54,9 @@ __device__ __forceinline__ void contrib64(const RdfArgs& r, double rd, unsigned 
 
 #define RDF_MAX_BINS 1024
 
-template <int CH>
+template <int CH, bool L16>
 __global__ __launch_bounds__(RS_BLOCK) void k_rdf_hist(SysParams p, RdfArgs r,
-                                                        const int* __restrict__ nbr,
+                                                        const typename ListEntry<L16>::type* __restrict__ nbr,
                                                         const int4* __restrict__ ncnt,
                                                         const int* __restrict__ order,
                                                         const double* __restrict__ x,
@@ -70,7 +70,7 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rdf_hist(SysParams p, RdfArgs r,
     const int4 nc4 = ncnt[gtid];
     for (int s = 0; s < LIST_SHELLS; ++s) {
       const int n = s == 0 ? nc4.x : (s == 1 ? nc4.y : nc4.z);
-      const int* L = nbr + list_base(p, gtid, s);
+      const typename ListEntry<L16>::type* L = nbr + list_base(p, gtid, s);
       const double* xi = x + 9 * (size_t)order[i];
       for (int k = 0; k < n; ++k) {
         const int j = L[(size_t)k * RS_WAVE];
@@ -116,9 +116,9 @@ __global__ void k_rdf_final(RdfArgs r, const long long* hist, const double* targ
 }
 
 // dL/dx_O (times the block weight) by owner-computes over the full list, FP64.
-template <int CH>
+template <int CH, bool L16>
 __global__ __launch_bounds__(RS_BLOCK) void k_rdf_grad(SysParams p, RdfArgs r,
-                                                        const int* __restrict__ nbr,
+                                                        const typename ListEntry<L16>::type* __restrict__ nbr,
                                                         const int4* __restrict__ ncnt,
                                                         const int* __restrict__ order,
                                                         const double* __restrict__ x,
@@ -134,7 +134,7 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rdf_grad(SysParams p, RdfArgs r,
     const int4 nc4 = ncnt[gtid];
     for (int s = 0; s < LIST_SHELLS; ++s) {
       const int n = s == 0 ? nc4.x : (s == 1 ? nc4.y : nc4.z);
-      const int* L = nbr + list_base(p, gtid, s);
+      const typename ListEntry<L16>::type* L = nbr + list_base(p, gtid, s);
       const double dxb = r.rmax / r.nb, iw = 1.0 / r.width, rmaxw = r.rmax + r.window;
       for (int k = 0; k < n; ++k) {
         const int j = L[(size_t)k * RS_WAVE];
@@ -282,17 +282,22 @@ void launch_rdf(const Launch& L, int nb, double rmax, double width, double windo
   r.pre = V / (4.0 * M_PI * (double)p.M * (double)p.M * dx);
   k_zero_ll<<<(nb + RS_BLOCK - 1) / RS_BLOCK, RS_BLOCK, 0, L.s>>>(L.b.hist, nb);
   int g = pair_blocks(p);
-  if (p.CH == 8)
-    k_rdf_hist<8><<<g, RS_BLOCK, 0, L.s>>>(p, r, L.b.nbr, L.b.ncnt, L.b.order, L.b.x, L.b.hist);
-  else
-    k_rdf_hist<16><<<g, RS_BLOCK, 0, L.s>>>(p, r, L.b.nbr, L.b.ncnt, L.b.order, L.b.x, L.b.hist);
-  k_rdf_final<<<1, RS_BLOCK, 0, L.s>>>(r, L.b.hist, L.b.target, L.b.rdf_g, L.b.rdfG, L.b.scal, loss_dst);
-  if (grad) {
-    if (p.CH == 8)
-      k_rdf_grad<8><<<g, RS_BLOCK, 0, L.s>>>(p, r, L.b.nbr, L.b.ncnt, L.b.order, L.b.x, L.b.rdfG, L.b.grad);
-    else
-      k_rdf_grad<16><<<g, RS_BLOCK, 0, L.s>>>(p, r, L.b.nbr, L.b.ncnt, L.b.order, L.b.x, L.b.rdfG, L.b.grad);
-  }
+  // p.CH is 8 or 16, the entry width follows p.list16
+  auto run = [&](auto ch, auto l16) {
+    constexpr int CH = decltype(ch)::value;
+    constexpr bool L16 = decltype(l16)::value;
+    typedef typename ListEntry<L16>::type LT;
+    const LT* nbr = (const LT*)L.b.nbr;
+    k_rdf_hist<CH, L16><<<g, RS_BLOCK, 0, L.s>>>(p, r, nbr, L.b.ncnt, L.b.order, L.b.x, L.b.hist);
+    k_rdf_final<<<1, RS_BLOCK, 0, L.s>>>(r, L.b.hist, L.b.target, L.b.rdf_g, L.b.rdfG, L.b.scal, loss_dst);
+    if (grad) k_rdf_grad<CH, L16><<<g, RS_BLOCK, 0, L.s>>>(p, r, nbr, L.b.ncnt, L.b.order, L.b.x, L.b.rdfG, L.b.grad);
+  };
+  auto by_width = [&](auto ch) {
+    if (p.list16) run(ch, std::true_type());
+    else run(ch, std::false_type());
+  };
+  if (p.CH == 8) by_width(std::integral_constant<int, 8>());
+  else by_width(std::integral_constant<int, 16>());
 }
 
 void launch_zero(const Launch& L, double* a, size_t n) {

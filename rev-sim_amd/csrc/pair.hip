@@ -40,7 +40,7 @@ namespace {
 
 struct PairArgs {
   SysParams p;
-  const int* __restrict__ nbr;
+  const void* __restrict__ nbr;  // ListEntry<L16> entries
   const int4* __restrict__ ncnt;
   const int* __restrict__ order;
   const void* __restrict__ pos;
@@ -168,9 +168,10 @@ constexpr bool pair_prefetch() { return sizeof(typename P::R) == 4 && !ADJ; }
 // STEP (forward only): the kernel ends with the integrator step of its molecules (St), so the
 // forward loop runs one kernel per step. The new sorted positions go to the other buffer
 // (St.posq_next / St.pos64_next): other blocks may still be reading this step's.
-template <class P, int CH, bool ADJ, bool FMLP, bool STEP>
+template <class P, int CH, bool ADJ, bool FMLP, bool STEP, bool L16>
 __global__ __launch_bounds__(RS_BLOCK) __attribute__((amdgpu_waves_per_eu(pair_waves<P, CH, ADJ, FMLP>())))
 void k_pair(PairArgs A, StepArgs St) {
+  typedef typename ListEntry<L16>::type LT;
   typedef typename P::R R;
   typedef typename P::Pos Pos;
   typedef typename P::U U;
@@ -225,12 +226,28 @@ void k_pair(PairArgs A, StepArgs St) {
   };
   __shared__ StepIn stin[STEP ? 3 * NM : 1];
   __shared__ int smol[STEP ? NM : 1];
-  const int sw = threadIdx.x / RS_WAVE, sl = threadIdx.x % RS_WAVE;  // step wave and lane
-  const bool slv = sl < 3 * MPW;
-  const int sml = sw * MPW + (slv ? sl / 3 : 0), sa = slv ? sl % 3 : 0;  // molecule, atom
-  const int sim = blk * NM + sml;                                        // its sorted index
-  const bool sal = slv && sml < NM && sim < p.M;
+  // a thread's place in the step: wave sw, lane sl, molecule sml of the block (sorted index sim)
+  // and atom sa; sal: the lane holds an atom. The epilogue recomputes it from an opaque copy of
+  // the thread index rather than keeping it live across the pair loops (in 64 VGPRs it spilled).
+  struct StepLane {
+    int sw, sl, sml, sa, sim;
+    bool slv, sal;
+  };
+  auto step_lane = [&](int tid) {
+    StepLane q;
+    q.sw = tid / RS_WAVE;
+    q.sl = tid % RS_WAVE;
+    q.slv = q.sl < 3 * MPW;
+    q.sml = q.sw * MPW + (q.slv ? q.sl / 3 : 0);
+    q.sa = q.slv ? q.sl % 3 : 0;
+    q.sim = blk * NM + q.sml;
+    q.sal = q.slv && q.sml < NM && q.sim < p.M;
+    return q;
+  };
   if constexpr (STEP) {
+    const StepLane SL = step_lane(threadIdx.x);
+    const int sml = SL.sml, sa = SL.sa, sim = SL.sim;
+    const bool sal = SL.sal;
     if (sal) {
       const int m = A.order[sim];
       const int atom = 3 * m + sa;
@@ -303,7 +320,7 @@ void k_pair(PairArgs A, StepArgs St) {
     constexpr int S = decltype(tag)::value;
     constexpr int b0 = S == 2 ? 1 : 0;  // first atom of j that can interact
     const int n = S == 0 ? ncnt.x : (S == 1 ? ncnt.y : ncnt.z);
-    const int* __restrict__ L = A.nbr + list_base(p, gtid, S);
+    const LT* __restrict__ L = (const LT*)A.nbr + list_base(p, gtid, S);
     // Software pipeline: the next entry's index (and, where registers allow, its inputs) is
     // loaded while this entry is evaluated; past the end the last entry is read again.
     constexpr bool PF = pair_prefetch<P, CH, ADJ>();
@@ -493,6 +510,11 @@ void k_pair(PairArgs A, StepArgs St) {
       for (int t = 0; t < 9; ++t) fpair[threadIdx.x / CH][t] = Fd[t];
     }
     __syncthreads();
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const StepLane SL = step_lane(tid);
+    const int sw = SL.sw, sl = SL.sl, sml = SL.sml, sa = SL.sa, sim = SL.sim;
+    const bool slv = SL.slv, sal = SL.sal;
     if (sw * MPW < NM) {  // wave-uniform
       const int m = sal ? smol[sml] : 0;
       const int atom = 3 * m + sa;
@@ -600,26 +622,32 @@ __global__ __launch_bounds__(RS_BLOCK) void k_vjp_reduce(const double* __restric
   if (threadIdx.x == 0) dth[k] += red[0];
 }
 
-template <class P, int CH, bool ADJ>
+template <class P, int CH, bool ADJ, bool L16>
 void dispatch_ch(const Launch& L, const PairArgs& a, const StepArgs* st) {
   const int g = pair_blocks(L.p);
   StepArgs none{};
   if constexpr (!ADJ) {
     if (st) {
-      if (a.mlp) k_pair<P, CH, false, true, true><<<g, RS_BLOCK, 0, L.s>>>(a, *st);
-      else k_pair<P, CH, false, false, true><<<g, RS_BLOCK, 0, L.s>>>(a, *st);
+      if (a.mlp) k_pair<P, CH, false, true, true, L16><<<g, RS_BLOCK, 0, L.s>>>(a, *st);
+      else k_pair<P, CH, false, false, true, L16><<<g, RS_BLOCK, 0, L.s>>>(a, *st);
       return;
     }
   }
   constexpr bool F32ADJ = ADJ && sizeof(typename P::R) == 4;
-  if (a.mlp && !F32ADJ) k_pair<P, CH, ADJ, !F32ADJ, false><<<g, RS_BLOCK, 0, L.s>>>(a, none);
-  else k_pair<P, CH, ADJ, false, false><<<g, RS_BLOCK, 0, L.s>>>(a, none);
+  if (a.mlp && !F32ADJ) k_pair<P, CH, ADJ, !F32ADJ, false, L16><<<g, RS_BLOCK, 0, L.s>>>(a, none);
+  else k_pair<P, CH, ADJ, false, false, L16><<<g, RS_BLOCK, 0, L.s>>>(a, none);
 }
 
+// lanes per molecule (8 or 16, rs_create) and list entry width (p.list16)
 template <class P, bool ADJ>
 void dispatch(const Launch& L, const PairArgs& a, const StepArgs* st) {
-  if (L.p.CH == 8) dispatch_ch<P, 8, ADJ>(L, a, st);
-  else dispatch_ch<P, 16, ADJ>(L, a, st);
+  if (L.p.CH == 8) {
+    if (L.p.list16) dispatch_ch<P, 8, ADJ, true>(L, a, st);
+    else dispatch_ch<P, 8, ADJ, false>(L, a, st);
+  } else {
+    if (L.p.list16) dispatch_ch<P, 16, ADJ, true>(L, a, st);
+    else dispatch_ch<P, 16, ADJ, false>(L, a, st);
+  }
 }
 
 }  // namespace
