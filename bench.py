@@ -43,8 +43,6 @@ PEAK_HBM = 8000.0  # GB/s
 # algorithmic FLOP per pair within r_c (add/sub/mul 1, FMA 2, sqrt/rcp/exp 1): base, extra for O-O LJ
 FLOP_FWD = (36, 15)
 FLOP_REV = (72, 33)
-ADJ_KERNEL = "k_pair<P32, 16, true, false, false>"
-ADJ_KERNEL64 = "k_pair<P64, 16, true, false, false>"
 PEAK_FP64 = 78.6  # TFLOP/s, MI355X vector FP64
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_water32768.json")
 
@@ -171,7 +169,16 @@ def cpu_model() -> str:
     return platform.processor() or "unknown"
 
 
-def traffic_from_profile(digest, kernel=ADJ_KERNEL):
+def adj_kernel(fp64: bool, pair_chunks: int, n_molecules: int) -> str:
+    """Name of the adjoint pair kernel a trajectory runs: k_pair<precision, lanes per molecule,
+    adjoint, MLP compiled in, step epilogue, 16-bit list entries>. Lanes default to 16 (FP32) and 8
+    (FP64); the reverse pass's lists have 16-bit entries up to 65,536 molecules."""
+    ch = pair_chunks or (8 if fp64 else 16)
+    l16 = "true" if n_molecules <= 65536 else "false"
+    return f"k_pair<{'P64' if fp64 else 'P32'}, {ch}, true, false, false, {l16}>"
+
+
+def traffic_from_profile(digest, kernel):
     """Bytes per launch of an adjoint pair kernel from the committed counter run
     (rev-sim_amd/measure_traffic.py), used only if it was taken on these kernel sources."""
     try:
@@ -266,10 +273,11 @@ def main():
     fp = tim["fwd_pair"]
     fwd_avg = fp["ms"] / max(fp["launches"], 1)
     digest = sources_digest()
-    traffic, traffic_src = (traffic_from_profile(digest) if a.workload == "water32768" and prec == FP32
+    adj_name = adj_kernel(prec != FP32, a.pair_chunks, wl["n_molecules"])
+    traffic, traffic_src = (traffic_from_profile(digest, adj_name) if a.workload == "water32768"
                             else (None, {"file": None}))
     roof = {
-        "kernel": ADJ_KERNEL + " (reverse pair forces + HVP + θ-VJP)",
+        "kernel": adj_name + " (reverse pair forces + HVP + θ-VJP)",
         "bound": "valu",
         "achieved": round(achieved, 3),
         "peak": PEAK_FP32,
@@ -344,10 +352,11 @@ def main():
         rp64 = tim64["rev_pair"]
         avg64 = rp64["ms"] / max(rp64["launches"], 1)
         ach64 = flop / (avg64 * 1e-3) / 1e12
-        tr64, tr64_src = (traffic_from_profile(digest, ADJ_KERNEL64) if a.workload == "water32768"
+        adj64 = adj_kernel(True, a.pair_chunks, wl["n_molecules"])
+        tr64, tr64_src = (traffic_from_profile(digest, adj64) if a.workload == "water32768"
                           else (None, {"file": None}))
         out["fp64"]["roofline"] = {
-            "kernel": ADJ_KERNEL64, "bound": "valu", "achieved": round(ach64, 3), "peak": PEAK_FP64,
+            "kernel": adj64, "bound": "valu", "achieved": round(ach64, 3), "peak": PEAK_FP64,
             "unit": "TFLOP/s", "frac": round(ach64 / PEAK_FP64, 4), "traffic": tr64,
             "traffic_unit": "bytes per launch", "traffic_source": tr64_src,
             "avg_launch_ms": round(avg64, 5),
