@@ -553,8 +553,8 @@ int forward_impl(rs_ctx* c, const double* x0, const double* v0, uint64_t seed, i
     return fail(c, RS_E_CONFIG, "the reweighting estimator needs the classical force field (use_mlp = 0)");
   const bool baro = g.barostat_every > 0;
   if (baro && P.msd_on) return fail(c, RS_E_CONFIG, "the MSD loss needs a constant box (barostat_every = 0)");
-  c->list16_next = 0;
   if (baro && c->cfg.use_mlp) return fail(c, RS_E_CONFIG, "the barostat needs the classical force field (use_mlp = 0)");
+  c->list16_next = 0;
   const double box0[3] = {g.box[0], g.box[1], g.box[2]};
   BarostatState bs;
   if (P.msd_on && (r = ensure_frames(c, P.nf))) return r;
