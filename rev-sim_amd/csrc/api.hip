@@ -404,6 +404,7 @@ int make_plan(rs_ctx* c, int n_steps, Plan* p) {
 int ensure_frames(rs_ctx* c, int nf) {
   DevBufs& b = c->L.b;
   if (nf <= c->frames_cap) return 0;
+  c->frames_cap = 0;  // until every buffer below is allocated again
   if (b.frames) hipFree(b.frames);
   if (b.msd_grad) hipFree(b.msd_grad);
   if (b.msd_del) hipFree(b.msd_del);
@@ -704,8 +705,13 @@ int reverse_impl(rs_ctx* c, const double* xN, const double* vN, uint64_t seed, i
   for (int attempt = 0;; ++attempt) {
     if (c->L.use_mlp)
       HIPCHK(c, hipMemsetAsync(b.mlp_slot, 0, sizeof(double) * (size_t)pair_blocks(c->L.p) * MLP_NG, c->s));
-    if (xN) HIPCHK(c, hipMemcpyAsync(b.x, xN, sizeof(double) * c->N3, hipMemcpyHostToDevice, c->s));
-    if (vN) HIPCHK(c, hipMemcpyAsync(b.v, vN, sizeof(double) * c->N3, hipMemcpyHostToDevice, c->s));
+    if (xN && vN) {
+      HIPCHK(c, hipMemcpyAsync(b.x, xN, sizeof(double) * c->N3, hipMemcpyHostToDevice, c->s));
+      HIPCHK(c, hipMemcpyAsync(b.v, vN, sizeof(double) * c->N3, hipMemcpyHostToDevice, c->s));
+    } else if (!restore(c, n_steps)) {
+      // no host state given (rs_value_and_grad): the forward pass's final snapshot, in HBM
+      return fail(c, RS_E_STATE, "internal: no snapshot of the final forward state");
+    }
     if ((r = reset_flags(c))) return r;
     HIPCHK(c, hipMemsetAsync(b.dth, 0, sizeof(double) * 8, c->s));
     HIPCHK(c, hipMemsetAsync(b.slots, 0, sizeof(double) * 8 * (size_t)nslots, c->s));
@@ -1070,9 +1076,9 @@ int rs_value_and_grad(rs_ctx* c, const double* x0, const double* v0, uint64_t se
   int r = begin_call(c);
   if (r) return r;
   if (!x0 || !v0) return fail(c, RS_E_CONFIG, "null x0/v0");
-  std::vector<double> xN(c->N3), vN(c->N3);
-  if ((r = forward_impl(c, x0, v0, seed, n_steps, xN.data(), vN.data(), nullptr))) return r;
-  if ((r = reverse_impl(c, xN.data(), vN.data(), seed, n_steps, loss_out, dth, nullptr, nullptr))) return r;
+  // the reverse pass starts from the forward pass's final state where it lies, in HBM
+  if ((r = forward_impl(c, x0, v0, seed, n_steps, nullptr, nullptr, nullptr))) return r;
+  if ((r = reverse_impl(c, nullptr, nullptr, seed, n_steps, loss_out, dth, nullptr, nullptr))) return r;
   if (dL_dmlp) {
     if (!c->L.use_mlp) return fail(c, RS_E_STATE, "dL_dmlp requested without the MLP term");
     std::vector<double> d(MLP_NW);
