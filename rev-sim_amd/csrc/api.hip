@@ -490,7 +490,7 @@ int finish(rs_ctx* c, int* retry) {
   if (f[FLAG_NONFINITE]) return fail(c, RS_E_NONFINITE, "non-finite positions, velocities or forces");
   if (f[FLAG_MLP_RANGE]) return fail(c, RS_E_NONFINITE, "an O-O pair came closer than the MLP table's range");
   if (f[FLAG_FIXED_RANGE])
-    return fail(c, RS_E_OVERFLOW, "an FP64 pair-force term exceeded the fixed-point sums' range (2^18 kJ/mol/nm)");
+    return fail(c, RS_E_OVERFLOW, "an FP64 pair-force term exceeded the fixed-point sums' range (2^17 kJ/mol/nm)");
   *retry = (f[FLAG_LIST_OVERFLOW] ? 1 : 0) | (f[FLAG_STALE] ? 2 : 0) | (f[FLAG_HEXTENT] ? 4 : 0) |
            (f[FLAG_CELL_OVERFLOW] ? 8 : 0);
   return 0;
@@ -805,7 +805,7 @@ int single_eval(rs_ctx* c, const double* x, bool with_u, Body body) {
     HIPCHK(c, hipMemcpy(f, b.flags, sizeof(f), hipMemcpyDeviceToHost));
     if (f[FLAG_NONFINITE]) return fail(c, RS_E_NONFINITE, "non-finite forces");
     if (f[FLAG_FIXED_RANGE])
-      return fail(c, RS_E_OVERFLOW, "an FP64 pair-force term exceeded the fixed-point sums' range (2^18 kJ/mol/nm)");
+      return fail(c, RS_E_OVERFLOW, "an FP64 pair-force term exceeded the fixed-point sums' range (2^17 kJ/mol/nm)");
     // a single evaluation builds its list at x: only capacity, the O-H extent and a cell's
     // occupancy can fail
     const int why = (f[FLAG_LIST_OVERFLOW] ? 1 : 0) | (f[FLAG_HEXTENT] ? 4 : 0) | (f[FLAG_CELL_OVERFLOW] ? 8 : 0);

@@ -17,7 +17,7 @@ enum {
   FLAG_HEXTENT = 3,        // an O-H distance exceeded the extent assumed by the list radius
   FLAG_MLP_RANGE = 4,      // an O-O pair closer than the MLP table's lower end
   FLAG_CELL_OVERFLOW = 5,  // a home cell of the list grid held more molecules than a build block stages
-  FLAG_FIXED_RANGE = 6,    // an FP64 pair-force term beyond the fixed-point sums' exact range (2^18)
+  FLAG_FIXED_RANGE = 6,    // an FP64 pair-force term beyond the fixed-point sums' exact range (2^17 kJ/mol/nm)
   FLAG_COUNT = 8
 };
 

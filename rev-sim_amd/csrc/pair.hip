@@ -97,7 +97,11 @@ __device__ __forceinline__ unsigned long long fix_rn(double v) {
 // bias once per term at the end.
 constexpr double FX_SCALE = 8589934592.0;        // 2^33
 constexpr double FX_INV = 1.0 / 8589934592.0;
-constexpr double FX_CMAX2 = 2251799813685248.0;  // 2^51: |c 2^33| at which the bias trick stops being exact
+// Range of a term: the bias trick is exact for |c 2^33| < 2^51. Each term is bounded by the pair's
+// |f| = |f/r| r (|d_t| <= r), checked as |f/r 2^33| <= 2^50 / r: one multiply by 1/r, which
+// the pair has, instead of forming r; the factor 2 covers 1/r's rounding. Out-of-range pairs
+// (f/r and 1/r zeroed) pass as 0 <= 0; NaN fails.
+constexpr double FX_CMAX2 = 1125899906842624.0;  // 2^50
 constexpr unsigned long long FX_BIAS = 0x4338000000000000ULL;
 __device__ __forceinline__ unsigned long long fx_term(double ftot2, double d) {
   return (unsigned long long)__double_as_longlong(add_nc(ftot2 * d, 6755399441055744.0));
@@ -455,11 +459,11 @@ void k_pair(PairArgs A, StepArgs St) {
             const R ftot2 = ftot * (R)FX_SCALE;
 #pragma unroll
             for (int t = 0; t < 3; ++t) Fq[a][t] += fx_term(ftot2, d[t]);
-            // |c| <= |f/r| r; NaN (coincident atoms) fails the test too. The adjoint carries the
-            // outcome through its loops and flags once (3 % faster there, fewer VGPRs); the
-            // forward flags where it happens (carried, it spills at 4 waves per SIMD).
-            if constexpr (ADJ) fxbad |= !(fabs(ftot2) * (r2 * ri) < (R)FX_CMAX2);
-            else if (!(fabs(ftot2) * (r2 * ri) < (R)FX_CMAX2)) atomicOr(&A.flags[FLAG_FIXED_RANGE], 1);
+            // |c| <= |f/r| r (FX_CMAX2); NaN (coincident atoms) fails the test too. The adjoint
+            // carries the outcome through its loops and flags once (3 % faster there, fewer
+            // VGPRs); the forward flags where it happens (carried, it spills at 4 waves per SIMD).
+            if constexpr (ADJ) fxbad |= !(fabs(ftot2) <= (R)FX_CMAX2 * ri);
+            else if (!(fabs(ftot2) <= (R)FX_CMAX2 * ri)) atomicOr(&A.flags[FLAG_FIXED_RANGE], 1);
           }
         }
       }

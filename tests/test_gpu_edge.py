@@ -95,7 +95,7 @@ def test_invalid_step_counts(gpu_lib, box216):
 
 
 def test_fixed_point_range_is_an_error(gpu_lib, box216):
-    """FP64 pair forces are exact 64-bit fixed-point sums of terms below 2^18 kJ/mol/nm. Two waters
+    """FP64 pair forces are exact 64-bit fixed-point sums of terms below 2^17 kJ/mol/nm. Two waters
     whose O atoms sit 0.12 nm apart (O-O Lennard-Jones force ~3e7 kJ/mol/nm) must make the call
     fail with RS_E_OVERFLOW rather than return a wrapped sum; the context stays usable and then
     matches the oracle."""
