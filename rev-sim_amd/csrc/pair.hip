@@ -163,6 +163,10 @@ constexpr int pair_waves() {
   return sizeof(typename P::R) == 4 ? (ADJ ? 4 : (FMLP ? 7 : 8)) : (ADJ ? 2 : 4);
 }
 
+// Whether the pair loop branches over out-of-range atom pairs (measured per kernel).
+template <class P, bool ADJ>
+constexpr bool pair_skip() { return true; }
+
 // Whether the pair loop gathers the next entry's inputs one iteration ahead.
 template <class P, int CH, bool ADJ>
 constexpr bool pair_prefetch() { return sizeof(typename P::R) == 4 && !ADJ; }
@@ -320,6 +324,10 @@ void k_pair(PairArgs A, StepArgs St) {
 
   const int4 ncnt = act ? A.ncnt[gtid] : make_int4(0, 0, 0, 0);
   bool fxbad = false;  // FP64 adjoint: a fixed-point term out of range (flagged after the loops)
+  // SKIP: out-of-range atom pairs are branched over rather than evaluated to zeros; the FP64 path
+  // then counts the terms each own atom added (nt) to remove their bias
+  constexpr bool SKIP = pair_skip<P, ADJ>();
+  unsigned nt[3] = {0u, 0u, 0u};
   auto shell = [&](auto tag) {
     constexpr int S = decltype(tag)::value;
     constexpr int b0 = S == 2 ? 1 : 0;  // first atom of j that can interact
@@ -367,6 +375,11 @@ void k_pair(PairArgs A, StepArgs St) {
           // fewer; in the FP64 forward the same change ran 4 % slower).
           constexpr bool MASK_F = !ADJ && sizeof(R) == 4;
           const bool in = r2 <= rc2;
+          if constexpr (SKIP) {
+            // out of range: exact zeros; a wave whose lanes all miss this atom pair skips it
+            if (!in) continue;
+            if constexpr (FX) ++nt[a];
+          }
           const R ri = (MASK_F || in) ? P::rinv(r2) : (R)0;
           const R ri2 = ri * ri, ri3 = ri2 * ri;
           const int ty = (a > 0) + (b > 0);
@@ -483,7 +496,8 @@ void k_pair(PairArgs A, StepArgs St) {
   for (int a = 0; a < 3; ++a) {
     // terms per own atom: shell 0 all three atoms of j, shell 1 no O-O, shell 2 H-H only
     const unsigned long long nterm =
-        3ull * (unsigned)ncnt.x + (a == 0 ? 2ull : 3ull) * (unsigned)ncnt.y + (a == 0 ? 0ull : 2ull) * (unsigned)ncnt.z;
+        SKIP ? (unsigned long long)nt[a]
+             : 3ull * (unsigned)ncnt.x + (a == 0 ? 2ull : 3ull) * (unsigned)ncnt.y + (a == 0 ? 0ull : 2ull) * (unsigned)ncnt.z;
 #pragma unroll
     for (int t = 0; t < 3; ++t) {
       if constexpr (FX) {
