@@ -97,7 +97,7 @@ typedef struct rs_config {
   double  w_msd;
   double  clip_norm;       /* clip ||dF/dx^T lambda|| at this value (<= 0 or inf: no clip) */
   int32_t use_mlp;         /* 0 = classical force field only, 1 = + radial-basis MLP on O-O pairs */
-  int32_t pair_chunks;     /* lanes cooperating on one molecule's list: 8 or 16 (0 = default: 16 with FP32 pair arithmetic, 8 with FP64) */
+  int32_t pair_chunks;     /* lanes cooperating on one molecule's list: 8 or 16 (0 = default: 16) */
   /* Enthalpy-of-vaporization block loss (enthalpy_vaporization / loss_enth_vap,
      water/water.jl:738-754), evaluated with the RDF loss every n_steps_loss steps:
      dH = dhvap_u_gas - U/n_molecules + kB*temperature, L = w_dhvap * (dH - dhvap_target)^2.

@@ -954,8 +954,7 @@ int rs_create(const rs_config* cfg, int32_t device, rs_ctx** out) {
   c->L.precision = g.precision;
   c->L.use_mlp = 0;
   c->L.mlp_rc = g.cutoff;
-  // lanes per molecule in the pair kernels: 16 for FP32 pair arithmetic, 8 for FP64 (pair.hip)
-  c->L.p.CH = g.pair_chunks > 0 ? g.pair_chunks : (g.precision == RS_PRECISION_FP64 ? 8 : 16);
+  c->L.p.CH = g.pair_chunks > 0 ? g.pair_chunks : 16;  // lanes per molecule in the pair kernels
   if (c->L.p.CH != 8 && c->L.p.CH != 16) {
     rs_destroy(c);
     return fail(nullptr, RS_E_CONFIG, "pair_chunks must be 8 or 16");

@@ -149,18 +149,16 @@ struct P64 {
 
 // ------------------------------------------------------------------ the pair kernel
 
-// Occupancy targets (waves per SIMD), the same for 16 and 8 lanes per molecule. FP32: the forward loop gathers one
-// entry ahead: without the MLP term in 64 VGPRs, 8 waves per SIMD, so its 8,192 waves at 98k atoms
-// run in one round (2 % faster than 7 waves); with it in 72 VGPRs, 7 waves (at 8 it spills inside
-// the loops). The adjoint fits 128 VGPRs, 4 waves (prefetching there needs 154 VGPRs, 3 waves, and
-// ran 1 % slower). FP64: the forward at 128 VGPRs, 4 waves (a few spills outside the loops) ran 2 %
-// faster than at 136 VGPRs, 3 waves; the adjoint needs 238 VGPRs, 2 waves (at 3 waves it spills
-// ~150 VGPRs). The FP64 path runs 8 lanes per molecule by default: its adjoint's 4,096 waves at
-// 98k atoms then take two rounds of 2 per SIMD instead of four (adjoint -4 %, forward +2.5 %, the
-// step +1 %); the FP32 path keeps 16 (at 8 both of its kernels ran 3-4 % slower).
+// Occupancy targets (waves per SIMD), the same for 16 and 8 lanes per molecule. FP32: the forward
+// loop gathers one entry ahead: without the MLP term in 64 VGPRs, 8 waves per SIMD, so its 8,192
+// waves at 98k atoms run in one round (2 % faster than 7 waves); with it in 72 VGPRs, 7 waves (at 8
+// it spills inside the loops). The adjoint fits 128 VGPRs, 4 waves (at 5 it spills 140 B per
+// lane). FP64: the forward needs ~90 VGPRs (5 waves); the adjoint without the MLP term fits 168
+// VGPRs, 3 waves (7 % faster than 2 waves), with it 210 VGPRs, 2 waves (at 3 it spills ~150 B).
+// Both precisions run 16 lanes per molecule by default (at 8: FP32 1 % and FP64 2 % slower).
 template <class P, int CH, bool ADJ, bool FMLP>
 constexpr int pair_waves() {
-  return sizeof(typename P::R) == 4 ? (ADJ ? 4 : (FMLP ? 7 : 8)) : (ADJ ? 2 : 4);
+  return sizeof(typename P::R) == 4 ? (ADJ ? 4 : (FMLP ? 7 : 8)) : (ADJ ? (FMLP ? 2 : 3) : 4);
 }
 
 // Whether the pair loop branches over out-of-range atom pairs (measured per kernel).

@@ -183,12 +183,12 @@ def test_value_and_grad(gpu_lib, prec, tol, box216):
     assert rel_err(g, go) < tol, (g, go)
 
 
-@pytest.mark.parametrize("prec,chunks,tol", [(FP64, 16, 1e-10), (FP32, 8, 1e-5)])
-def test_non_default_lanes_per_molecule(gpu_lib, prec, chunks, tol, box216):
-    """The lane count each precision does not use by default (16 for FP32, 8 for FP64): with 8
-    lanes the forward kernel runs the integrator step on two waves per block (32 molecules; 216
-    waters leave the last block partly filled), with 16 on one. The forward trajectory and the
-    loss gradient against the oracle, to the default's tolerances."""
+@pytest.mark.parametrize("prec,chunks,tol", [(FP64, 8, 1e-10), (FP32, 8, 1e-5)])
+def test_eight_lanes_per_molecule(gpu_lib, prec, chunks, tol, box216):
+    """pair_chunks = 8 (the default is 16): the 8-lane pair kernels, whose forward kernel runs the
+    integrator step on two waves per block (32 molecules; 216 waters leave the last block partly
+    filled). The forward trajectory and the loss gradient against the oracle, to the 16-lane
+    tolerances."""
     import revsim
 
     sim, orc = _pair(prec, pair_chunks=chunks)
