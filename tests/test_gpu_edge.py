@@ -1,8 +1,9 @@
 """Edge cases of the GPU path against the FP64 oracle: a molecule that outruns the list's skin and
 an O-H bond beyond the assumed extent (the call is rerun with a wider skin or extent and the
-results equal the oracle's, which needs no list), a molecule count that fills no whole wave or
-block on the cell-list path, invalid step counts, and a close contact whose FP64 pair force leaves
-the fixed-point sums' range."""
+results equal the oracle's, which needs no list), a packing dense enough to overflow the list's
+capacity (rebuilt larger), a molecule count that fills no whole wave or block on the cell-list
+path, invalid step counts, and a close contact whose FP64 pair force leaves the fixed-point sums'
+range."""
 import numpy as np
 import pytest
 
@@ -57,6 +58,24 @@ def test_extent_rerun(gpu_lib, box3000, prec, tol):
     F, Fo = sim.forces(x), orc.forces(x)
     assert sim.list_radius() == pytest.approx(r0 + 0.02, abs=1e-12)
     assert per_atom_rel(F, Fo) < tol
+
+
+@pytest.mark.parametrize("prec", [FP64, FP32])
+def test_list_capacity_rebuild(gpu_lib, box3000, prec):
+    """The 3,000 waters packed into one octant of the box (each molecule translated so that its O
+    sits at half its coordinates: 8x the density, ~1,300 molecules within 1.06 nm of a central
+    one). The first build overflows the list's capacity (49 slots per lane and shell, 784 entries
+    per molecule and shell), the call rebuilds with twice the capacity, and the pair set equals
+    the oracle's."""
+    sim, orc = _pair(prec, 3000, 4.48, 1.0)
+    x = box3000.coords.reshape(-1, 3, 3)
+    o = np.mod(x[:, 0, :], 4.48)
+    xs = np.ascontiguousarray(np.mod(x + (0.5 * o - x[:, 0, :])[:, None, :], 4.48).reshape(-1))
+    got = sim.molecule_pairs(xs)
+    assert sim.stats()["reruns_call"] >= 1
+    want = orc.molecule_pairs(xs, sim.list_radius())
+    assert got.shape == want.shape
+    assert np.array_equal(got, want)
 
 
 @pytest.mark.parametrize("prec,tol,gtol", [(FP64, 1e-11, 1e-10), (FP32, 1e-5, 1e-5)])
