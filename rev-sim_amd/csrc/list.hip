@@ -270,11 +270,9 @@ __global__ __launch_bounds__(RS_BLOCK) void k_list_cells(SysParams p, const int*
                                                          typename ListEntry<L16>::type* nbr,
                                                          int4* ncnt, int* flags) {
   typedef typename ListEntry<L16>::type LT;
-  // quantized O positions as three ints (24 KB of LDS per block in all: six blocks per CU)
-  struct Q3 {
-    int x, y, z;
-  };
-  __shared__ Q3 sq[LC_CHUNK];
+  // quantized O positions, one array per axis, and the candidates' sorted indices (24 KB of LDS
+  // per block in all: six blocks per CU)
+  __shared__ int sqx[LC_CHUNK], sqy[LC_CHUNK], sqz[LC_CHUNK];
   __shared__ int sj[LC_CHUNK];
   __shared__ int rbeg[LC_MAXR], roff[LC_MAXR + 1];
   __shared__ int Ts[LIST_SHELLS][LC_MAXH];
@@ -354,7 +352,9 @@ __global__ __launch_bounds__(RS_BLOCK) void k_list_cells(SysParams p, const int*
         if (gi >= 0 && gi < nchunk) {
           sj[gi] = b + t;
           const int4 q = Oq[b + t];
-          sq[gi] = Q3{q.x, q.y, q.z};
+          sqx[gi] = q.x;
+          sqy[gi] = q.y;
+          sqz[gi] = q.z;
         }
       }
     }
@@ -367,43 +367,58 @@ __global__ __launch_bounds__(RS_BLOCK) void k_list_cells(SysParams p, const int*
       LT* out0 = nbr + list_base(p, g0, 0);
       const unsigned shell_words = (unsigned)p.capc * RS_WAVE;  // one shell of a wave's list
       unsigned T0 = Ts[0][m], T1 = Ts[1][m], T2 = Ts[2][m];
+      const bool cubic = p.rlo2q >= 0.0f;  // uniform: quantized shells and screen
 #pragma unroll 2
       for (int c0 = 0; c0 < nchunk; c0 += RS_WAVE) {
         const int t = c0 + lane;
         // read at a clamped index and mask, so the LDS reads do not wait on a branch
         const int tc = t < nchunk ? t : nchunk - 1;
         const int j = sj[tc];
-        const Q3 qj = sq[tc];
-        const bool valid = t < nchunk && j != i;
-        // FP32 screen in quantized units (cubic box: one unit for all axes); the FP64 rule
-        // decides inside the uncertainty band
-        const float r2q = r2f_of(qdiffq((uint32_t)qi.x, (uint32_t)qj.x),
-                                 qdiffq((uint32_t)qi.y, (uint32_t)qj.y),
-                                 qdiffq((uint32_t)qi.z, (uint32_t)qj.z));
-        bool in = valid && r2q < p.rlo2q;
-        // shells from the quantized distance (cubic box), else from the FP64 one, which a
-        // non-cubic box computes for every candidate (its band is unbounded)
-        bool s0 = r2q <= p.sh2q[0], s01 = r2q <= p.sh2q[1];
-        if (valid && !in && r2q <= p.rhi2q) {
-          const double4 oj = O64[j];
-          const double dx = mic1(oj.x - oi.x, p.L[0], p.invL[0]);
-          const double dy = mic1(oj.y - oi.y, p.L[1], p.invL[1]);
-          const double dz = mic1(oj.z - oi.z, p.L[2], p.invL[2]);
-          const double r2 = r2_of(dx, dy, dz);
-          in = r2 <= p.rlist2;
-          if (p.rlo2q < 0.0f) {
-            s0 = r2 <= p.sh2[0];
-            s01 = r2 <= p.sh2[1];
+        // FP32 screen in quantized units (cubic box: one unit for all axes): accept below rlo2q,
+        // reject above rhi2q, the FP64 rule decides in between (a non-cubic box has rlo2q < 0
+        // and rhi2q = inf: every candidate goes to the FP64 rule, shells included)
+        const float r2q = r2f_of(qdiffq((uint32_t)qi.x, (uint32_t)sqx[tc]),
+                                 qdiffq((uint32_t)qi.y, (uint32_t)sqy[tc]),
+                                 qdiffq((uint32_t)qi.z, (uint32_t)sqz[tc]));
+        const bool lt = t < nchunk, ne = j != i;
+        const bool ql = r2q < p.rlo2q, q0 = r2q <= p.sh2q[0], q01 = r2q <= p.sh2q[1];
+        // Wave masks, each the ballot of one compare (the compare writes it to a scalar pair
+        // directly); the per-lane flags are the compares themselves.
+        const unsigned long long vm = __ballot(lt) & __ballot(ne);
+        unsigned long long ba = vm & __ballot(ql);
+        unsigned long long m0 = __ballot(q0), m01 = __ballot(q01);
+        const unsigned long long bandm = vm & ~ba & __ballot(r2q <= p.rhi2q);
+        bool in = lt && ne && ql, s0 = q0, s01 = q01;
+        if (bandm) {  // uniform, rare in a cubic box: FP64 decisions patch masks and flags
+          bool in64 = false, s064 = false, s0164 = false;
+          if ((bandm >> lane) & 1ull) {
+            const double4 oj = O64[j];
+            const double dx = mic1(oj.x - oi.x, p.L[0], p.invL[0]);
+            const double dy = mic1(oj.y - oi.y, p.L[1], p.invL[1]);
+            const double dz = mic1(oj.z - oi.z, p.L[2], p.invL[2]);
+            const double r2 = r2_of(dx, dy, dz);
+            in64 = r2 <= p.rlist2;
+            s064 = r2 <= p.sh2[0];
+            s0164 = r2 <= p.sh2[1];
+          }
+          ba |= __ballot(in64);
+          in = in || in64;
+          if (!cubic) {
+            m0 = __ballot(s064);
+            m01 = __ballot(s0164);
+            s0 = s064;
+            s01 = s0164;
           }
         }
-        // one ballot per shell (the masks are wave-uniform); a lane's rank in its shell counts
-        // the mask's bits below it (mbcnt)
-        const unsigned long long ba = __ballot(in), b0 = __ballot(in && s0), b01 = __ballot(in && s01);
+        const unsigned long long b0 = ba & m0, b01 = ba & m01;
         const unsigned long long m1 = b01 & ~b0, m2 = ba & ~b01;
         if (in) {
-          const unsigned long long ms = s0 ? b0 : (s01 ? m1 : m2);
-          const unsigned g = (s0 ? T0 : (s01 ? T1 : T2)) +
-                             __builtin_amdgcn_mbcnt_hi((unsigned)(ms >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)ms, 0u));
+          // rank in the lane's shell: the shell mask's bits below the lane (mbcnt) plus the
+          // shell's entries so far; candidate g goes to sub-list g % CH, slot g / CH
+          const unsigned g0 = __builtin_amdgcn_mbcnt_hi((unsigned)(b0 >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)b0, T0));
+          const unsigned g1 = __builtin_amdgcn_mbcnt_hi((unsigned)(m1 >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m1, T1));
+          const unsigned g2 = __builtin_amdgcn_mbcnt_hi((unsigned)(m2 >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m2, T2));
+          const unsigned g = s0 ? g0 : (s01 ? g1 : g2);
           const unsigned slot = g / CH;
           const unsigned shoff = s0 ? 0u : (s01 ? shell_words : 2u * shell_words);
           if (slot < (unsigned)p.capc) out0[shoff + slot * RS_WAVE + (g % CH)] = (LT)j;
