@@ -171,8 +171,8 @@ def cpu_model() -> str:
 
 def adj_kernel(fp64: bool, pair_chunks: int, n_molecules: int) -> str:
     """Name of the adjoint pair kernel a trajectory runs: k_pair<precision, lanes per molecule,
-    adjoint, MLP compiled in, step epilogue, 16-bit list entries>. Lanes default to 16; the reverse
-    pass's lists have 16-bit entries up to 65,536 molecules."""
+    adjoint, MLP compiled in, step epilogue, 16-bit list entries>. Lanes default to 16; lists have
+    16-bit entries up to 65,536 molecules."""
     ch = pair_chunks or 16
     l16 = "true" if n_molecules <= 65536 else "false"
     return f"k_pair<{'P64' if fp64 else 'P32'}, {ch}, true, false, false, {l16}>"

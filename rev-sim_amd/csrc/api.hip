@@ -57,11 +57,6 @@ struct rs_ctx {
   double hext_extra = 0.0;  // added to the O-H extent after an extent violation
   int cell_cap = 0;         // allocated length of the per-cell arrays (>= ncell + 1)
   int cell_refine = 0;      // cells refined below r_list/2 after a home cell overflowed a build block
-  // entry width of the next list builds (rebuild() copies it into L.p.list16, which every list
-  // consumer reads, so a list is always read at the width it was built with): 16-bit in the reverse
-  // pass and adjoint evaluations where indices fit (half the adjoint kernel's list stream, its
-  // time unchanged), 32-bit elsewhere (the forward kernel ran 1.4 % slower on 16-bit entries)
-  int list16_next = 0;
   // list-driven reruns (capacity, stale list, O-H extent, cell overflow) and list builds of the
   // last trajectory or evaluation call, and reruns since the context was created
   int64_t reruns_call = 0, builds_call = 0, reruns_total = 0;
@@ -102,6 +97,9 @@ void set_params(rs_ctx* c) {
   const rs_config& g = c->cfg;
   p.M = c->M;
   p.N = c->N;
+  // list entries: 16-bit sorted indices wherever they fit (half the pair kernels' list stream and
+  // the build's stores, at unchanged kernel times), else 32-bit
+  p.list16 = c->M <= 65536 ? 1 : 0;
   for (int k = 0; k < 3; ++k) {
     p.L[k] = g.box[k];
     p.invL[k] = 1.0 / g.box[k];
@@ -458,7 +456,6 @@ void rebuild(rs_ctx* c, bool with_u, bool filed) {
   // (ensure_cells), and cells_fit() refuses a call that would launch past them
   if (c->L.p.ncell + 1 > c->cell_cap) return;
   ++c->builds_call;
-  c->L.p.list16 = c->list16_next;
   tbegin(c, RS_PHASE_LIST);
   launch_list_build(c->L, with_u, filed);
   tend(c);
@@ -555,7 +552,6 @@ int forward_impl(rs_ctx* c, const double* x0, const double* v0, uint64_t seed, i
   const bool baro = g.barostat_every > 0;
   if (baro && P.msd_on) return fail(c, RS_E_CONFIG, "the MSD loss needs a constant box (barostat_every = 0)");
   if (baro && c->cfg.use_mlp) return fail(c, RS_E_CONFIG, "the barostat needs the classical force field (use_mlp = 0)");
-  c->list16_next = 0;
   const double box0[3] = {g.box[0], g.box[1], g.box[2]};
   BarostatState bs;
   if (P.msd_on && (r = ensure_frames(c, P.nf))) return r;
@@ -701,7 +697,6 @@ int reverse_impl(rs_ctx* c, const double* xN, const double* vN, uint64_t seed, i
   const bool clip = g.clip_norm > 0.0 && std::isfinite(g.clip_norm);
   const int nslots = pair_blocks(c->L.p);
   if (c->cfg.use_mlp && !c->mlp_set) return fail(c, RS_E_STATE, "use_mlp is set but rs_set_mlp was not called");
-  c->list16_next = c->M <= 65536 ? 1 : 0;
   for (int attempt = 0;; ++attempt) {
     if (c->L.use_mlp)
       HIPCHK(c, hipMemsetAsync(b.mlp_slot, 0, sizeof(double) * (size_t)pair_blocks(c->L.p) * MLP_NG, c->s));
@@ -793,7 +788,6 @@ int single_eval(rs_ctx* c, const double* x, bool with_u, Body body) {
   DevBufs& b = c->L.b;
   int r = cells_fit(c);
   if (r) return r;
-  c->list16_next = with_u && c->M <= 65536 ? 1 : 0;  // the adjoint's width where it evaluates one
   for (int attempt = 0;; ++attempt) {
     HIPCHK(c, hipMemcpyAsync(b.x, x, sizeof(double) * c->N3, hipMemcpyHostToDevice, c->s));
     if ((r = reset_flags(c))) return r;

@@ -48,7 +48,7 @@ struct SysParams {
   float rlo2f, rhi2f;  // FP32 screen of the list rule: accept below, reject above, else FP64
   float rlo2q, rhi2q;  // the same band in quantized units (cubic boxes)
   int CH, capc;
-  // entries of the current molecule list: 16-bit words (at most 65,536 molecules) or 32-bit ints
+  // molecule list entries: 16-bit words (at most 65,536 molecules) or 32-bit ints
   int list16;
 };
 
