@@ -2,8 +2,9 @@
 blocks with truncation and snapshot restore (water/water.jl:1491-1547), the MSD loss injected at
 intermediate frames (diffusion/diffusion.jl:1534-1539), gradient clipping (water/water.jl:1510-1515),
 replay of caller-supplied noise, reversibility of the FP32 path (the reference's FP32 bound,
-water/water.jl:1613), and the 3,000-water box's trajectory gradient against the oracle and against
-central finite differences of the trajectory loss (FP64 path, smooth RDF window)."""
+water/water.jl:1613), bit-for-bit repeatability of the cell-list path's gradient, and the 3,000-water
+box's trajectory gradient against the oracle and against central finite differences of the
+trajectory loss (FP64 path, smooth RDF window)."""
 import numpy as np
 import pytest
 
@@ -113,6 +114,29 @@ def test_fp32_reversibility(gpu_lib, box3000):
     d -= 4.48 * np.round(d / 4.48)
     assert np.abs(d).max() < 1e-4
     assert abs(lr - lf) < 1e-7 * max(1.0, abs(lf))
+
+
+@pytest.mark.parametrize("prec", [FP64, FP32])
+def test_gradient_is_deterministic_3000_waters(gpu_lib, box3000, prec):
+    """The cell-list path reduces in fixed orders only (ballot-ranked list entries, DPP lane sums,
+    per-block θ-VJP slots summed in order; FP64 forces as integer sums): a 50 + 50-step gradient
+    with the RDF + MSD loss is the same bit for bit when run again in the same context and in a
+    fresh one."""
+    import revsim
+
+    def run():
+        cfg = revsim.make_config(3000, 4.48, 1.0, precision=prec, msd_window_steps=20, msd_every=5, w_msd=1e-3)
+        sim = revsim.Simulator(cfg, device=0)
+        sim.set_theta(revsim.DEFAULT_THETA)
+        sim.set_target_rdf(revsim.perturbed_target(sim.rdf(box3000.coords)[0]))
+        out = [sim.value_and_grad(box3000.coords, box3000.velocities, 17, 50) for _ in range(2)]
+        return out, sim.stats()["reruns_total"]
+
+    (a, b), reruns = run()
+    (c, _), _ = run()
+    assert reruns == 0
+    for L, g in (b, c):
+        assert L == a[0] and np.array_equal(g, a[1])
 
 
 def test_trajectory_gradient_3000_waters(gpu_lib, box3000):
