@@ -152,13 +152,14 @@ struct P64 {
 // Occupancy targets (waves per SIMD), the same for 16 and 8 lanes per molecule. FP32: the forward
 // loop gathers one entry ahead: without the MLP term in 64 VGPRs, 8 waves per SIMD, so its 8,192
 // waves at 98k atoms run in one round (2 % faster than 7 waves); with it in 72 VGPRs, 7 waves (at 8
-// it spills inside the loops). The adjoint fits 128 VGPRs, 4 waves (at 5 it spills 140 B per
-// lane). FP64: the forward needs ~90 VGPRs (5 waves); the adjoint without the MLP term fits 168
+// it spills inside the loops). The adjoint without the MLP term fits 96 VGPRs, 5 waves (20 B
+// spilled outside the loops; at 4 waves, 100 VGPRs and no spill, it ran 0.3 % slower), with it
+// 128 VGPRs, 4 waves. FP64: the forward needs ~90 VGPRs (5 waves); the adjoint without the MLP term fits 168
 // VGPRs, 3 waves (7 % faster than 2 waves), with it 210 VGPRs, 2 waves (at 3 it spills ~150 B).
 // Both precisions run 16 lanes per molecule by default (at 8: FP32 1 % and FP64 2 % slower).
 template <class P, int CH, bool ADJ, bool FMLP>
 constexpr int pair_waves() {
-  return sizeof(typename P::R) == 4 ? (ADJ ? 4 : (FMLP ? 7 : 8)) : (ADJ ? (FMLP ? 2 : 3) : 4);
+  return sizeof(typename P::R) == 4 ? (ADJ ? (FMLP ? 4 : 5) : (FMLP ? 7 : 8)) : (ADJ ? (FMLP ? 2 : 3) : 4);
 }
 
 // Whether the pair loop branches over out-of-range atom pairs (measured per kernel).
@@ -169,8 +170,7 @@ constexpr bool pair_skip() { return true; }
 template <class P, int CH, bool ADJ>
 constexpr bool pair_prefetch() { return sizeof(typename P::R) == 4 && !ADJ; }
 
-// FMLP: the forward variant's MLP term as a compile-time switch (the plain forward kernel then
-// holds none of its state); the adjoint variants follow the runtime flag.
+// FMLP: the MLP term as a compile-time switch (the plain kernels then hold none of its state).
 // STEP (forward only): the kernel ends with the integrator step of its molecules (St), so the
 // forward loop runs one kernel per step. The new sorted positions go to the other buffer
 // (St.posq_next / St.pos64_next): other blocks may still be reading this step's.
@@ -301,8 +301,7 @@ void k_pair(PairArgs A, StepArgs St) {
   __shared__ float2 stab[sizeof(R) == 4 ? MLP_NT : 1];
   R wO[3] = {0, 0, 0};
   double gscale = 0.0;
-  // (the FP32 adjoint follows the runtime flag; every other variant has it compiled in)
-  const bool mlp = ADJ && sizeof(R) == 4 ? (bool)A.mlp : FMLP;
+  const bool mlp = FMLP;
   if (mlp) {
     if (sizeof(R) == 4)
       for (int q = threadIdx.x; q < MLP_NT; q += RS_BLOCK) stab[q] = ((const float2*)A.mtab)[q];
@@ -649,8 +648,7 @@ void dispatch_ch(const Launch& L, const PairArgs& a, const StepArgs* st) {
       return;
     }
   }
-  constexpr bool F32ADJ = ADJ && sizeof(typename P::R) == 4;
-  if (a.mlp && !F32ADJ) k_pair<P, CH, ADJ, !F32ADJ, false, L16><<<g, RS_BLOCK, 0, L.s>>>(a, none);
+  if (a.mlp) k_pair<P, CH, ADJ, true, false, L16><<<g, RS_BLOCK, 0, L.s>>>(a, none);
   else k_pair<P, CH, ADJ, false, false, L16><<<g, RS_BLOCK, 0, L.s>>>(a, none);
 }
 
