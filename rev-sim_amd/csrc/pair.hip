@@ -152,14 +152,14 @@ struct P64 {
 // Occupancy targets (waves per SIMD), the same for 16 and 8 lanes per molecule. FP32: the forward
 // loop gathers one entry ahead: without the MLP term in 64 VGPRs, 8 waves per SIMD, so its 8,192
 // waves at 98k atoms run in one round (2 % faster than 7 waves); with it in 72 VGPRs, 7 waves (at 8
-// it spills inside the loops). The adjoint without the MLP term fits 96 VGPRs, 5 waves (20 B
-// spilled outside the loops; at 4 waves, 100 VGPRs and no spill, it ran 0.3 % slower), with it
-// 128 VGPRs, 4 waves. FP64: the forward needs ~90 VGPRs (5 waves); the adjoint without the MLP term fits 168
+// it spills inside the loops). The adjoint holds 100 VGPRs without the MLP term and 128 with
+// it, no spill, 4 waves (at 5 waves the plain one spills 20 B per lane outside the loops: 0.3 %
+// faster, 19 % more HBM traffic). FP64: the forward needs ~90 VGPRs (5 waves); the adjoint without the MLP term fits 168
 // VGPRs, 3 waves (7 % faster than 2 waves), with it 210 VGPRs, 2 waves (at 3 it spills ~150 B).
 // Both precisions run 16 lanes per molecule by default (at 8: FP32 1 % and FP64 2 % slower).
 template <class P, int CH, bool ADJ, bool FMLP>
 constexpr int pair_waves() {
-  return sizeof(typename P::R) == 4 ? (ADJ ? (FMLP ? 4 : 5) : (FMLP ? 7 : 8)) : (ADJ ? (FMLP ? 2 : 3) : 4);
+  return sizeof(typename P::R) == 4 ? (ADJ ? 4 : (FMLP ? 7 : 8)) : (ADJ ? (FMLP ? 2 : 3) : 4);
 }
 
 // Whether the pair loop branches over out-of-range atom pairs (measured per kernel).
