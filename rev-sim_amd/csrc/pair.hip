@@ -162,10 +162,6 @@ constexpr int pair_waves() {
   return sizeof(typename P::R) == 4 ? (ADJ ? 4 : (FMLP ? 7 : 8)) : (ADJ ? (FMLP ? 2 : 3) : 4);
 }
 
-// Whether the pair loop branches over out-of-range atom pairs (measured per kernel).
-template <class P, bool ADJ>
-constexpr bool pair_skip() { return true; }
-
 // Whether the pair loop gathers the next entry's inputs one iteration ahead.
 template <class P, int CH, bool ADJ>
 constexpr bool pair_prefetch() { return sizeof(typename P::R) == 4 && !ADJ; }
@@ -321,9 +317,7 @@ void k_pair(PairArgs A, StepArgs St) {
 
   const int4 ncnt = act ? A.ncnt[gtid] : make_int4(0, 0, 0, 0);
   bool fxbad = false;  // FP64 adjoint: a fixed-point term out of range (flagged after the loops)
-  // SKIP: out-of-range atom pairs are branched over rather than evaluated to zeros; the FP64 path
-  // then counts the terms each own atom added (nt) to remove their bias
-  constexpr bool SKIP = pair_skip<P, ADJ>();
+  // fixed-point terms each own atom added (FP64 path), whose bias is removed after the loops
   unsigned nt[3] = {0u, 0u, 0u};
   auto shell = [&](auto tag) {
     constexpr int S = decltype(tag)::value;
@@ -365,23 +359,16 @@ void k_pair(PairArgs A, StepArgs St) {
           R d[3];
           P::diff(p, own[a], pj[b], d[0], d[1], d[2]);
           const R r2 = P::r2(d[0], d[1], d[2]);
+          // An atom pair outside the cutoff contributes nothing: the lane branches over it, and a
+          // wave whose lanes all miss it skips it (the branch is taken for whole waves often
+          // enough, most of all in the outer shells, that this beats evaluating it to zeros).
           // r2 == 0 (coincident atoms) within the cutoff gives non-finite forces, which are flagged.
-          // Outside the cutoff 1/r and the reaction-field constant are zeroed, which zeroes every
-          // term below (all carry a power of 1/r) without further selects. The FP32 forward, whose
-          // pair terms are the force factors alone, zeroes those instead (one select per pair
-          // fewer; in the FP64 forward the same change ran 4 % slower).
-          constexpr bool MASK_F = !ADJ && sizeof(R) == 4;
-          const bool in = r2 <= rc2;
-          if constexpr (SKIP) {
-            // out of range: exact zeros; a wave whose lanes all miss this atom pair skips it
-            if (!in) continue;
-            if constexpr (FX) ++nt[a];
-          }
-          const R ri = (MASK_F || in) ? P::rinv(r2) : (R)0;
+          if (!(r2 <= rc2)) continue;
+          if constexpr (FX) ++nt[a];
+          const R ri = P::rinv(r2);
           const R ri2 = ri * ri, ri3 = ri2 * ri;
           const int ty = (a > 0) + (b > 0);
-          const R fr = MASK_F ? (in ? P::fma_(kA[ty], ri3, -kB[ty]) : (R)0)
-                              : P::fma_(kA[ty], ri3, in ? -kB[ty] : (R)0);
+          const R fr = P::fma_(kA[ty], ri3, -kB[ty]);
           R ftot = fr;  // f/r of all the pair's terms (FP64 path)
           if constexpr (!FX) {
 #pragma unroll
@@ -400,7 +387,7 @@ void k_pair(PairArgs A, StepArgs St) {
           if (S == 0 && a == 0 && b == 0) {
             const R s2 = sig2 * ri2, s6 = s2 * s2 * s2, s12 = s6 * s6;
             const R frl0 = (eps24 * ri2) * P::fma_((R)2, s12, -s6);
-            const R frl = (!MASK_F || in) ? frl0 : (R)0;
+            const R frl = frl0;
             if constexpr (FX) {
               ftot = add_nc(ftot, frl);
             } else {
@@ -421,7 +408,7 @@ void k_pair(PairArgs A, StepArgs St) {
               // f(r) = -u'(r) by cubic Hermite between table nodes k, k+1
               const R rr = (r2 * ri) * lu;
               R tk = (rr - mrlo) * minvh;
-              if (in && tk < (R)0) atomicOr(&A.flags[FLAG_MLP_RANGE], 1);
+              if (tk < (R)0) atomicOr(&A.flags[FLAG_MLP_RANGE], 1);
               tk = tk < (R)0 ? (R)0 : tk;
               int kk = (int)tk;
               kk = kk > MLP_NT - 2 ? MLP_NT - 2 : kk;
@@ -440,7 +427,7 @@ void k_pair(PairArgs A, StepArgs St) {
               hermite<R>(tt, hb, hdb);
               const R f = hb[0] * fk0 + hb[1] * mh * dk0 + hb[2] * fk1 + hb[3] * mh * dk1;
               const R fp = (hdb[0] * fk0 + hdb[1] * mh * dk0 + hdb[2] * fk1 + hdb[3] * mh * dk1) * minvh;
-              const R frm = (!MASK_F || in) ? f * ri : (R)0;
+              const R frm = f * ri;
               if constexpr (FX) {
                 ftot = add_nc(ftot, frm);
               } else {
@@ -455,13 +442,11 @@ void k_pair(PairArgs A, StepArgs St) {
                   Fml[t] -= frm * d[t];
                 }
                 // weight VJP: w_O . dF_O/d(table) = -(w_O . r̂) ∂f/∂(table)
-                if (in) {
-                  const double q = -(double)((wO[0] * d[0] + wO[1] * d[1] + wO[2] * d[2]) * ri) * gscale;
-                  atomicAdd((unsigned long long*)&lg[2 * kk], fix_rn(q * (double)hb[0]));
-                  atomicAdd((unsigned long long*)&lg[2 * kk + 1], fix_rn(q * (double)(mh * hb[1])));
-                  atomicAdd((unsigned long long*)&lg[2 * kk + 2], fix_rn(q * (double)hb[2]));
-                  atomicAdd((unsigned long long*)&lg[2 * kk + 3], fix_rn(q * (double)(mh * hb[3])));
-                }
+                const double q = -(double)((wO[0] * d[0] + wO[1] * d[1] + wO[2] * d[2]) * ri) * gscale;
+                atomicAdd((unsigned long long*)&lg[2 * kk], fix_rn(q * (double)hb[0]));
+                atomicAdd((unsigned long long*)&lg[2 * kk + 1], fix_rn(q * (double)(mh * hb[1])));
+                atomicAdd((unsigned long long*)&lg[2 * kk + 2], fix_rn(q * (double)hb[2]));
+                atomicAdd((unsigned long long*)&lg[2 * kk + 3], fix_rn(q * (double)(mh * hb[3])));
               }
             }
           }
@@ -491,10 +476,7 @@ void k_pair(PairArgs A, StepArgs St) {
   double Fd[9], hd[9], Fljd[3], Fsd[3], Fmld[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    // terms per own atom: shell 0 all three atoms of j, shell 1 no O-O, shell 2 H-H only
-    const unsigned long long nterm =
-        SKIP ? (unsigned long long)nt[a]
-             : 3ull * (unsigned)ncnt.x + (a == 0 ? 2ull : 3ull) * (unsigned)ncnt.y + (a == 0 ? 0ull : 2ull) * (unsigned)ncnt.z;
+    const unsigned long long nterm = nt[a];  // fixed-point terms this own atom added
 #pragma unroll
     for (int t = 0; t < 3; ++t) {
       if constexpr (FX) {
