@@ -270,10 +270,10 @@ __global__ __launch_bounds__(RS_BLOCK) void k_list_cells(SysParams p, const int*
                                                          typename ListEntry<L16>::type* nbr,
                                                          int4* ncnt, int* flags) {
   typedef typename ListEntry<L16>::type LT;
-  // quantized O positions, one array per axis, and the candidates' sorted indices (24 KB of LDS
-  // per block in all: six blocks per CU)
+  // quantized O positions, one array per axis, and the candidates' sorted indices (22 KB of LDS
+  // per block in all with 16-bit indices: seven blocks per CU; 24.6 KB and six with 32-bit ones)
   __shared__ int sqx[LC_CHUNK], sqy[LC_CHUNK], sqz[LC_CHUNK];
-  __shared__ int sj[LC_CHUNK];
+  __shared__ LT sj[LC_CHUNK];  // sorted indices, at the list's entry width
   __shared__ int rbeg[LC_MAXR], roff[LC_MAXR + 1];
   __shared__ int Ts[LIST_SHELLS][LC_MAXH];
   __shared__ int nr;
@@ -350,7 +350,7 @@ __global__ __launch_bounds__(RS_BLOCK) void k_list_cells(SysParams p, const int*
       for (int t = lane; t < len; t += RS_WAVE) {
         const int gi = o + t;
         if (gi >= 0 && gi < nchunk) {
-          sj[gi] = b + t;
+          sj[gi] = (LT)(b + t);
           const int4 q = Oq[b + t];
           sqx[gi] = q.x;
           sqy[gi] = q.y;
@@ -373,7 +373,7 @@ __global__ __launch_bounds__(RS_BLOCK) void k_list_cells(SysParams p, const int*
         const int t = c0 + lane;
         // read at a clamped index and mask, so the LDS reads do not wait on a branch
         const int tc = t < nchunk ? t : nchunk - 1;
-        const int j = sj[tc];
+        const int j = (int)sj[tc];
         // FP32 screen in quantized units (cubic box: one unit for all axes): accept below rlo2q,
         // reject above rhi2q, the FP64 rule decides in between (a non-cubic box has rlo2q < 0
         // and rhi2q = inf: every candidate goes to the FP64 rule, shells included)
