@@ -61,7 +61,7 @@ def test_trajectory_bit_exact(gpu_lib, n, box, rc, steps):
     assert np.array_equal(x, xo) and np.array_equal(v, vo), (np.abs(x - xo).max(), np.abs(v - vo).max())
 
 
-def test_config0_full_length(gpu_lib, box216):
+def test_216_waters_full_length(gpu_lib, box216):
     """216 waters, 1.862 nm, r_c 0.8 nm, 200 + 200 steps, RDF loss: the loss equals the oracle's
     bit for bit and dL/dθ to 1e-12 (was 4e-11 with FP64 sums in different orders)."""
     import revsim
@@ -77,7 +77,7 @@ def test_config0_full_length(gpu_lib, box216):
 
 
 @pytest.mark.timeout(900)
-def test_config1_full_length(gpu_lib, box3000):
+def test_3000_waters_full_length(gpu_lib, box3000):
     """3,000 waters, 4.48 nm, r_c 1.0 nm, 1,000 + 1,000 steps, RDF loss, FP64 path: loss bit-exact,
     dL/dθ within 1e-10 of the oracle at the full length, and the reconstructed start state too."""
     import revsim
@@ -93,7 +93,7 @@ def test_config1_full_length(gpu_lib, box3000):
 
 
 @pytest.mark.timeout(900)
-def test_config2_98k_rdf_msd(gpu_lib):
+def test_98k_atoms_rdf_msd(gpu_lib):
     """32,768 waters (98,304 atoms), 9.95 nm, r_c 1.0 nm, RDF + MSD (last 50 steps) loss over
     100 + 100 steps, FP64 path: loss bit-exact, dL/dθ within 1e-10; the per-term losses equal."""
     import revsim
@@ -131,7 +131,7 @@ def test_lists_beyond_16_bit_indices(gpu_lib):
 
 
 @pytest.mark.timeout(1200)
-def test_config2_98k_full_length(gpu_lib):
+def test_98k_atoms_full_length(gpu_lib):
     """The headline accuracy target: 98,304 atoms, RDF + MSD loss (last 500 fs), 1,000 + 1,000
     steps, FP64 path against the FP64 oracle — the loss bit-exact and dL/dθ within 1e-10, at the
     horizon where two FP64 runs with differently ordered sums would already disagree at O(1e-2)."""

@@ -100,7 +100,7 @@ _ORACLE_CACHE = {}
 
 @pytest.mark.timeout(1200)
 @pytest.mark.parametrize("prec,tol", [(FP64, 1e-8), (FP32, 5e-5)])
-def test_config4_98k_mlp_first_100_steps(gpu_lib, prec, tol):
+def test_98k_atoms_mlp_first_100_steps(gpu_lib, prec, tol):
     """Learned correction at scale: the 98,304-atom box with the MLP term and the RDF + MSD loss,
     against the FP64 oracle (exact per-pair network) on the first 100 steps. The FP64 path's table
     is within ~1e-11 of the network, which chaos grows ~100x over 100 steps; the FP32 path's

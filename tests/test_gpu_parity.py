@@ -240,8 +240,8 @@ def test_reversibility_fp64_98k_atoms(gpu_lib):
 
 
 @pytest.mark.timeout(600)
-def test_config2_98k_fp32_rdf_msd(gpu_lib):
-    """configs[2]'s FP32 path: 32,768 waters (98,304 atoms), 9.95 nm, r_c 1.0 nm, list every 10
+def test_98k_atoms_fp32_rdf_msd(gpu_lib):
+    """The FP32 path at 32,768 waters (98,304 atoms), 9.95 nm, r_c 1.0 nm, list every 10
     steps, RDF + MSD loss (MSD over the last 20 steps), 20 + 20 steps, against the oracle with the
     same FP32 pair-membership rule: loss and dL/dθ within 1e-5 (measured 3e-8 and 7e-8), the
     per-term losses within 1e-5, and no list rerun."""

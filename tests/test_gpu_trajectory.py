@@ -175,7 +175,7 @@ def test_trajectory_gradient_3000_waters(gpu_lib, box3000):
         assert abs(g[k] - fd) < 5e-3 * abs(fd), (k, g[k], fd)
 
 
-def test_config1_50_step_gradient_against_stored_trajectory_adjoint(gpu_lib, box3000):
+def test_3000_waters_50_step_gradient_against_stored_trajectory_adjoint(gpu_lib, box3000):
     """The 3,000-water check on a 50-step sub-trajectory: the FP64 path's reversible dL/dθ against
     the oracle's stored-trajectory adjoint, the counterpart of the reference's Enzyme direct
     differentiation (water/water.jl:1052-1099, compared at :1613-1621), with the default RDF
