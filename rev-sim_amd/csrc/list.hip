@@ -10,7 +10,8 @@
 // The list is full (both i->j and j->i) so each molecule's forces are summed by its own lanes:
 // no atomics, deterministic. CH lanes cooperate on one molecule; lane c owns sub-list c, built
 // here by the same lane mapping the pair kernel uses (gtid = i*CH + c), in ELL-by-wave layout so
-// that the pair kernel's list reads are 256-byte coalesced.
+// that each of the pair kernel's list reads is one coalesced 128-byte row (16-bit entries, at most
+// 65,536 molecules) or 256-byte row (32-bit).
 //
 // Each molecule's list is split into three shells by the O-O distance at the build: which atom
 // pairs of a listed molecule can come within r_c before the next build follows from it (O-O only

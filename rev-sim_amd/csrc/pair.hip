@@ -15,10 +15,10 @@
 // accumulates only i's side. The list's three distance shells are walked in turn, each in its own
 // loop with the atom pairs that can be within r_c there: all nine (shell 0, with the O-O
 // Lennard-Jones and MLP terms), the eight without O-O (shell 1), the four H-H (shell 2). A pair
-// left out would contribute exact zeros, so the sums are the same as over all nine. The CH
-// partial sums are combined by xor-shuffles in a fixed order, so the result is deterministic
-// without atomics. Lane c == 0 then adds them to the molecule's outputs in the caller's atom
-// order.
+// left out would contribute exact zeros, so the sums are the same as over all nine; within a loop,
+// atom pairs outside r_c are branched over. The CH partial sums are combined by DPP row steps in
+// a fixed order (chunk_sum_dpp, common.h), so the result is deterministic without atomics. Lane
+// c == 0 then adds them to the molecule's outputs in the caller's atom order.
 //
 // Pair arithmetic: FP32 on quantized positions (the int32 difference of 2^32-periodic positions is
 // the minimum image by construction) or FP64 (parity mode). The FP32 path computes in the quantized

@@ -519,13 +519,17 @@ def test_fixed_point_force_sums(n, box, rc):
     nterms = 2 * 3 * 4.0 / 3.0 * np.pi * (rc + 0.2) ** 3 * (3 * n / box ** 3)
     assert np.abs(F - Ff).max() <= nterms * 2.0 ** -34 + 1e-12 * np.abs(Ff).max()
     assert np.abs(F - Ff).max() > 0  # the two sums do differ
+    import os
+
+    from conftest import ROOT
+
+    paths = [os.path.join(ROOT, "oracle"), os.path.join(ROOT, "rev-sim_amd")]
     code = (
         "import sys, numpy as np; sys.path[:0] = %r\n"
         "import revsim, oracle as O\n"
         "o = O.Oracle(revsim.make_config(%d, %r, %r)); o.set_theta(revsim.DEFAULT_THETA)\n"
         "x = np.frombuffer(sys.stdin.buffer.read(), dtype=np.float64).copy()\n"
-        "sys.stdout.buffer.write(o.forces(x).tobytes())\n" % (sys.path[:4], n, box, rc))
-    import os
+        "sys.stdout.buffer.write(o.forces(x).tobytes())\n" % (paths, n, box, rc))
 
     out = {}
     for th in ("1", "3"):
